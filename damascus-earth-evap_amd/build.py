@@ -1,0 +1,41 @@
+"""Build libdmsun.so (gfx950) in-tree with hipcc.
+
+Usage: python3 build.py        (from any directory)
+
+Flags:
+  -ffp-contract=off   keeps every FP64 expression a separate IEEE operation on
+                      host and device, so kernel results match the CPU
+                      restatement bit for bit;
+  -packed-fp32-ops    disabled (no v_pk_* FP32 forms are emitted).
+"""
+import os
+import pathlib
+import subprocess
+import sys
+
+HERE = pathlib.Path(__file__).resolve().parent
+CSRC = HERE / "csrc"
+LIB = HERE / "libdmsun.so"
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc" if os.path.exists("/opt/rocm/bin/hipcc") else "hipcc")
+ARCH = "gfx950"
+
+COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wno-unused-result", "-Wno-unused-value",
+          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+
+def build(verbose=False):
+    cmd = [HIPCC, f"--offload-arch={ARCH}", *COMMON, "-shared", "-o", str(LIB) + ".tmp",
+           str(CSRC / "dms_runtime.hip"), str(CSRC / "dms_tables.cpp")]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("hipcc failed building libdmsun.so")
+    os.replace(str(LIB) + ".tmp", LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

@@ -1,0 +1,1510 @@
+/*
+ * Per-lane trajectory state machine of the FP64 kernel.
+ *
+ * The reference runs Trajectory_Simulator::Simulate (nested loops:
+ * Simulate -> Propagate_Freely -> RKF45 retries) on one CPU thread.  Here the
+ * loops are flattened into phases so that a wavefront executes one RKF45
+ * attempt per iteration of the persistent loop in every busy lane, and the
+ * rarer phases (scatter, segment start, trajectory end) diverge briefly:
+ *
+ *   PREP  top of the Propagate_Freely step loop (step caps, rate_before)
+ *   RK    one RKF45 attempt (inner retry loop unrolled across iterations)
+ *   POST  optical depth, scatter location, boundary / Kepler return, commit
+ *   SCAT  Scatter (target choice is done wave-cooperatively by the kernel)
+ *   SEG   Propagate_Freely entry (new free-flight segment)
+ *   FIN   post-loop classification of Simulate
+ *
+ * All arithmetic follows src/Simulation_Trajectory.cpp in the reference with
+ * the same operation order, so under -ffp-contract=off a lane reproduces the
+ * CPU restatement bit for bit.
+ */
+#ifndef DMS_ENGINE_H
+#define DMS_ENGINE_H
+
+#include "dms_physics.h"
+#include "philox.h"
+#include "../../include/dms.h"
+
+enum DmsPhase
+{
+	PH_DONE = 0,
+	PH_NEED = 1,
+	PH_START = 2,
+	PH_SEG = 3,
+	PH_PREP = 4,
+	PH_RK = 5,
+	PH_POST = 6,
+	PH_SCAT = 7,
+	PH_FIN = 8
+};
+
+/* Per-trajectory summary the host needs for run accounting. */
+struct DmsSummary
+{
+	int32_t reason;
+	uint32_t flags;
+	uint64_t n_scat;
+	double t_capture_sec;
+	double t_unbind_sec;
+	uint64_t n_rk;
+	uint32_t n_int, n_ext, n_rej, n_tv;
+};
+
+struct DmsJobView
+{
+	uint64_t max_time_steps, max_scatterings, rk_budget;
+	double max_distance;
+	int terminate_on_capture;
+	int nbins;
+	double bin_width_km;
+	uint32_t seed, point_id;
+	uint64_t first_id;
+};
+
+struct DmsProp
+{
+	double time, radius, phi, vr, J;
+	double ax[3], ay[3];
+	double ts;
+	int radial;
+};
+
+struct DmsLane
+{
+	int phase;
+	uint32_t slot;
+	dm_rng rng;
+	double ev_t, ev_r[3], ev_v[3];
+	uint64_t n_scat;
+	int reason;
+	/* capture / survival bookkeeping (TrajectoryBincount scalars) */
+	int captured, bound, event_observed, boundary_escape_observed, survival_valid, nie;
+	double t_capture, t_last_bound, t_unbind, t_boundary, r_first_neg, E_first_neg, dE_first_neg;
+	double drift_eV, drift_rel;
+	double prev_time_sec, prev_r_km, prev_v2, prev_cap_E, ff_ref_E;
+	/* bincount: running accumulator of the bin currently being filled */
+	int cur_bin;
+	double acc_dt, acc_v2;
+	uint32_t acc_cnt;
+	/* work counters */
+	uint64_t rk_all;
+	uint32_t n_int, n_ext, n_rej, n_tv;
+	/* Propagate_Freely segment */
+	double time_origin, mlx;
+	uint64_t time_steps;
+	uint32_t od_retries;
+	DmsProp prop;
+	/* step-level state carried from PREP to POST */
+	double pb_time, pb_radius, pb_phi, pb_vr, pb_ts;
+	double eb_t, eb_r[3], eb_v[3];
+	double r_before, v_before, rate_before;
+	int inner_iter, rk_ok;
+	/* scatter hand-off */
+	double sc_r, sc_v, sc_xi;
+	int sc_target;
+};
+
+/* Rows written per trajectory: dt[nbins], v2dt[nbins] (FP64), counts[nbins] (u32). */
+struct DmsRows
+{
+	double* rows;     /* n x 2 x nbins or NULL */
+	uint32_t* counts; /* n x nbins or NULL */
+};
+
+DMS_HD double dms_rk45_sanitized(double step)
+{
+	if(!dm_isfinite(step) || step <= 0.0)
+		return 0.1 * DMS_sec;
+	return dm_min(step, 1.0e6 * DMS_sec);
+}
+
+/* Free_Particle_Propagator constructor, src/Simulation_Trajectory.cpp:1403-1464 */
+DMS_HD void dms_prop_init(DmsProp& p, double t, const double* r, const double* v)
+{
+	p.time = t;
+	p.radius = dms_norm(r);
+	p.phi = 0.0;
+	p.radial = 0;
+	p.ts = 0.1 * DMS_sec;
+	p.J = 0.0;
+	const double speed = dms_norm(v);
+	const double radius_eps = 1.0e-12 * DMS_km;
+	const double angular_scale = dm_max(p.radius * speed, 1.0 * DMS_km * DMS_km / DMS_sec);
+	const double angular_eps = 1.0e-12 * angular_scale;
+	double L[3];
+	dms_cross(r, v, L);
+	const double Lnorm = dms_norm(L);
+	if(dm_isfinite(p.radius) && p.radius > radius_eps)
+	{
+		for(int c = 0; c < 3; c++)
+			p.ax[c] = r[c] / p.radius;
+		p.vr = dms_dot(r, v) / p.radius;
+	}
+	else if(dm_isfinite(speed) && speed > 0.0)
+	{
+		for(int c = 0; c < 3; c++)
+			p.ax[c] = v[c] / speed;
+		p.vr = speed;
+		p.radius = 0.0;
+		p.radial = 1;
+	}
+	else
+	{
+		p.ax[0] = 1.0;
+		p.ax[1] = 0.0;
+		p.ax[2] = 0.0;
+		p.vr = 0.0;
+		p.radius = 0.0;
+		p.radial = 1;
+	}
+	if(!p.radial && dm_isfinite(Lnorm) && Lnorm > angular_eps)
+	{
+		double az[3];
+		for(int c = 0; c < 3; c++)
+			az[c] = L[c] / Lnorm;
+		dms_cross(az, p.ax, p.ay);
+		const double ayn = dms_norm(p.ay);
+		if(dm_isfinite(ayn) && ayn > 0.0)
+		{
+			for(int c = 0; c < 3; c++)
+				p.ay[c] = p.ay[c] / ayn;
+			double t3[3];
+			dms_cross(p.ay, az, t3);
+			dms_normalize(t3, p.ax);
+			p.J = Lnorm;
+		}
+		else
+			p.radial = 1;
+	}
+	else
+		p.radial = 1;
+	if(p.radial)
+	{
+		/* Build_Stable_Orthonormal_Basis, :130-142 */
+		const double n = dms_norm(p.ax);
+		const double e1[3] = {p.ax[0] / n, p.ax[1] / n, p.ax[2] / n};
+		p.ax[0] = e1[0];
+		p.ax[1] = e1[1];
+		p.ax[2] = e1[2];
+		dms_frame(e1, p.ay, L); /* L reused for the (unused) third axis */
+		p.J = 0.0;
+	}
+}
+
+DMS_HD double dms_prop_radius(const DmsProp& p) { return (p.radius < 0.0) ? 0.0 : p.radius; }
+
+DMS_HD double dms_prop_speed(const DmsProp& p)
+{
+	const double r = (p.radius < 0.0) ? 0.0 : p.radius;
+	if(r == 0 || p.J == 0)
+		return dm_fabs(p.vr);
+	return dm_sqrt(p.vr * p.vr + p.J * p.J / r / r);
+}
+
+/* Event_In_3D, :1715-1742 */
+DMS_HD void dms_prop_event(const DmsProp& p, double& t, double* r, double* v)
+{
+	const double safe_radius = (dm_isfinite(p.radius) && p.radius > 0.0) ? p.radius : 0.0;
+	t = p.time;
+	if(p.radial || p.J == 0.0 || safe_radius == 0.0)
+	{
+		for(int c = 0; c < 3; c++)
+		{
+			r[c] = safe_radius * p.ax[c];
+			v[c] = p.vr * p.ax[c];
+		}
+		return;
+	}
+	const double cos_phi = dm_cos(p.phi);
+	const double sin_phi = dm_sin(p.phi);
+	const double tangential = p.J / safe_radius;
+	const double vx = p.vr * cos_phi - tangential * sin_phi;
+	const double vy = p.vr * sin_phi + tangential * cos_phi;
+	for(int c = 0; c < 3; c++)
+	{
+		r[c] = safe_radius * (cos_phi * p.ax[c] + sin_phi * p.ay[c]);
+		v[c] = vx * p.ax[c] + vy * p.ay[c];
+	}
+}
+
+DMS_HD double dms_dv_dt(const DmsProp& p, double r, double mass)
+{
+	if(r < 1.0 * DMS_km)
+		r = 1.0 * DMS_km;
+	return p.J * p.J / r / r / r - DMS_G_Newton * mass / r / r;
+}
+
+DMS_HD double dms_dphi_dt(const DmsProp& p, double r)
+{
+	if(p.radial || p.J == 0.0 || r <= 0.0)
+		return 0.0;
+	return p.J / r / r;
+}
+
+/* One attempt of Runge_Kutta_45_Step (:1487-1602).  Returns 1 accepted,
+ * 0 forced-accepted (retry cap or minimum step: the step "fails"), -1 retry. */
+DMS_HD int dms_rk45_attempt(DmsProp& p, const DmsSolarView& S, int inner_iter)
+{
+	const double tol0 = 1.0 * DMS_km, tol1 = 1.0e-3 * DMS_km / DMS_sec, tol2 = 1.0e-7;
+	const double h = p.ts;
+	double kr[6], kv[6], kp[6], ri;
+	kr[0] = h * p.vr;
+	kv[0] = h * dms_dv_dt(p, p.radius, dms_mass(S, p.radius));
+	kp[0] = h * dms_dphi_dt(p, p.radius);
+
+	ri = p.radius + kr[0] / 4.0;
+	kr[1] = h * (p.vr + kv[0] / 4.0);
+	kv[1] = h * dms_dv_dt(p, ri, dms_mass(S, ri));
+
+	ri = p.radius + 3.0 / 32.0 * kr[0] + 9.0 / 32.0 * kr[1];
+	kr[2] = h * (p.vr + 3.0 / 32.0 * kv[0] + 9.0 / 32.0 * kv[1]);
+	kv[2] = h * dms_dv_dt(p, ri, dms_mass(S, ri));
+	kp[2] = h * dms_dphi_dt(p, ri);
+
+	ri = p.radius + 1932.0 / 2197.0 * kr[0] - 7200.0 / 2197.0 * kr[1] + 7296.0 / 2197.0 * kr[2];
+	kr[3] = h * (p.vr + 1932.0 / 2197.0 * kv[0] - 7200.0 / 2197.0 * kv[1] + 7296.0 / 2197.0 * kv[2]);
+	kv[3] = h * dms_dv_dt(p, ri, dms_mass(S, ri));
+	kp[3] = h * dms_dphi_dt(p, ri);
+
+	ri = p.radius + 439.0 / 216.0 * kr[0] - 8.0 * kr[1] + 3680.0 / 513.0 * kr[2] - 845.0 / 4104.0 * kr[3];
+	kr[4] = h * (p.vr + 439.0 / 216.0 * kv[0] - 8.0 * kv[1] + 3680.0 / 513.0 * kv[2] - 845.0 / 4104.0 * kv[3]);
+	kv[4] = h * dms_dv_dt(p, ri, dms_mass(S, ri));
+	kp[4] = h * dms_dphi_dt(p, ri);
+
+	ri = p.radius - 8.0 / 27.0 * kr[0] + 2.0 * kr[1] - 3544.0 / 2565.0 * kr[2] + 1859.0 / 4104.0 * kr[3] - 11.0 / 40.0 * kr[4];
+	kr[5] = h * (p.vr - 8.0 / 27.0 * kv[0] + 2.0 * kv[1] - 3544.0 / 2565.0 * kv[2] + 1859.0 / 4104.0 * kv[3] - 11.0 / 40.0 * kv[4]);
+	kv[5] = h * dms_dv_dt(p, ri, dms_mass(S, ri));
+	kp[5] = h * dms_dphi_dt(p, ri);
+
+	const double r4 = p.radius + 25.0 / 216.0 * kr[0] + 1408.0 / 2565.0 * kr[2] + 2197.0 / 4104.0 * kr[3] - 1.0 / 5.0 * kr[4];
+	const double v4 = p.vr + 25.0 / 216.0 * kv[0] + 1408.0 / 2565.0 * kv[2] + 2197.0 / 4104.0 * kv[3] - 1.0 / 5.0 * kv[4];
+	const double p4 = p.phi + 25.0 / 216.0 * kp[0] + 1408.0 / 2565.0 * kp[2] + 2197.0 / 4104.0 * kp[3] - 1.0 / 5.0 * kp[4];
+	const double r5 = p.radius + 16.0 / 135.0 * kr[0] + 6656.0 / 12825.0 * kr[2] + 28561.0 / 56430.0 * kr[3] - 9.0 / 50.0 * kr[4] + 2.0 / 55.0 * kr[5];
+	const double v5 = p.vr + 16.0 / 135.0 * kv[0] + 6656.0 / 12825.0 * kv[2] + 28561.0 / 56430.0 * kv[3] - 9.0 / 50.0 * kv[4] + 2.0 / 55.0 * kv[5];
+	const double p5 = p.phi + 16.0 / 135.0 * kp[0] + 6656.0 / 12825.0 * kp[2] + 28561.0 / 56430.0 * kp[3] - 9.0 / 50.0 * kp[4] + 2.0 / 55.0 * kp[5];
+	const double e0 = dm_fabs(r5 - r4), e1 = dm_fabs(v5 - v4), e2 = dm_fabs(p5 - p4);
+	/* RK45_Next_Step_Size, :477-502 */
+	double h_new;
+	if(!dm_isfinite(e0) || !dm_isfinite(e1) || !dm_isfinite(e2))
+		h_new = DMS_RK45_MIN_STEP_FACTOR * h;
+	else
+	{
+		double factor = DMS_RK45_MAX_STEP_FACTOR;
+		const double ee[3] = {e0, e1, e2};
+		const double tt[3] = {tol0, tol1, tol2};
+		for(int i = 0; i < 3; i++)
+		{
+			if(ee[i] <= 0.0)
+				continue;
+			const double candidate = 0.84 * dm_pow_quarter(tt[i] / ee[i]);
+			if(dm_isfinite(candidate))
+				factor = dm_min(factor, candidate);
+		}
+		factor = dm_max(DMS_RK45_MIN_STEP_FACTOR, dm_min(factor, DMS_RK45_MAX_STEP_FACTOR));
+		h_new = dms_rk45_sanitized(factor * h);
+	}
+	/* RK45_Errors_Within_Tolerance, :466-475 */
+	const bool within = dm_isfinite(e0) && !(e0 > tol0) && dm_isfinite(e1) && !(e1 > tol1) && dm_isfinite(e2) && !(e2 > tol2);
+	if(within)
+	{
+		p.time = p.time + h;
+		p.radius = r4;
+		if(p.radius < 0.0)
+			p.radius = 0.0;
+		p.vr = v4;
+		p.phi = p4;
+		p.ts = h_new;
+		return 1;
+	}
+	const double abs_min_step = 1.0e-8 * DMS_sec;
+	if(h_new < abs_min_step || inner_iter >= DMS_RK45_MAX_INNER_RETRIES)
+	{
+		const bool state_ok = dm_isfinite(r4) && dm_isfinite(v4) && dm_isfinite(p4);
+		p.time = p.time + abs_min_step;
+		if(state_ok)
+		{
+			p.radius = r4;
+			if(p.radius < 0.0)
+				p.radius = 0.0;
+			p.vr = v4;
+			p.phi = p4;
+		}
+		p.ts = abs_min_step;
+		return 0;
+	}
+	p.ts = h_new;
+	return -1;
+}
+
+/* ---- helpers of Propagate_Freely, :53-464 ------------------------------------ */
+DMS_HD double dms_free_propagation_cap(double radius, double speed, double maximum_distance)
+{
+	double cap = 1.0e6 * DMS_sec;
+	const double safe_speed = dm_max(dm_fabs(speed), 1.0e-12 * DMS_km / DMS_sec);
+	const double crossing_scale = dm_max(0.25 * maximum_distance, 10.0 * DMS_km);
+	cap = dm_min(cap, crossing_scale / safe_speed);
+	const double safe_radius = dm_max(radius, 1.0 * DMS_km);
+	const double dynamical_time = dm_sqrt(safe_radius * safe_radius * safe_radius / (DMS_G_Newton * DMS_mSun));
+	if(dm_isfinite(dynamical_time) && dynamical_time > 0.0)
+		cap = dm_min(cap, 0.1 * dynamical_time);
+	return dm_max(cap, 1.0e-8 * DMS_sec);
+}
+
+DMS_HD bool dms_outward_escaping(const DmsSolarView& S, const double* r, const double* v, double boundary)
+{
+	const double radius = dms_norm(r);
+	if(radius < boundary)
+		return false;
+	const double vr = (radius > 0.0) ? dms_dot(r, v) / radius : 0.0;
+	return vr > 0.0 && dms_norm(v) > dms_local_escape_speed(S, radius);
+}
+
+/* Unit_Vector_At_Angle_From_Axis, :109-128 (false where the reference throws) */
+DMS_HD bool dms_unit_vector_at_angle(double cos_theta, double phi, const double* axis, double* out)
+{
+	if(!dm_isfinite(cos_theta) || cos_theta < -1.0 - 1.0e-12 || cos_theta > 1.0 + 1.0e-12 || !dm_isfinite(phi))
+		return false;
+	const double axis_norm = dms_norm(axis);
+	if(!dm_isfinite(axis_norm) || axis_norm <= 0.0)
+		return false;
+	const double e3[3] = {axis[0] / axis_norm, axis[1] / axis_norm, axis[2] / axis_norm};
+	double e1[3], e2[3];
+	dms_frame(e3, e1, e2);
+	cos_theta = dm_clamp_cos(cos_theta);
+	const double sin_theta = dm_sqrt(dm_max(0.0, 1.0 - cos_theta * cos_theta));
+	const double cp = dm_cos(phi), sp = dm_sin(phi);
+	for(int c = 0; c < 3; c++)
+		out[c] = cos_theta * e3[c] + sin_theta * cp * e1[c] + sin_theta * sp * e2[c];
+	return true;
+}
+
+/* Bound_Kepler_Return_At_Same_Radius, :152-201 (times are segment-local) */
+DMS_HD bool dms_bound_kepler_return(double ot, const double* orr, const double* ov, double& it, double* ir, double* iv)
+{
+	const double radius = dms_norm(orr);
+	const double speed = dms_norm(ov);
+	const double vr = (radius <= 0.0) ? 0.0 : dms_dot(orr, ov) / radius;
+	if(!dm_isfinite(radius) || !dm_isfinite(speed) || !dm_isfinite(vr) || radius <= DMS_rSun || speed <= 0.0 || vr <= 0.0)
+		return false;
+	const double mu = DMS_G_Newton * DMS_mSun;
+	const double specific_energy = 0.5 * speed * speed - mu / radius;
+	if(!dm_isfinite(specific_energy) || specific_energy >= 0.0)
+		return false;
+	double L[3];
+	dms_cross(orr, ov, L);
+	const double J = dms_norm(L);
+	if(!dm_isfinite(J))
+		return false;
+	const double a = -mu / (2.0 * specific_energy);
+	if(!dm_isfinite(a) || a <= 0.0)
+		return false;
+	double e2 = 1.0 + 2.0 * specific_energy * J * J / (mu * mu);
+	if(!dm_isfinite(e2) || e2 < 0.0)
+		return false;
+	e2 = dm_max(0.0, e2);
+	const double ecc = dm_sqrt(e2);
+	if(ecc <= 1.0e-14 || ecc >= 1.0)
+		return false;
+	const double cosE = dm_clamp_cos((1.0 - radius / a) / ecc);
+	const double E = dm_acos(cosE);
+	const double sinE = dm_sqrt(dm_max(0.0, 1.0 - cosE * cosE));
+	const double n = dm_sqrt(mu / (a * a * a));
+	if(!dm_isfinite(n) || n <= 0.0)
+		return false;
+	const double return_time = (2.0 * DM_PI - 2.0 * E + 2.0 * ecc * sinE) / n;
+	if(!dm_isfinite(return_time) || return_time <= 0.0)
+		return false;
+	it = ot + return_time;
+	for(int c = 0; c < 3; c++)
+	{
+		ir[c] = orr[c];
+		const double ru = orr[c] / radius;
+		iv[c] = ov[c] - 2.0 * vr * ru;
+	}
+	return it > ot && dm_isfinite(dms_norm(iv));
+}
+
+DMS_HD void dms_interpolate_event(double bt, const double* br, const double* bv, double at, const double* ar,
+                                  const double* av, double f, double& t, double* r, double* v)
+{
+	f = dm_clamp_unit(f);
+	for(int c = 0; c < 3; c++)
+	{
+		r[c] = br[c] + f * (ar[c] - br[c]);
+		v[c] = bv[c] + f * (av[c] - bv[c]);
+	}
+	t = bt + f * (at - bt);
+}
+
+DMS_HD double dms_radius_at_fraction(const double* br, const double* ar, double f)
+{
+	f = dm_clamp_unit(f);
+	double rs = 0.0;
+	for(int c = 0; c < 3; c++)
+	{
+		const double q = br[c] + f * (ar[c] - br[c]);
+		rs += q * q;
+	}
+	return dm_sqrt(dm_max(0.0, rs));
+}
+
+/* Solar_Interior_Fraction_Interval with Surface_Crossing_Fractions and
+ * Find_Surface_Crossing_Fraction, :231-325 */
+DMS_HD bool dms_interior_interval(const double* br, const double* ar, double r_before, double r_after, double& start, double& end)
+{
+	const bool before_inside = r_before < DMS_rSun;
+	const bool after_inside = r_after < DMS_rSun;
+	double first = 0.0, second = 0.0;
+	bool has = false;
+	{
+		double qa = 0.0, qb = 0.0, qc = -DMS_rSun * DMS_rSun;
+		for(int c = 0; c < 3; c++)
+		{
+			const double d = ar[c] - br[c];
+			qa += d * d;
+			qb += 2.0 * br[c] * d;
+			qc += br[c] * br[c];
+		}
+		if(qa > 0.0)
+		{
+			const double disc = qb * qb - 4.0 * qa * qc;
+			if(dm_isfinite(disc) && !(disc < 0.0))
+			{
+				const double sd = dm_sqrt(dm_max(0.0, disc));
+				first = (-qb - sd) / (2.0 * qa);
+				second = (-qb + sd) / (2.0 * qa);
+				if(first > second)
+				{
+					const double t = first;
+					first = second;
+					second = t;
+				}
+				has = true;
+			}
+		}
+	}
+	const double tol = 1.0e-12;
+	const bool f_in = first >= -tol && first <= 1.0 + tol;
+	const bool s_in = second >= -tol && second <= 1.0 + tol;
+	if(before_inside && after_inside)
+	{
+		start = 0.0;
+		end = 1.0;
+		return true;
+	}
+	bool need_bisection = false;
+	if(!before_inside && after_inside)
+	{
+		if(has && f_in)
+			start = dm_clamp_unit(first);
+		else if(has && s_in)
+			start = dm_clamp_unit(second);
+		else
+			need_bisection = true;
+		end = 1.0;
+	}
+	else if(before_inside && !after_inside)
+	{
+		start = 0.0;
+		if(has && s_in)
+			end = dm_clamp_unit(second);
+		else if(has && f_in)
+			end = dm_clamp_unit(first);
+		else
+			need_bisection = true;
+	}
+	else
+	{
+		if(has && f_in && s_in)
+		{
+			start = dm_clamp_unit(first);
+			end = dm_clamp_unit(second);
+			return end > start;
+		}
+		return false;
+	}
+	if(need_bisection)
+	{
+		double lower = 0.0, upper = 1.0;
+		double r_lower = dms_radius_at_fraction(br, ar, lower) - DMS_rSun;
+		for(int it = 0; it < 60; it++)
+		{
+			const double mid = 0.5 * (lower + upper);
+			const double r_mid = dms_radius_at_fraction(br, ar, mid) - DMS_rSun;
+			if((r_lower <= 0.0 && r_mid <= 0.0) || (r_lower >= 0.0 && r_mid >= 0.0))
+			{
+				lower = mid;
+				r_lower = r_mid;
+			}
+			else
+				upper = mid;
+		}
+		const double f = dm_clamp_unit(0.5 * (lower + upper));
+		if(!before_inside)
+			start = f;
+		else
+			end = f;
+	}
+	return end > start;
+}
+
+/* Scattering_Rate_At_Fraction, :327-345 */
+DMS_HD double dms_rate_at_fraction(const DmsPointView& P, const DmsSolarView& S, const double* br, const double* bv,
+                                   const double* ar, const double* av, double f)
+{
+	f = dm_clamp_unit(f);
+	double rs = 0.0, vs = 0.0;
+	for(int c = 0; c < 3; c++)
+	{
+		const double pos = br[c] + f * (ar[c] - br[c]);
+		const double vel = bv[c] + f * (av[c] - bv[c]);
+		rs += pos * pos;
+		vs += vel * vel;
+	}
+	const double radius = dm_sqrt(dm_max(0.0, rs));
+	if(radius >= DMS_rSun)
+		return 0.0;
+	return dms_rate_total(P, S, radius, dm_sqrt(dm_max(0.0, vs)));
+}
+
+/* Solve_Linear_Rate_Optical_Depth_Fraction, :416-464 */
+DMS_HD double dms_solve_linear_rate(double rs, double re, double dt, double target_tau)
+{
+	if(target_tau <= 0.0)
+		return 0.0;
+	if(!dm_isfinite(target_tau) || dt <= 0.0)
+		return 1.0;
+	if(!dm_isfinite(rs) || !dm_isfinite(re) || rs < 0.0 || re < 0.0)
+		return 1.0;
+	const double total = 0.5 * (rs + re) * dt;
+	if(!dm_isfinite(total) || total <= 0.0)
+		return 1.0;
+	if(target_tau >= total)
+		return 1.0;
+	const double target = target_tau / dt;
+	const double qa = 0.5 * (re - rs);
+	const double qb = rs;
+	if(dm_fabs(qa) < 1.0e-300)
+		return (qb > 0.0) ? dm_clamp_unit(target / qb) : 1.0;
+	const double disc = qb * qb + 4.0 * qa * target;
+	if(dm_isfinite(disc) && disc >= 0.0)
+	{
+		const double sd = dm_sqrt(disc);
+		const double r0 = (-qb + sd) / (2.0 * qa);
+		if(dm_isfinite(r0) && r0 >= 0.0 && r0 <= 1.0)
+			return dm_clamp_unit(r0);
+		const double r1 = (-qb - sd) / (2.0 * qa);
+		if(dm_isfinite(r1) && r1 >= 0.0 && r1 <= 1.0)
+			return dm_clamp_unit(r1);
+	}
+	double low = 0.0, high = 1.0;
+	for(int it = 0; it < 30; it++)
+	{
+		const double mid = 0.5 * (low + high);
+		const double tau_mid = dt * (rs * mid + 0.5 * (re - rs) * mid * mid);
+		if(tau_mid < target_tau)
+			low = mid;
+		else
+			high = mid;
+	}
+	return dm_clamp_unit(0.5 * (low + high));
+}
+
+/* ---- trajectory bookkeeping --------------------------------------------------- */
+DMS_HD double dms_capture_energy_eV(const DmsSolarView& S, double dm_mass, double radius, double speed)
+{
+	const double vesc = dms_local_escape_speed(S, radius);
+	const double E = 0.5 * dm_mass * (speed * speed - vesc * vesc);
+	return E / DMS_eV;
+}
+
+/* true where the reference's abort_if_uncaptured_bound lambda fires (:767-778) */
+DMS_HD bool dms_uncaptured_bound(const DmsLane& L, const DmsSolarView& S, double dm_mass, const double* r, const double* v)
+{
+	if(L.captured)
+		return false;
+	const double E = dms_capture_energy_eV(S, dm_mass, dms_norm(r), dms_norm(v));
+	return !(dm_isfinite(E) && E >= 0.0);
+}
+
+/* Update_Capture_State, :689-760 */
+DMS_HD int dms_update_capture_state(DmsLane& L, const DmsSolarView& S, double dm_mass, int terminate_on_capture,
+                                    double radius, double speed, double time, int allow_new)
+{
+	if(terminate_on_capture && !allow_new && !L.captured)
+		return 0;
+	const double E = dms_capture_energy_eV(S, dm_mass, radius, speed);
+	double dE = dm_nan();
+	if(dm_isfinite(L.prev_cap_E))
+		dE = E - L.prev_cap_E;
+	L.prev_cap_E = E;
+	const int negative = E < 0.0;
+	const double t_now = time / DMS_sec;
+	if(!allow_new)
+	{
+		if(L.captured && dm_isfinite(L.ff_ref_E))
+		{
+			const double drift = dm_fabs(E - L.ff_ref_E);
+			if(dm_isfinite(drift) && drift > L.drift_eV)
+				L.drift_eV = drift;
+			const double scale = dm_max(dm_fabs(L.ff_ref_E), DMS_FREE_ENERGY_DRIFT_REL_E_SCALE_EV);
+			const double rel = drift / scale;
+			if(dm_isfinite(rel) && rel > L.drift_rel)
+				L.drift_rel = rel;
+		}
+		if(L.bound && L.captured)
+			L.t_last_bound = t_now;
+		return 0;
+	}
+	const int was_bound = L.bound;
+	L.bound = negative;
+	L.ff_ref_E = E;
+	if(negative)
+	{
+		if(!L.captured)
+		{
+			L.captured = 1;
+			L.t_capture = t_now;
+			L.t_last_bound = t_now;
+			L.r_first_neg = radius / DMS_km;
+			L.E_first_neg = E;
+			L.dE_first_neg = dE;
+		}
+		else if(!was_bound)
+			L.t_unbind = dm_nan();
+		L.t_last_bound = t_now;
+		return 1;
+	}
+	if(was_bound && L.captured)
+		L.t_unbind = t_now;
+	return 0;
+}
+
+/* Accumulate_Bincount_Step (:640-669) into the trajectory's row, keeping the
+ * bin being filled in registers; the per-bin addition sequence is unchanged. */
+DMS_HD void dms_bin_flush(DmsLane& L, const DmsRows& R, int nbins)
+{
+	if(L.cur_bin < 0)
+		return;
+	if(R.rows)
+	{
+		double* row = R.rows + (size_t) L.slot * 2 * (size_t) nbins;
+		row[L.cur_bin] = L.acc_dt;
+		row[nbins + L.cur_bin] = L.acc_v2;
+	}
+	if(R.counts)
+		R.counts[(size_t) L.slot * (size_t) nbins + L.cur_bin] = L.acc_cnt;
+	L.cur_bin = -1;
+}
+
+DMS_HD void dms_accumulate_bincount(DmsLane& L, const DmsRows& R, const DmsJobView& J, double r_km, double v2, double dt_sec)
+{
+	if(!dm_isfinite(dt_sec) || dt_sec <= 0.0)
+		return;
+	const double bin_max = 2.0 * DMS_R_SUN_KM;
+	int bin = -1;
+	if(dm_isfinite(r_km) && r_km >= 0.0 && r_km < bin_max)
+	{
+		bin = (int) (r_km / J.bin_width_km);
+		if(bin < 0)
+			bin = 0;
+		if(bin >= J.nbins)
+			bin = -1;
+	}
+	if(bin < 0)
+		return;
+	if(bin != L.cur_bin)
+	{
+		dms_bin_flush(L, R, J.nbins);
+		L.cur_bin = bin;
+		if(R.rows)
+		{
+			const double* row = R.rows + (size_t) L.slot * 2 * (size_t) J.nbins;
+			L.acc_dt = row[bin];
+			L.acc_v2 = row[J.nbins + bin];
+		}
+		else
+		{
+			L.acc_dt = 0.0;
+			L.acc_v2 = 0.0;
+		}
+		L.acc_cnt = R.counts ? R.counts[(size_t) L.slot * (size_t) J.nbins + bin] : 0u;
+	}
+	const double v2dt = v2 * dt_sec;
+	L.acc_dt += dt_sec;
+	L.acc_v2 += v2dt;
+	L.acc_cnt += 1u;
+}
+
+/* commit_accepted_event lambda, :842-874 (segment-local event) */
+DMS_HD int dms_commit(DmsLane& L, const DmsRows& R, const DmsJobView& J, const DmsSolarView& S, double dm_mass,
+                      double lt, const double* lr, const double* lv)
+{
+	const double abs_t = lt + L.time_origin;
+	const double t_now = lt / DMS_sec;
+	if(!J.terminate_on_capture)
+	{
+		const double r_now = dms_norm(lr) / DMS_km;
+		const double v_now = dms_norm(lv) / (DMS_km / DMS_sec);
+		const double v2 = v_now * v_now;
+		if(L.prev_time_sec >= 0.0)
+		{
+			const double dts = t_now - L.prev_time_sec;
+			dms_accumulate_bincount(L, R, J, L.prev_r_km, L.prev_v2, dts);
+		}
+		L.prev_time_sec = t_now;
+		L.prev_r_km = r_now;
+		L.prev_v2 = v2;
+	}
+	return dms_update_capture_state(L, S, dm_mass, J.terminate_on_capture, dms_norm(lr), dms_norm(lv), abs_t, 0);
+}
+
+/* Propagate_Freely entry (:764-805).  Returns -1 to continue with PREP or the
+ * termination reason the reference returns before the step loop. */
+DMS_HD int dms_segment_start(DmsLane& L, const DmsSolarView& S, double dm_mass, const DmsJobView& J)
+{
+	if(dms_outward_escaping(S, L.ev_r, L.ev_v, J.max_distance))
+		return DMS_OUTWARD_ESCAPE;
+	if(dms_uncaptured_bound(L, S, dm_mass, L.ev_r, L.ev_v))
+		return DMS_NUMERICAL_FAILURE;
+	L.time_origin = L.ev_t;
+	dms_prop_init(L.prop, 0.0, L.ev_r, L.ev_v);
+	L.mlx = -dm_log(dm_max(dm_uniform(&L.rng), DM_DBL_MIN));
+	L.time_steps = 0;
+	L.od_retries = 0;
+	if(!J.terminate_on_capture && L.prev_time_sec >= 0.0)
+		L.prev_time_sec = 0.0;
+	return -1;
+}
+
+/* Top of the step loop (:876-918).  Returns -1 when an RKF45 step follows,
+ * otherwise the termination reason. */
+DMS_HD int dms_step_prep(DmsLane& L, const DmsPointView& P, const DmsSolarView& S, const DmsJobView& J)
+{
+	if(!(L.time_steps < J.max_time_steps))
+		return DMS_MAX_FREE_STEPS;
+	if(J.rk_budget > 0 && L.rk_all > J.rk_budget)
+	{
+		double t;
+		dms_prop_event(L.prop, t, L.ev_r, L.ev_v);
+		L.ev_t = t + L.time_origin;
+		return DMS_WALL_TIME_LIMIT;
+	}
+	dms_prop_event(L.prop, L.eb_t, L.eb_r, L.eb_v);
+	L.r_before = dms_prop_radius(L.prop);
+	L.v_before = dms_prop_speed(L.prop);
+	L.rate_before = 0.0;
+	if(L.r_before >= DMS_rSun)
+	{
+		const double cap = dms_free_propagation_cap(L.r_before, L.v_before, J.max_distance);
+		L.prop.ts = dm_min(dms_rk45_sanitized(L.prop.ts), cap);
+		const double rvb = (L.r_before > 0.0) ? dms_dot(L.eb_r, L.eb_v) / L.r_before : 0.0;
+		if(L.r_before > DMS_rSun && rvb < 0.0)
+		{
+			const double est = (L.r_before - DMS_rSun) / (-rvb);
+			if(dm_isfinite(est) && est > 0.0)
+				L.prop.ts = dm_min(L.prop.ts, dm_max(est, 1.0e-8 * DMS_sec));
+		}
+	}
+	else
+	{
+		L.prop.ts = dms_rk45_sanitized(L.prop.ts);
+		L.rate_before = dms_rate_total(P, S, L.r_before, L.v_before);
+		if(!dm_isfinite(L.rate_before) || L.rate_before < 0.0)
+		{
+			double t;
+			dms_prop_event(L.prop, t, L.ev_r, L.ev_v);
+			L.ev_t = t + L.time_origin;
+			return DMS_NUMERICAL_FAILURE;
+		}
+		const double od_limit = J.terminate_on_capture ? DMS_CAPTURE_MODE_MAX_OPTICAL_DEPTH_STEP : DMS_MAX_OPTICAL_DEPTH_STEP;
+		if(L.rate_before > 0.0)
+			L.prop.ts = dm_min(L.prop.ts, od_limit / L.rate_before);
+	}
+	L.pb_time = L.prop.time;
+	L.pb_radius = L.prop.radius;
+	L.pb_phi = L.prop.phi;
+	L.pb_vr = L.prop.vr;
+	L.pb_ts = L.prop.ts;
+	/* Runge_Kutta_45_Step entry */
+	L.prop.ts = dms_rk45_sanitized(L.prop.ts);
+	L.inner_iter = 0;
+	return -1;
+}
+
+/* Loop body after the RKF45 step (:921-1115).  Returns -2 to run PREP again,
+ * -3 when the segment ended in a scattering (Scatter follows), or the
+ * Propagate_Freely return value for every other exit. */
+DMS_HD int dms_step_post(DmsLane& L, const DmsPointView& P, const DmsSolarView& S, const DmsJobView& J, const DmsRows& R)
+{
+	const double actual_dt = L.prop.time - L.pb_time;
+	const double r_after = dms_prop_radius(L.prop);
+	const double v_after = dms_prop_speed(L.prop);
+	double ea_t, ea_r[3], ea_v[3];
+	dms_prop_event(L.prop, ea_t, ea_r, ea_v);
+#define DMS_RETURN_AT(tt, rr, vv, code) \
+	do                                  \
+	{                                   \
+		L.ev_t = (tt) + L.time_origin;  \
+		for(int c_ = 0; c_ < 3; c_++)   \
+		{                               \
+			L.ev_r[c_] = (rr)[c_];      \
+			L.ev_v[c_] = (vv)[c_];      \
+		}                               \
+		return (code);                  \
+	} while(0)
+	if(!L.rk_ok)
+		DMS_RETURN_AT(ea_t, ea_r, ea_v, DMS_NUMERICAL_FAILURE);
+	if(!dm_isfinite(r_after) || !dm_isfinite(v_after) || !dm_isfinite(actual_dt) || actual_dt <= 0.0)
+		DMS_RETURN_AT(ea_t, ea_r, ea_v, DMS_NON_FINITE_STATE);
+	if(v_after > DMS_V_MAX)
+		DMS_RETURN_AT(ea_t, ea_r, ea_v, DMS_SPEED_LIMIT);
+	if(J.rk_budget > 0 && L.rk_all > J.rk_budget)
+		DMS_RETURN_AT(ea_t, ea_r, ea_v, DMS_WALL_TIME_LIMIT);
+	const int fast = J.terminate_on_capture;
+	const double od_limit = fast ? DMS_CAPTURE_MODE_MAX_OPTICAL_DEPTH_STEP : DMS_MAX_OPTICAL_DEPTH_STEP;
+	const int od_count = fast ? DMS_CAPTURE_MODE_OPTICAL_DEPTH_PIECES : DMS_MAX_OPTICAL_DEPTH_PIECES;
+	int scattering = 0, reflection = 0;
+	double acc_t = ea_t, acc_r[3] = {ea_r[0], ea_r[1], ea_r[2]}, acc_v[3] = {ea_v[0], ea_v[1], ea_v[2]};
+	double is = 0.0, ie = 0.0;
+	if(dms_interior_interval(L.eb_r, ea_r, L.r_before, r_after, is, ie))
+	{
+		L.n_int++;
+		if(r_after < DMS_rSun && v_after < 0.0)
+			return DMS_MAX_FREE_STEPS; /* unreachable 'break' of the reference, :964-968 */
+		/* Build_Optical_Depth_Pieces, :347-414 */
+		double fr[DMS_MAX_OPTICAL_DEPTH_PIECES + 1], rt[DMS_MAX_OPTICAL_DEPTH_PIECES + 1];
+		const bool use_cached = L.r_before < DMS_rSun && is <= 1.0e-12;
+		double tau2 = 0.0, dtau = 0.0;
+		int invalid = 0, pc = 0;
+		if(ie > is && actual_dt > 0.0)
+		{
+			const double span = ie - is;
+			for(int i = 0; i <= od_count; i++)
+				fr[i] = is + (double) i / (double) od_count * span;
+			for(int i = 0; i <= od_count; i++)
+			{
+				if(i == 0 && use_cached)
+					rt[i] = L.rate_before;
+				else
+					rt[i] = dms_rate_at_fraction(P, S, L.eb_r, L.eb_v, ea_r, ea_v, fr[i]);
+				if(!dm_isfinite(rt[i]) || rt[i] < 0.0)
+				{
+					invalid = 1;
+					break;
+				}
+			}
+			if(!invalid)
+			{
+#define DMS_TRAP(s, e) (0.5 * (rt[s] + rt[e]) * (actual_dt * (fr[e] - fr[s])))
+				if(od_count >= 4)
+					tau2 = DMS_TRAP(0, 2) + DMS_TRAP(2, 4);
+				else
+					for(int i = 0; i < od_count; i++)
+						tau2 += DMS_TRAP(i, i + 1);
+				for(int i = 0; i < od_count; i++)
+					if(fr[i + 1] > fr[i])
+					{
+						pc++;
+						dtau += DMS_TRAP(i, i + 1);
+					}
+#undef DMS_TRAP
+			}
+		}
+		if(invalid || !dm_isfinite(dtau) || dtau < 0.0)
+			DMS_RETURN_AT(ea_t, ea_r, ea_v, DMS_NUMERICAL_FAILURE);
+		const double scale = dm_max(dtau, DMS_OPTICAL_DEPTH_ABSOLUTE_TOLERANCE);
+		const double relerr = dm_fabs(dtau - tau2) / scale;
+		const bool rej_od = dtau > od_limit;
+		const bool rej_acc = !fast && relerr > DMS_OPTICAL_DEPTH_RELATIVE_TOLERANCE;
+		if(rej_od || rej_acc)
+		{
+			L.n_rej++;
+			L.od_retries++;
+			if(L.od_retries > DMS_MAX_OPTICAL_DEPTH_RETRIES)
+				DMS_RETURN_AT(L.eb_t, L.eb_r, L.eb_v, DMS_NUMERICAL_FAILURE);
+			L.prop.time = L.pb_time;
+			L.prop.radius = L.pb_radius;
+			L.prop.phi = L.pb_phi;
+			L.prop.vr = L.pb_vr;
+			L.prop.ts = L.pb_ts;
+			double factor = DMS_RK45_MAX_STEP_FACTOR;
+			if(dtau > od_limit && dtau > 0.0)
+				factor = dm_min(factor, 0.8 * od_limit / dtau);
+			if(rej_acc && relerr > 0.0)
+				factor = dm_min(factor, 0.8 * dm_sqrt(DMS_OPTICAL_DEPTH_RELATIVE_TOLERANCE / relerr));
+			factor = dm_max(DMS_RK45_MIN_STEP_FACTOR, dm_min(factor, 1.0));
+			L.prop.ts = dms_rk45_sanitized(factor * actual_dt);
+			return -2;
+		}
+		if(dtau >= L.mlx)
+		{
+			double remaining = L.mlx;
+			int k = 0;
+			for(int i = 0; i < od_count; i++)
+			{
+				if(!(fr[i + 1] > fr[i]))
+					continue;
+				const double ps = fr[i], pe = fr[i + 1];
+				const double piece_dt = actual_dt * (pe - ps);
+				const double piece_tau = 0.5 * (rt[i] + rt[i + 1]) * piece_dt;
+				k++;
+				if(piece_tau >= remaining)
+				{
+					const double lf = dms_solve_linear_rate(rt[i], rt[i + 1], piece_dt, remaining);
+					const double fraction = ps + lf * (pe - ps);
+					dms_interpolate_event(L.eb_t, L.eb_r, L.eb_v, ea_t, ea_r, ea_v, fraction, acc_t, acc_r, acc_v);
+					scattering = 1;
+					break;
+				}
+				remaining -= piece_tau;
+			}
+			(void) k;
+			if(!scattering)
+			{
+				dms_interpolate_event(L.eb_t, L.eb_r, L.eb_v, ea_t, ea_r, ea_v, ie, acc_t, acc_r, acc_v);
+				scattering = 1;
+			}
+		}
+		else
+			L.mlx -= dtau;
+	}
+	else
+		L.n_ext++;
+	if(!scattering)
+	{
+		double bt = ea_t, br[3] = {ea_r[0], ea_r[1], ea_r[2]}, bv[3] = {ea_v[0], ea_v[1], ea_v[2]};
+		double rb = r_after, vb = v_after;
+		if(L.r_before < J.max_distance && r_after >= J.max_distance && r_after > L.r_before)
+		{
+			const double bf = (J.max_distance - L.r_before) / (r_after - L.r_before);
+			dms_interpolate_event(L.eb_t, L.eb_r, L.eb_v, ea_t, ea_r, ea_v, bf, bt, br, bv);
+			rb = dms_norm(br);
+			vb = dms_norm(bv);
+		}
+		if(rb >= J.max_distance && rb >= L.r_before && vb > dms_local_escape_speed(S, rb))
+		{
+			acc_t = bt;
+			for(int c = 0; c < 3; c++)
+			{
+				acc_r[c] = br[c];
+				acc_v[c] = bv[c];
+			}
+			reflection = 1;
+		}
+		else if(rb >= J.max_distance && rb >= L.r_before)
+		{
+			if(dms_uncaptured_bound(L, S, P.mass, br, bv))
+				DMS_RETURN_AT(bt, br, bv, DMS_NUMERICAL_FAILURE);
+			double it, ir[3], iv[3];
+			if(dms_bound_kepler_return(bt, br, bv, it, ir, iv))
+			{
+				L.time_steps++;
+				L.od_retries = 0;
+				dms_commit(L, R, J, S, P.mass, bt, br, bv);
+				L.ev_t = it + L.time_origin;
+				for(int c = 0; c < 3; c++)
+				{
+					L.ev_r[c] = ir[c];
+					L.ev_v[c] = iv[c];
+				}
+				/* reset_propagator_at_absolute_event, :814-822 */
+				L.time_origin = L.ev_t;
+				dms_prop_init(L.prop, 0.0, L.ev_r, L.ev_v);
+				if(!J.terminate_on_capture && L.prev_time_sec >= 0.0)
+					L.prev_time_sec = 0.0;
+				return -2;
+			}
+		}
+	}
+	L.time_steps++;
+	L.od_retries = 0;
+	const int captured_now = dms_commit(L, R, J, S, P.mass, acc_t, acc_r, acc_v);
+	L.ev_t = acc_t + L.time_origin;
+	for(int c = 0; c < 3; c++)
+	{
+		L.ev_r[c] = acc_r[c];
+		L.ev_v[c] = acc_v[c];
+	}
+	if(dms_uncaptured_bound(L, S, P.mass, L.ev_r, L.ev_v))
+		return DMS_NUMERICAL_FAILURE;
+	if(J.terminate_on_capture && captured_now)
+		return DMS_CAPTURE_MODE;
+	if(reflection)
+		return DMS_OUTWARD_ESCAPE;
+	if(scattering)
+		return -3;
+	return -2;
+#undef DMS_RETURN_AT
+}
+
+/* Sample_Target_Velocity (Romano & Walsh), :1160-1226; false where it throws */
+DMS_HD bool dms_sample_target_velocity(DmsLane& L, double temperature, double target_mass, const double* vel, double* out)
+{
+	if(!dm_isfinite(temperature) || temperature <= 0.0 || !dm_isfinite(target_mass) || target_mass <= 0.0)
+		return false;
+	const double kappa = dm_sqrt(target_mass / 2.0 / temperature);
+	const double vDM = dms_norm(vel);
+	if(!dm_isfinite(kappa) || kappa <= 0.0 || !dm_isfinite(vDM) || vDM <= 0.0)
+		return false;
+	const double y = kappa * vDM;
+	double x = y, mu = 1.0;
+	unsigned long attempts = 0;
+	for(;;)
+	{
+		L.n_tv++;
+		const double xs = x * x, ys = y * y;
+		const double raw = xs + ys - 2.0 * x * y * mu;
+		const double den = x + y;
+		const double roundoff = dm_max(1.0, xs + ys + 2.0 * dm_fabs(x * y));
+		if(!dm_isfinite(x) || x < 0.0 || !dm_isfinite(y) || y <= 0.0 || !dm_isfinite(mu) || mu < -1.0 || mu > 1.0
+		   || !dm_isfinite(raw) || raw < -1.0e-12 * roundoff || !dm_isfinite(den) || den <= 0.0)
+			return false;
+		const double ratio = dm_sqrt(dm_max(0.0, raw)) / den;
+		if(!dm_isfinite(ratio) || ratio < 0.0 || ratio > 1.0 + 1.0e-12)
+			return false;
+		const double accept = dm_min(1.0, ratio);
+		if(!(accept < dm_uniform(&L.rng)))
+			break;
+		attempts++;
+		if(attempts > DMS_TARGET_VELOCITY_MAX_REJECTION_ATTEMPTS)
+			return false;
+		mu = -1.0 + 2.0 * dm_uniform(&L.rng);
+		const double xi_1 = dm_uniform(&L.rng);
+		if(xi_1 < 2.0 / (DM_SQRT_PI * y + 2.0))
+		{
+			const double xi_2 = dm_max(dm_uniform(&L.rng), DM_DBL_MIN);
+			const double xi_3 = dm_max(dm_uniform(&L.rng), DM_DBL_MIN);
+			const double z = -dm_log(xi_2) - dm_log(xi_3);
+			x = dm_sqrt(z);
+		}
+		else
+		{
+			const double xi_2 = dm_max(dm_uniform(&L.rng), DM_DBL_MIN);
+			const double xi_3 = dm_uniform(&L.rng);
+			const double xi_4 = dm_max(dm_uniform(&L.rng), DM_DBL_MIN);
+			const double c = dm_cos(DM_PI_2 * xi_3);
+			const double z = -dm_log(xi_2) - c * c * dm_log(xi_4);
+			x = dm_sqrt(z);
+		}
+	}
+	const double vT = x / kappa;
+	const double phi = 2.0 * DM_PI * dm_uniform(&L.rng);
+	double u[3];
+	if(!dms_unit_vector_at_angle(mu, phi, vel, u))
+		return false;
+	for(int c = 0; c < 3; c++)
+		out[c] = vT * u[c];
+	return dm_isfinite(dms_norm(out));
+}
+
+/* Scatter after the target choice (:1249-1269): target velocity, CM angle
+ * (low-mass contact 2 xi - 1; dark photon src/Dark_Photon.cpp:286-325) and
+ * New_DM_Velocity (:1228-1247).  target: -1 electron, >= 0 isotope. */
+DMS_HD bool dms_scatter_rest(DmsLane& L, const DmsPointView& P, const DmsSolarView& S, int target)
+{
+	const double r = dms_norm(L.ev_r);
+	const double v = dms_norm(L.ev_v);
+	const double mT = (target == -1) ? DMS_mElectron : S.iso_mass[target];
+	double vT[3];
+	if(!dms_sample_target_velocity(L, dms_temperature(S, r), mT, L.ev_v, vT))
+		return false;
+	const double xi = dm_uniform(&L.rng);
+	double cos_a;
+	if(P.interaction == 2 && P.ff == 1)
+	{
+		const double m2 = P.m_dark_photon * P.m_dark_photon;
+		const double mu = (target < 0) ? P.mu_e : P.mass * S.iso_mass[target] / (P.mass + S.iso_mass[target]);
+		const double q2max = 4.0 * ((mu * v) * (mu * v));
+		cos_a = (m2 * (2.0 * xi - 1.0) + q2max * xi) / (m2 + q2max * xi);
+	}
+	else
+		cos_a = 2.0 * xi - 1.0;
+	const double mDM = P.mass;
+	if(!dm_isfinite(cos_a) || cos_a < -1.0 - 1.0e-12 || cos_a > 1.0 + 1.0e-12 || !dm_isfinite(mDM) || mDM <= 0.0
+	   || !dm_isfinite(mT) || mT <= 0.0 || !dm_isfinite(dms_norm(L.ev_v)) || !dm_isfinite(dms_norm(vT)))
+		return false;
+	cos_a = dm_clamp_cos(cos_a);
+	const double phi = 2.0 * DM_PI * dm_uniform(&L.rng);
+	double n[3];
+	if(!dms_unit_vector_at_angle(cos_a, phi, L.ev_v, n))
+		return false;
+	const double d[3] = {vT[0] - L.ev_v[0], vT[1] - L.ev_v[1], vT[2] - L.ev_v[2]};
+	const double rel = dms_norm(d);
+	const double sum_m = mT + mDM;
+	const double f = mT * rel / sum_m;
+	double out[3];
+	for(int c = 0; c < 3; c++)
+		out[c] = f * n[c] + (mDM * L.ev_v[c] + mT * vT[c]) / (mT + mDM);
+	if(!dm_isfinite(dms_norm(out)))
+		return false;
+	for(int c = 0; c < 3; c++)
+		L.ev_v[c] = out[c];
+	return true;
+}
+
+/* Target choice of Sample_Target (:1140-1157) from the per-target rates
+ * rates[0..n_iso-1] (nuclei) and rates[63] (electrons); -2 on an invalid rate. */
+DMS_HD int dms_choose_target(const double* rates, int n_iso, double xi)
+{
+	for(int i = 0; i < n_iso; i++)
+		if(!dm_isfinite(rates[i]) || rates[i] < 0.0)
+			return -2;
+	const double re = rates[DMS_MAX_TARGETS - 1];
+	if(!dm_isfinite(re) || re < 0.0)
+		return -2;
+	double total = re;
+	for(int i = 0; i < n_iso; i++)
+		total = total + rates[i];
+	if(!dm_isfinite(total) || total <= 0.0)
+		return -2;
+	double sum = re / total;
+	if(sum > xi)
+		return -1;
+	for(int i = 0; i < n_iso; i++)
+	{
+		sum += rates[i] / total;
+		if(sum > xi || i == n_iso - 1)
+			return i;
+	}
+	return -2;
+}
+
+/* Simulate start for one trajectory (:1284-1302). */
+DMS_HD void dms_trajectory_begin(DmsLane& L, const DmsSolarView& S, double dm_mass, const DmsJobView& J, uint64_t id,
+                                 const dms_event& ic)
+{
+	dm_rng_init(&L.rng, J.seed, J.point_id, id, DMS_STREAM_SIM);
+	L.ev_t = ic.t;
+	for(int c = 0; c < 3; c++)
+	{
+		L.ev_r[c] = ic.r[c];
+		L.ev_v[c] = ic.v[c];
+	}
+	L.n_scat = 0;
+	L.reason = DMS_UNKNOWN;
+	L.captured = 0;
+	L.bound = 0;
+	L.event_observed = 0;
+	L.boundary_escape_observed = 0;
+	L.survival_valid = 1;
+	L.nie = 0;
+	L.t_capture = -1.0;
+	L.t_last_bound = -1.0;
+	L.t_unbind = dm_nan();
+	L.t_boundary = dm_nan();
+	L.r_first_neg = -1.0;
+	L.E_first_neg = dm_nan();
+	L.dE_first_neg = dm_nan();
+	L.drift_eV = 0.0;
+	L.drift_rel = 0.0;
+	L.prev_time_sec = -1.0;
+	L.prev_r_km = 0.0;
+	L.prev_v2 = 0.0;
+	L.prev_cap_E = dms_capture_energy_eV(S, dm_mass, dms_norm(ic.r), dms_norm(ic.v));
+	L.ff_ref_E = dm_nan();
+	L.cur_bin = -1;
+	L.acc_dt = 0.0;
+	L.acc_v2 = 0.0;
+	L.acc_cnt = 0;
+	L.rk_all = 0;
+	L.n_int = 0;
+	L.n_ext = 0;
+	L.n_rej = 0;
+	L.n_tv = 0;
+}
+
+/* After Propagate_Freely returned pr (:1313-1347).  Returns the next phase. */
+DMS_HD int dms_after_propagation(DmsLane& L, int pr)
+{
+	if(pr == DMS_SCATTER && dms_norm(L.ev_r) < DMS_rSun)
+		return PH_SCAT;
+	L.reason = pr;
+	return PH_FIN;
+}
+
+/* Scatter bookkeeping after a successful Scatter (:1327-1341). Returns next phase. */
+DMS_HD int dms_after_scatter(DmsLane& L, const DmsSolarView& S, double dm_mass, const DmsJobView& J)
+{
+	L.n_scat++;
+	const int cap = dms_update_capture_state(L, S, dm_mass, J.terminate_on_capture, dms_norm(L.ev_r), dms_norm(L.ev_v), L.ev_t, 1);
+	if(!J.terminate_on_capture)
+	{
+		/* Reset_Bincount_Anchor, :671-680 */
+		L.prev_time_sec = L.ev_t / DMS_sec;
+		L.prev_r_km = dms_norm(L.ev_r) / DMS_km;
+		const double v = dms_norm(L.ev_v) / (DMS_km / DMS_sec);
+		L.prev_v2 = v * v;
+	}
+	if(J.terminate_on_capture && cap)
+	{
+		L.reason = DMS_CAPTURE_MODE;
+		return PH_FIN;
+	}
+	return PH_SEG;
+}
+
+DMS_HD bool dms_invalidates_survival(int reason)
+{
+	switch(reason)
+	{
+		case DMS_ENERGY_DRIFT_ESCAPE:
+		case DMS_NUMERICAL_FAILURE:
+		case DMS_NON_FINITE_STATE:
+		case DMS_SPEED_LIMIT:
+		case DMS_WALL_TIME_LIMIT:
+		case DMS_MAX_FREE_STEPS:
+		case DMS_MAX_SCATTERINGS:
+		case DMS_UNKNOWN: return true;
+		default: return false;
+	}
+}
+
+/* Post-loop classification of Simulate (:1349-1397) plus the
+ * Particle_Free / Particle_Reflected flags (:529-542) and the 1 AU shift of
+ * reflected particles (src/Data_Generation.cpp:829).  Fills the record. */
+DMS_HD void dms_trajectory_end(DmsLane& L, const DmsSolarView& S, double dm_mass, const DmsJobView& J, uint64_t id,
+                               const dms_event& ic, dms_traj_record* rec, DmsSummary* sum)
+{
+	int reason = L.reason;
+	if(reason == DMS_UNKNOWN && L.n_scat >= J.max_scatterings)
+		reason = DMS_MAX_SCATTERINGS;
+	const double t_term = L.ev_t / DMS_sec;
+	double t_boundary = dm_nan();
+	double t_unbind = L.t_unbind;
+	if(L.captured)
+	{
+		const double rf = dms_norm(L.ev_r), vf = dms_norm(L.ev_v);
+		const double vesc = dms_local_escape_speed(S, rf);
+		const double Ef = 0.5 * dm_mass * (vf * vf - vesc * vesc);
+		const double Ef_eV = Ef / DMS_eV;
+		if(reason == DMS_OUTWARD_ESCAPE && Ef_eV >= 0.0)
+		{
+			if(dm_isfinite(t_unbind))
+			{
+				L.boundary_escape_observed = 1;
+				t_boundary = t_term;
+				L.event_observed = 1;
+			}
+			else
+			{
+				reason = DMS_ENERGY_DRIFT_ESCAPE;
+				L.survival_valid = 0;
+				L.nie = 1;
+				L.boundary_escape_observed = 0;
+				L.event_observed = 0;
+			}
+		}
+	}
+	if(dms_invalidates_survival(reason))
+		L.survival_valid = 0;
+	if(L.captured && !L.event_observed)
+	{
+		t_unbind = dm_nan();
+		if(reason != DMS_CAPTURE_MODE)
+			L.survival_valid = 0;
+	}
+	uint32_t flags = 0;
+	if(L.captured)
+		flags |= DMS_F_CAPTURED;
+	if(L.event_observed)
+		flags |= DMS_F_EVENT_OBSERVED;
+	if(L.boundary_escape_observed)
+		flags |= DMS_F_BOUNDARY_ESCAPE_OBSERVED;
+	if(L.survival_valid)
+		flags |= DMS_F_SURVIVAL_VALID;
+	if(L.nie)
+		flags |= DMS_F_NUMERICALLY_INVALID_ESCAPE;
+	if(reason == DMS_OUTWARD_ESCAPE)
+	{
+		if(L.n_scat == 0)
+			flags |= DMS_F_FREE;
+		const double rf = dms_norm(L.ev_r);
+		const double vesc = dm_sqrt(2 * DMS_G_Newton * DMS_mSun / rf);
+		if(rf > DMS_rSun && dms_norm(L.ev_v) > vesc && L.n_scat > 0)
+		{
+			flags |= DMS_F_REFLECTED;
+			double r[3] = {L.ev_r[0], L.ev_r[1], L.ev_r[2]}, v[3] = {L.ev_v[0], L.ev_v[1], L.ev_v[2]};
+			if(!dms_kepler_shift(r, v, 1.0 * DMS_AU))
+				flags |= DMS_F_REFLECTION_SHIFT_FAILED;
+		}
+	}
+	if(sum)
+	{
+		sum->reason = reason;
+		sum->flags = flags;
+		sum->n_scat = L.n_scat;
+		sum->t_capture_sec = L.t_capture;
+		sum->t_unbind_sec = t_unbind;
+		sum->n_rk = L.rk_all;
+		sum->n_int = L.n_int;
+		sum->n_ext = L.n_ext;
+		sum->n_rej = L.n_rej;
+		sum->n_tv = L.n_tv;
+	}
+	if(rec)
+	{
+		rec->initial = ic;
+		rec->final.t = L.ev_t;
+		for(int c = 0; c < 3; c++)
+		{
+			rec->final.r[c] = L.ev_r[c];
+			rec->final.v[c] = L.ev_v[c];
+		}
+		rec->trajectory_id = id;
+		rec->n_scatterings = L.n_scat;
+		rec->reason = reason;
+		rec->flags = flags;
+		rec->t_capture_sec = L.t_capture;
+		rec->t_last_bound_sec = L.t_last_bound;
+		rec->t_final_unbinding_scatter_sec = t_unbind;
+		rec->t_boundary_escape_sec = t_boundary;
+		rec->t_termination_sec = t_term;
+		rec->r_first_negative_km = L.r_first_neg;
+		rec->E_first_negative_eV = L.E_first_neg;
+		rec->dE_first_negative_from_prev_eV = L.dE_first_neg;
+		rec->max_free_energy_drift_eV = L.drift_eV;
+		rec->max_free_energy_drift_rel = L.drift_rel;
+		rec->n_rk_attempts = L.rk_all;
+		rec->n_steps_interior = L.n_int;
+		rec->n_steps_exterior = L.n_ext;
+		rec->n_tau_rejections = L.n_rej;
+		rec->n_tv_iterations = L.n_tv;
+	}
+}
+
+/* Record of a trajectory whose initial Kepler shift failed
+ * (src/Data_Generation.cpp:749-761). */
+DMS_HD void dms_shift_failure_record(uint64_t id, const dms_event& ic, dms_traj_record* rec, DmsSummary* sum)
+{
+	if(sum)
+	{
+		sum->reason = DMS_NUMERICAL_FAILURE;
+		sum->flags = DMS_F_SHIFT_FAILED;
+		sum->n_scat = 0;
+		sum->t_capture_sec = -1.0;
+		sum->t_unbind_sec = dm_nan();
+		sum->n_rk = 0;
+		sum->n_int = sum->n_ext = sum->n_rej = sum->n_tv = 0;
+	}
+	if(rec)
+	{
+		rec->initial = ic;
+		rec->final = ic;
+		rec->trajectory_id = id;
+		rec->n_scatterings = 0;
+		rec->reason = DMS_NUMERICAL_FAILURE;
+		rec->flags = DMS_F_SHIFT_FAILED;
+		rec->t_capture_sec = -1.0;
+		rec->t_last_bound_sec = -1.0;
+		rec->t_final_unbinding_scatter_sec = dm_nan();
+		rec->t_boundary_escape_sec = dm_nan();
+		rec->t_termination_sec = -1.0;
+		rec->r_first_negative_km = -1.0;
+		rec->E_first_negative_eV = dm_nan();
+		rec->dE_first_negative_from_prev_eV = dm_nan();
+		rec->max_free_energy_drift_eV = 0.0;
+		rec->max_free_energy_drift_rel = 0.0;
+		rec->n_rk_attempts = 0;
+		rec->n_steps_interior = rec->n_steps_exterior = rec->n_tau_rejections = rec->n_tv_iterations = 0;
+	}
+}
+
+
+/* ---- phase drivers shared by the kernel and host replays ------------------ */
+struct DmsLaneCtx
+{
+	DmsSolarView S;
+	DmsPointView P;
+	DmsJobView J;
+	DmsRows R;
+};
+
+/* RK attempt, step post-processing and the scatter hand-off (r, v and the
+ * Sample_Target uniform are drawn here, before the wave-level target choice). */
+DMS_HD void dms_lane_advance_a(DmsLane& L, const DmsLaneCtx& c)
+{
+	if(L.phase == PH_RK)
+	{
+		L.inner_iter++;
+		L.rk_all++;
+		const int r = dms_rk45_attempt(L.prop, c.S, L.inner_iter);
+		if(r >= 0)
+		{
+			L.rk_ok = r;
+			L.phase = PH_POST;
+		}
+	}
+	if(L.phase == PH_POST)
+	{
+		const int pr = dms_step_post(L, c.P, c.S, c.J, c.R);
+		if(pr == -2)
+			L.phase = PH_PREP;
+		else
+			L.phase = dms_after_propagation(L, (pr == -3) ? (int) DMS_SCATTER : pr);
+	}
+	if(L.phase == PH_SCAT)
+	{
+		L.sc_r = dms_norm(L.ev_r);
+		L.sc_v = dms_norm(L.ev_v);
+		L.sc_xi = dm_uniform(&L.rng);
+	}
+}
+
+/* Rest of Scatter once sc_target holds the Sample_Target result. */
+DMS_HD void dms_lane_finish_scatter(DmsLane& L, const DmsLaneCtx& c)
+{
+	if(L.phase != PH_SCAT)
+		return;
+	if(L.sc_target < -1 || !dms_scatter_rest(L, c.P, c.S, L.sc_target))
+	{
+		L.reason = DMS_NUMERICAL_FAILURE;
+		L.phase = PH_FIN;
+	}
+	else
+		L.phase = dms_after_scatter(L, c.S, c.P.mass, c.J);
+}
+
+DMS_HD void dms_lane_finish(DmsLane& L, const DmsLaneCtx& c, const dms_event& ic, dms_traj_record* rec, DmsSummary* sum)
+{
+	dms_bin_flush(L, c.R, c.J.nbins);
+	dms_trajectory_end(L, c.S, c.P.mass, c.J, c.J.first_id + L.slot, ic, rec, sum);
+	L.phase = PH_NEED;
+}
+
+DMS_HD void dms_lane_start(DmsLane& L, const DmsLaneCtx& c, const dms_event& ic, bool shift_ok, dms_traj_record* rec, DmsSummary* sum)
+{
+	if(!shift_ok)
+	{
+		dms_shift_failure_record(c.J.first_id + L.slot, ic, rec, sum);
+		L.phase = PH_NEED;
+		return;
+	}
+	dms_trajectory_begin(L, c.S, c.P.mass, c.J, c.J.first_id + L.slot, ic);
+	L.phase = PH_SEG;
+}
+
+/* Segment start and step preparation. */
+DMS_HD void dms_lane_advance_b(DmsLane& L, const DmsLaneCtx& c)
+{
+	if(L.phase == PH_SEG)
+	{
+		if(!(L.n_scat < c.J.max_scatterings))
+		{
+			L.reason = DMS_UNKNOWN;
+			L.phase = PH_FIN;
+		}
+		else
+		{
+			const int pr = dms_segment_start(L, c.S, c.P.mass, c.J);
+			L.phase = (pr < 0) ? (int) PH_PREP : dms_after_propagation(L, pr);
+		}
+	}
+	if(L.phase == PH_PREP)
+	{
+		const int pr = dms_step_prep(L, c.P, c.S, c.J);
+		L.phase = (pr < 0) ? (int) PH_RK : dms_after_propagation(L, pr);
+	}
+}
+
+#endif

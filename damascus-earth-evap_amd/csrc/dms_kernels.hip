@@ -1,0 +1,254 @@
+/*
+ * gfx950 kernels of the solar dark-matter trajectory path (FP64).
+ *
+ *   dms_traj_kernel   persistent, one lane per trajectory, lane refill from an
+ *                     atomic work counter; Sample_Target's 63 nuclear + 1
+ *                     electron rates are evaluated by the 64 lanes of the wave
+ *                     together (one target per lane) for each scattering lane.
+ *   dms_ic_kernel     Initial_Conditions + Hyperbolic_Kepler_Shift, one lane each.
+ *   dms_grid_kernel   Interpolate_Total_DM_Scattering_Rate, one node per lane.
+ *   dms_rates_kernel  point queries of the rate functions.
+ *   dms_hist_*        per-bin sums over trajectory rows, fixed (deterministic) order.
+ *   dms_probe_kernel  elementary functions on the device (for host comparison).
+ */
+#include <hip/hip_runtime.h>
+
+#include "dms_engine.h"
+#include "dms_halo.h"
+#include "dms_kernels.hpp"
+
+namespace
+{
+
+__device__ inline double readlane_d(double x, int l)
+{
+	const int lo = __builtin_amdgcn_readlane(__double2loint(x), l);
+	const int hi = __builtin_amdgcn_readlane(__double2hiint(x), l);
+	return __hiloint2double(hi, lo);
+}
+
+__device__ inline double shfl_d(double x, int src) { return __shfl(x, src); }
+
+} // namespace
+
+__global__ void __launch_bounds__(DMS_TRAJ_BLOCK) dms_traj_kernel(DmsTrajArgs a)
+{
+	const int lane = threadIdx.x & 63;
+	DmsLaneCtx c;
+	c.S = a.S;
+	c.P = a.P;
+	c.J = a.J;
+	c.R = a.R;
+	DmsLane L;
+	L.phase = PH_NEED;
+	L.cur_bin = -1;
+	L.slot = 0;
+	L.sc_target = -2;
+	const int n_iso = a.S.n_iso;
+	for(;;)
+	{
+		dms_lane_advance_a(L, c);
+		/* Sample_Target (src/Simulation_Trajectory.cpp:1122-1158): for each
+		 * scattering lane s the 64 lanes evaluate one target rate each. */
+		unsigned long long m = __ballot(L.phase == PH_SCAT);
+		while(m)
+		{
+			const int s = __ffsll((long long) m) - 1;
+			m &= m - 1;
+			const double r = shfl_d(L.sc_r, s);
+			const double v = shfl_d(L.sc_v, s);
+			const double xi = shfl_d(L.sc_xi, s);
+			const int k = (lane < n_iso) ? lane : ((lane == DMS_MAX_TARGETS - 1) ? lane : -1);
+			double rate = 0.0;
+			if(k >= 0)
+				rate = dms_rate_target(a.P, a.S, r, v, k);
+			const bool bad = (k >= 0) && (!dm_isfinite(rate) || rate < 0.0);
+			int target = -2;
+			if(__ballot(bad) == 0)
+			{
+				/* same summation and comparison order as dms_choose_target */
+				const double re = readlane_d(rate, DMS_MAX_TARGETS - 1);
+				double total = re;
+				for(int i = 0; i < n_iso; i++)
+					total = total + readlane_d(rate, i);
+				if(dm_isfinite(total) && total > 0.0)
+				{
+					const double q = rate / total;
+					double sum = readlane_d(q, DMS_MAX_TARGETS - 1);
+					if(sum > xi)
+						target = -1;
+					else
+						for(int i = 0; i < n_iso; i++)
+						{
+							sum += readlane_d(q, i);
+							if(sum > xi || i == n_iso - 1)
+							{
+								target = i;
+								break;
+							}
+						}
+				}
+			}
+			if(lane == s)
+				L.sc_target = target;
+		}
+		dms_lane_finish_scatter(L, c);
+		if(L.phase == PH_FIN)
+			dms_lane_finish(L, c, a.ics[L.slot], a.records ? a.records + L.slot : nullptr, a.summ ? a.summ + L.slot : nullptr);
+		/* lane refill: one atomic per wave */
+		const unsigned long long nmask = __ballot(L.phase == PH_NEED);
+		if(nmask)
+		{
+			const int leader = __ffsll((long long) nmask) - 1;
+			unsigned int base = 0;
+			if(lane == leader)
+				base = atomicAdd(a.next, (unsigned int) __popcll(nmask));
+			base = __shfl(base, leader);
+			if(L.phase == PH_NEED)
+			{
+				const unsigned int rank = (unsigned int) __popcll(nmask & ((1ull << lane) - 1ull));
+				const unsigned int idx = base + rank;
+				if(idx < a.n)
+				{
+					L.slot = idx;
+					L.phase = PH_START;
+				}
+				else
+					L.phase = PH_DONE;
+			}
+		}
+		if(L.phase == PH_START)
+			dms_lane_start(L, c, a.ics[L.slot], a.shift_ok == nullptr || a.shift_ok[L.slot] != 0,
+			               a.records ? a.records + L.slot : nullptr, a.summ ? a.summ + L.slot : nullptr);
+		dms_lane_advance_b(L, c);
+		if(__ballot(L.phase != PH_DONE) == 0)
+			break;
+	}
+}
+
+__global__ void dms_ic_kernel(DmsHaloView H, uint32_t seed, uint32_t point_id, uint64_t first_id, uint32_t n, double R_final,
+                              dms_event* out, int32_t* ok)
+{
+	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+	if(i >= n)
+		return;
+	dm_rng rng;
+	dm_rng_init(&rng, seed, point_id, first_id + i, DMS_STREAM_IC);
+	dms_event e;
+	dms_initial_condition(H, &rng, &e);
+	int s = 1;
+	if(R_final > 0.0)
+		s = dms_kepler_shift(e.r, e.v, R_final) ? 1 : 0;
+	out[i] = e;
+	ok[i] = s;
+}
+
+__global__ void dms_grid_kernel(DmsPointView P, DmsSolarView S, int Nr, int Nv, double vmax, double* grid)
+{
+	const size_t idx = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+	if(idx >= (size_t) Nr * (size_t) Nv)
+		return;
+	const int i = (int) (idx / (size_t) Nv);
+	const int j = (int) (idx % (size_t) Nv);
+	const double dv = (vmax - 0.0) / (double) (Nv - 1);
+	const double radius = DMS_rSun * (double) i / (double) (Nr - 1);
+	const double speed = 0.0 + j * dv;
+	grid[idx] = dms_rate_computed(P, S, radius, speed);
+}
+
+__global__ void dms_rates_kernel(DmsPointView P, DmsSolarView S, int which, const double* r, const double* v, double* out, size_t n)
+{
+	const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+	if(i >= n)
+		return;
+	double res;
+	if(which == 0)
+		res = dms_rate_total(P, S, r[i], v[i]);
+	else if(which == 1)
+		res = dms_rate_computed(P, S, r[i], v[i]);
+	else if(which == 2)
+		res = dms_rate_target(P, S, r[i], v[i], DMS_MAX_TARGETS - 1);
+	else
+		res = dms_rate_target(P, S, r[i], v[i], which - 3);
+	out[i] = res;
+}
+
+/* Partial per-bin sums over trajectories [blk*TB, min(n, (blk+1)*TB)) in id order.
+ * cls: 1 captured, 2 completed outward escape, 0 not counted.
+ * partial layout: [blk][nbins][8] doubles, pcount: [blk][nbins][2]. */
+__global__ void dms_hist_partial_kernel(const double* rows, const uint32_t* counts, const uint8_t* cls, uint32_t n, int nbins,
+                                        uint32_t tb, double* partial, unsigned long long* pcount)
+{
+	const int b = blockIdx.x * blockDim.x + threadIdx.x;
+	const uint32_t blk = blockIdx.y;
+	if(b >= nbins)
+		return;
+	const uint32_t i0 = blk * tb;
+	const uint32_t i1 = (i0 + tb < n) ? i0 + tb : n;
+	double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+	unsigned long long c[2] = {0, 0};
+	for(uint32_t i = i0; i < i1; i++)
+	{
+		const uint8_t k = cls[i];
+		if(k == 0)
+			continue;
+		const double dt = rows[(size_t) i * 2 * nbins + b];
+		const double v2 = rows[(size_t) i * 2 * nbins + nbins + b];
+		const uint32_t cn = counts ? counts[(size_t) i * nbins + b] : 0u;
+		const int o = (k == 1) ? 0 : 4;
+		s[o + 0] += dt;
+		s[o + 1] += v2;
+		s[o + 2] += dt * dt;
+		s[o + 3] += v2 * v2;
+		c[k - 1] += cn;
+	}
+	double* p = partial + ((size_t) blk * nbins + b) * 8;
+	for(int q = 0; q < 8; q++)
+		p[q] = s[q];
+	pcount[((size_t) blk * nbins + b) * 2 + 0] = c[0];
+	pcount[((size_t) blk * nbins + b) * 2 + 1] = c[1];
+}
+
+/* hist[q][b] += sum over blocks (in block order); counts[k][b] likewise. */
+__global__ void dms_hist_final_kernel(const double* partial, const unsigned long long* pcount, uint32_t nblk, int nbins,
+                                      double* hist, unsigned long long* counts)
+{
+	const int b = blockIdx.x * blockDim.x + threadIdx.x;
+	if(b >= nbins)
+		return;
+	double s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+	unsigned long long c0 = 0, c1 = 0;
+	for(uint32_t blk = 0; blk < nblk; blk++)
+	{
+		const double* p = partial + ((size_t) blk * nbins + b) * 8;
+		for(int q = 0; q < 8; q++)
+			s[q] += p[q];
+		c0 += pcount[((size_t) blk * nbins + b) * 2 + 0];
+		c1 += pcount[((size_t) blk * nbins + b) * 2 + 1];
+	}
+	for(int q = 0; q < 8; q++)
+		hist[(size_t) q * nbins + b] += s[q];
+	counts[b] += c0;
+	counts[nbins + b] += c1;
+}
+
+__global__ void dms_probe_kernel(int fn, const double* x, const double* y, double* out, size_t n)
+{
+	const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+	if(i >= n)
+		return;
+	const double a = x[i];
+	double r;
+	switch(fn)
+	{
+		case 0: r = dm_sqrt(a); break;
+		case 1: r = a / y[i]; break;
+		case 2: r = dm_exp(a); break;
+		case 3: r = dm_log(a); break;
+		case 4: r = dm_sin(a); break;
+		case 5: r = dm_cos(a); break;
+		case 6: r = dm_erf(a); break;
+		default: r = dm_acos(a); break;
+	}
+	out[i] = r;
+}

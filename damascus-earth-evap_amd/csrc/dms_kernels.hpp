@@ -1,0 +1,37 @@
+/* Kernel argument blocks and launch entry points (private to libdmsun). */
+#ifndef DMS_KERNELS_HPP
+#define DMS_KERNELS_HPP
+
+#include <hip/hip_runtime.h>
+
+#include "dms_engine.h"
+#include "dms_halo.h"
+
+#define DMS_TRAJ_BLOCK 256
+
+struct DmsTrajArgs
+{
+	DmsSolarView S;
+	DmsPointView P;
+	DmsJobView J;
+	const dms_event* ics;
+	const int32_t* shift_ok;
+	uint32_t n;
+	dms_traj_record* records;
+	DmsSummary* summ;
+	DmsRows R;
+	unsigned int* next;
+};
+
+__global__ void dms_traj_kernel(DmsTrajArgs a);
+__global__ void dms_ic_kernel(DmsHaloView H, uint32_t seed, uint32_t point_id, uint64_t first_id, uint32_t n, double R_final,
+                              dms_event* out, int32_t* ok);
+__global__ void dms_grid_kernel(DmsPointView P, DmsSolarView S, int Nr, int Nv, double vmax, double* grid);
+__global__ void dms_rates_kernel(DmsPointView P, DmsSolarView S, int which, const double* r, const double* v, double* out, size_t n);
+__global__ void dms_hist_partial_kernel(const double* rows, const uint32_t* counts, const uint8_t* cls, uint32_t n, int nbins,
+                                        uint32_t tb, double* partial, unsigned long long* pcount);
+__global__ void dms_hist_final_kernel(const double* partial, const unsigned long long* pcount, uint32_t nblk, int nbins,
+                                      double* hist, unsigned long long* counts);
+__global__ void dms_probe_kernel(int fn, const double* x, const double* y, double* out, size_t n);
+
+#endif

@@ -1,0 +1,1175 @@
+/*
+ * libdmsun: C ABI (include/dms.h) over the gfx950 trajectory kernels.
+ *
+ * Job flow (Simulation_Data::Generate_Data, src/Data_Generation.cpp:488-972):
+ * trajectories are simulated speculatively in large id-ordered chunks
+ * (initial conditions on the device, then the persistent trajectory kernel),
+ * the host folds the per-trajectory summaries into the run totals strictly in
+ * id order with the reference's round logic (sync interval, capture target,
+ * run-level fuses), and a deterministic reduction kernel sums the per-bin rows
+ * of the trajectories that count.  Results therefore do not depend on launch
+ * geometry or chunking.
+ */
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "dms_host.hpp"
+#include "dms_kernels.hip"
+
+struct dms_ctx
+{
+	int device = -1;
+	hipStream_t stream = nullptr;
+	std::mutex mu;
+	std::string err;
+	dms::SolarTables T;
+	int n_cu = 0;
+	int traj_blocks_per_cu = 0;
+	/* device copies of the solar tables */
+	double* d_x = nullptr;
+	DmsCubic *d_mass = nullptr, *d_temp = nullptr, *d_vesc2 = nullptr, *d_dens = nullptr;
+	double *d_pref = nullptr, *d_imass = nullptr;
+	int* d_Z = nullptr;
+	DmsSolarView dev;
+};
+
+struct dms_point
+{
+	dms_ctx* ctx = nullptr;
+	dms_point_desc desc;
+	dms::PointParams P;
+	double* d_sigmaN = nullptr;
+	double* d_grid = nullptr;
+	int N = 0;
+	DmsPointView dev;
+};
+
+struct dms_job
+{
+	dms_point* pt = nullptr;
+	dms_job_desc desc;
+	std::thread worker;
+	std::atomic<uint64_t> done_traj{0}, done_captured{0}, done_scat{0};
+	std::atomic<int> finished{0};
+	int status = DMS_OK;
+	std::string err;
+	dms_counters C;
+	std::vector<double> hist;
+	std::vector<unsigned long long> counts;
+	std::vector<dms_evap_event> events;
+	std::vector<dms_traj_record> records;
+	double kernel_ms = 0.0, other_ms = 0.0, launches = 0.0;
+};
+
+namespace
+{
+
+thread_local std::string g_last_error;
+
+int fail(dms_ctx* ctx, int code, const std::string& msg)
+{
+	if(ctx)
+		ctx->err = msg;
+	g_last_error = msg;
+	return code;
+}
+
+#define DMS_HIP(ctx, call)                                                                                         \
+	do                                                                                                           \
+	{                                                                                                            \
+		hipError_t e_ = (call);                                                                                  \
+		if(e_ != hipSuccess)                                                                                     \
+			return fail((ctx), DMS_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_));               \
+	} while(0)
+
+template <typename T>
+hipError_t dev_alloc_copy(T** d, const T* h, size_t n)
+{
+	hipError_t e = hipMalloc((void**) d, sizeof(T) * (n ? n : 1));
+	if(e != hipSuccess)
+		return e;
+	if(n)
+		e = hipMemcpy(*d, h, sizeof(T) * n, hipMemcpyHostToDevice);
+	return e;
+}
+
+DmsPointView host_point_view(const dms_point* pt)
+{
+	DmsPointView v;
+	v.mass = pt->P.mass;
+	v.mu_e = pt->P.mu_e;
+	v.interaction = pt->P.interaction;
+	v.ff = pt->P.ff;
+	v.m_dark_photon = pt->P.m_dark_photon;
+	v.q_reference = pt->P.q_reference;
+	v.sigma_e = pt->P.sigma_e;
+	v.sigma_N = pt->P.sigma_N;
+	v.use_grid = pt->N >= 2;
+	v.Nr = pt->N;
+	v.Nv = pt->N;
+	v.inv_dr = pt->N >= 2 ? (double) (pt->N - 1) / DMS_rSun : 0.0;
+	v.inv_dv = pt->N >= 2 ? (double) (pt->N - 1) / DMS_V_MAX : 0.0;
+	v.vmax = pt->N >= 2 ? DMS_V_MAX : 0.0;
+	v.grid = nullptr;
+	return v;
+}
+
+DmsJobView job_view(const dms_job_desc& d)
+{
+	DmsJobView J;
+	J.max_time_steps = d.max_free_steps;
+	J.max_scatterings = d.max_scatterings;
+	J.rk_budget = d.rk_attempt_budget;
+	J.max_distance = d.r_max_rsun * DMS_rSun;
+	J.terminate_on_capture = (d.mode == DMS_MODE_CAPTURE);
+	J.nbins = d.n_bins > 0 ? d.n_bins : 2000;
+	J.bin_width_km = (2.0 * DMS_R_SUN_KM) / J.nbins;
+	J.seed = d.seed;
+	J.point_id = d.point_id;
+	J.first_id = d.first_trajectory_id;
+	return J;
+}
+
+std::string check_job(const dms_job_desc& d)
+{
+	if(d.mode != DMS_MODE_CAPTURE && d.mode != DMS_MODE_PARAMETER_POINT)
+		return "mode must be capture or parameter point";
+	if(d.precision != DMS_FP64 && d.precision != DMS_MIXED)
+		return "precision must be fp64 or mixed";
+	if(!(d.r_max_rsun >= 1.0) || !std::isfinite(d.r_max_rsun))
+		return "r_max_rsun must be finite and >= 1 (maximum distance of Trajectory_Simulator)";
+	if(d.n_bins < 0 || d.n_bins > (1 << 20))
+		return "n_bins out of range";
+	return "";
+}
+
+DmsHaloView halo_view(const dms_ctx* ctx, const dms_halo& h)
+{
+	DmsHaloView H;
+	dms_halo_init(H, h);
+	const DmsSolarView V = ctx->T.view();
+	H.vescR = dms_local_escape_speed(V, DMS_rSun);
+	H.vesc_asym = dms_local_escape_speed(V, 1000.0 * DMS_AU);
+	H.norm = dms_shm_moment(H, 1) + H.vescR * H.vescR * dms_shm_moment(H, -1);
+	return H;
+}
+
+std::string check_halo(const dms_halo& h)
+{
+	if(!(h.v0_kms > 0.0) || !(h.vesc_kms > 0.0) || !std::isfinite(h.v0_kms) || !std::isfinite(h.vesc_kms))
+		return "halo v0 and v_esc must be finite and positive";
+	for(int c = 0; c < 3; c++)
+		if(!std::isfinite(h.vobs_kms[c]))
+			return "halo observer velocity must be finite";
+	const double vo = std::sqrt(h.vobs_kms[0] * h.vobs_kms[0] + h.vobs_kms[1] * h.vobs_kms[1] + h.vobs_kms[2] * h.vobs_kms[2]);
+	if(!(vo < h.vesc_kms))
+		return "observer speed must be below the galactic escape speed";
+	return "";
+}
+
+/* Device buffers of one chunk, grown on demand. */
+struct ChunkBuffers
+{
+	dms_event* ics = nullptr;
+	int32_t* ok = nullptr;
+	DmsSummary* summ = nullptr;
+	dms_traj_record* recs = nullptr;
+	double* rows = nullptr;
+	uint32_t* counts = nullptr;
+	uint8_t* cls = nullptr;
+	unsigned int* next = nullptr;
+	size_t cap = 0, rows_cap = 0, recs_cap = 0;
+
+	void release()
+	{
+		hipFree(ics);
+		hipFree(ok);
+		hipFree(summ);
+		hipFree(recs);
+		hipFree(rows);
+		hipFree(counts);
+		hipFree(cls);
+		hipFree(next);
+		*this = ChunkBuffers();
+	}
+};
+
+hipError_t ensure(ChunkBuffers& B, size_t n, int nbins, bool rows, bool recs)
+{
+	hipError_t e = hipSuccess;
+	if(B.next == nullptr && (e = hipMalloc((void**) &B.next, sizeof(unsigned int))) != hipSuccess)
+		return e;
+	if(n > B.cap)
+	{
+		hipFree(B.ics);
+		hipFree(B.ok);
+		hipFree(B.summ);
+		hipFree(B.cls);
+		B.ics = nullptr;
+		B.ok = nullptr;
+		B.summ = nullptr;
+		B.cls = nullptr;
+		if((e = hipMalloc((void**) &B.ics, sizeof(dms_event) * n)) != hipSuccess)
+			return e;
+		if((e = hipMalloc((void**) &B.ok, sizeof(int32_t) * n)) != hipSuccess)
+			return e;
+		if((e = hipMalloc((void**) &B.summ, sizeof(DmsSummary) * n)) != hipSuccess)
+			return e;
+		if((e = hipMalloc((void**) &B.cls, n)) != hipSuccess)
+			return e;
+		B.cap = n;
+	}
+	if(rows && n > B.rows_cap)
+	{
+		hipFree(B.rows);
+		hipFree(B.counts);
+		B.rows = nullptr;
+		B.counts = nullptr;
+		if((e = hipMalloc((void**) &B.rows, sizeof(double) * 2 * (size_t) nbins * n)) != hipSuccess)
+			return e;
+		if((e = hipMalloc((void**) &B.counts, sizeof(uint32_t) * (size_t) nbins * n)) != hipSuccess)
+			return e;
+		B.rows_cap = n;
+	}
+	if(recs && n > B.recs_cap)
+	{
+		hipFree(B.recs);
+		B.recs = nullptr;
+		if((e = hipMalloc((void**) &B.recs, sizeof(dms_traj_record) * n)) != hipSuccess)
+			return e;
+		B.recs_cap = n;
+	}
+	return hipSuccess;
+}
+
+int traj_grid(const dms_ctx* ctx, size_t n)
+{
+	const size_t need = (n + DMS_TRAJ_BLOCK - 1) / DMS_TRAJ_BLOCK;
+	const size_t resident = (size_t) ctx->n_cu * (size_t) (ctx->traj_blocks_per_cu > 0 ? ctx->traj_blocks_per_cu : 1);
+	const size_t g = need < resident ? need : resident;
+	return (int) (g > 0 ? g : 1);
+}
+
+/* Launch the trajectory kernel on ids [first, first+n) with buffers B. Returns the kernel time. */
+hipError_t run_trajectories(dms_ctx* ctx, const dms_point* pt, const DmsJobView& J, ChunkBuffers& B, size_t n, bool rows,
+                            bool recs, float* ms)
+{
+	DmsTrajArgs a;
+	a.S = ctx->dev;
+	a.P = pt->dev;
+	a.J = J;
+	a.ics = B.ics;
+	a.shift_ok = B.ok;
+	a.n = (uint32_t) n;
+	a.records = recs ? B.recs : nullptr;
+	a.summ = B.summ;
+	a.R.rows = rows ? B.rows : nullptr;
+	a.R.counts = rows ? B.counts : nullptr;
+	a.next = B.next;
+	hipError_t e;
+	if((e = hipMemsetAsync(B.next, 0, sizeof(unsigned int), ctx->stream)) != hipSuccess)
+		return e;
+	if(rows)
+	{
+		if((e = hipMemsetAsync(B.rows, 0, sizeof(double) * 2 * (size_t) J.nbins * n, ctx->stream)) != hipSuccess)
+			return e;
+		if((e = hipMemsetAsync(B.counts, 0, sizeof(uint32_t) * (size_t) J.nbins * n, ctx->stream)) != hipSuccess)
+			return e;
+	}
+	hipEvent_t e0, e1;
+	hipEventCreate(&e0);
+	hipEventCreate(&e1);
+	hipEventRecord(e0, ctx->stream);
+	hipLaunchKernelGGL(dms_traj_kernel, dim3(traj_grid(ctx, n)), dim3(DMS_TRAJ_BLOCK), 0, ctx->stream, a);
+	e = hipGetLastError();
+	hipEventRecord(e1, ctx->stream);
+	if(e == hipSuccess)
+		e = hipEventSynchronize(e1);
+	float t = 0.0f;
+	hipEventElapsedTime(&t, e0, e1);
+	if(ms)
+		*ms = t;
+	hipEventDestroy(e0);
+	hipEventDestroy(e1);
+	return e;
+}
+
+double now_ms()
+{
+	return 1e-6 * (double) std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+bool is_numerical(int r)
+{
+	return r == DMS_NUMERICAL_FAILURE || r == DMS_NON_FINITE_STATE || r == DMS_SPEED_LIMIT || r == DMS_ENERGY_DRIFT_ESCAPE || r == DMS_UNKNOWN;
+}
+bool is_truncation(int r) { return r == DMS_WALL_TIME_LIMIT || r == DMS_MAX_FREE_STEPS || r == DMS_MAX_SCATTERINGS; }
+
+/* Round logic of Generate_Data for one rank (src/Data_Generation.cpp:696-938). */
+struct RoundReplay
+{
+	uint64_t target, max_traj, interval;
+	bool capture, unlimited;
+	uint64_t local_total = 0, captured = 0, left_in_round = 0;
+	bool stopped = false;
+	int stop = 0;
+	dms_counters* C;
+
+	bool open_round()
+	{
+		if(left_in_round > 0)
+			return true;
+		if(!(captured < target))
+		{
+			stopped = true;
+			return false;
+		}
+		const uint64_t remaining = target - captured;
+		uint64_t capacity = 0;
+		if(local_total < max_traj)
+			capacity = (max_traj - local_total < interval) ? max_traj - local_total : interval;
+		if(capacity == 0)
+		{
+			stop = 1;
+			stopped = true;
+			return false;
+		}
+		const uint64_t batch = capture ? (remaining < capacity ? remaining : capacity) : capacity;
+		C->sync_rounds++;
+		C->final_round_trajectories = batch;
+		left_in_round = batch;
+		return true;
+	}
+
+	void close_one()
+	{
+		left_in_round--;
+		if(left_in_round == 0 && C->trajectories > 0)
+		{
+			const double shift_fraction = (double) C->initial_shift_failures / (double) C->trajectories;
+			if(shift_fraction > 1.0e-2)
+			{
+				stop = 3;
+				stopped = true;
+				return;
+			}
+			const double invalid_fraction = (double) (C->numerical_failures + C->computational_truncations) / (double) C->trajectories;
+			if(unlimited && invalid_fraction > 1.0e-2)
+			{
+				stop = 4;
+				stopped = true;
+			}
+		}
+	}
+};
+
+void run_job(dms_job* job)
+{
+	dms_point* pt = job->pt;
+	dms_ctx* ctx = pt->ctx;
+	std::lock_guard<std::mutex> lock(ctx->mu);
+	const dms_job_desc& d = job->desc;
+	const DmsJobView J = job_view(d);
+	const bool capture = (d.mode == DMS_MODE_CAPTURE);
+	const bool want_rows = !capture;
+	const bool want_recs = d.record_trajectories != 0;
+	const int nb = J.nbins;
+	dms_counters& C = job->C;
+	memset(&C, 0, sizeof(C));
+	job->hist.assign((size_t) 8 * nb, 0.0);
+	job->counts.assign((size_t) 2 * nb, 0ull);
+	RoundReplay R;
+	R.C = &C;
+	R.capture = capture;
+	R.target = d.capture_target == 0 ? UINT64_MAX : d.capture_target;
+	R.unlimited = d.max_trajectories == 0;
+	R.max_traj = R.unlimited ? UINT64_MAX : d.max_trajectories;
+	R.interval = capture ? 32 : dms::normal_mode_sync_interval(pt->P.sigma_p / (DMS_cm * DMS_cm));
+	C.sync_interval = (int32_t) R.interval;
+	const DmsHaloView H = halo_view(ctx, d.halo);
+
+	ChunkBuffers B;
+	double* d_hist = nullptr;
+	unsigned long long* d_counts = nullptr;
+	double* d_partial = nullptr;
+	unsigned long long* d_pcount = nullptr;
+	size_t partial_cap = 0;
+	const uint32_t TB = 512;
+	auto set_err = [&](const std::string& m) {
+		job->status = DMS_ERR_DEVICE;
+		job->err = m;
+	};
+	if(want_rows)
+	{
+		if(hipMalloc((void**) &d_hist, sizeof(double) * 8 * nb) != hipSuccess
+		   || hipMalloc((void**) &d_counts, sizeof(unsigned long long) * 2 * nb) != hipSuccess)
+		{
+			set_err("hipMalloc of histogram accumulators failed");
+			job->finished = 1;
+			return;
+		}
+		hipMemset(d_hist, 0, sizeof(double) * 8 * nb);
+		hipMemset(d_counts, 0, sizeof(unsigned long long) * 2 * nb);
+	}
+	/* Chunk size: bounded by a device-memory budget for the per-trajectory rows. */
+	const size_t row_bytes = want_rows ? (size_t) nb * (2 * sizeof(double) + sizeof(uint32_t)) : 0;
+	const size_t rec_bytes = want_recs ? sizeof(dms_traj_record) : 0;
+	const size_t per_traj = row_bytes + rec_bytes + sizeof(dms_event) + sizeof(DmsSummary) + 8;
+	const size_t budget = (size_t) 48 << 30;
+	size_t max_chunk = budget / per_traj;
+	if(max_chunk > (size_t) 1 << 26)
+		max_chunk = (size_t) 1 << 26;
+	uint64_t next_id = d.first_trajectory_id;
+	uint64_t simulated = 0;
+	const double t_start = now_ms();
+	std::vector<DmsSummary> hsum;
+	std::vector<uint8_t> hcls;
+	std::vector<dms_traj_record> hrec;
+	while(!R.stopped && R.open_round())
+	{
+		/* how many trajectories to simulate speculatively */
+		uint64_t want;
+		if(R.unlimited && R.target == UINT64_MAX)
+			want = 1 << 20;
+		else if(R.target == UINT64_MAX)
+			want = R.max_traj - R.local_total;
+		else
+		{
+			const double rate = (C.trajectories > 0 && C.captured > 0) ? (double) C.captured / (double) C.trajectories : 0.0;
+			const uint64_t need = R.target - R.captured;
+			want = rate > 0.0 ? (uint64_t) (1.25 * (double) need / rate) + 4096 : 65536;
+			if(!R.unlimited && want > R.max_traj - R.local_total)
+				want = R.max_traj - R.local_total;
+			if(want < R.left_in_round)
+				want = R.left_in_round;
+		}
+		size_t n = (size_t) (want < max_chunk ? want : max_chunk);
+		if(n == 0)
+			n = 1;
+		if(ensure(B, n, nb, want_rows, want_recs) != hipSuccess)
+		{
+			set_err("device allocation for a trajectory chunk failed");
+			break;
+		}
+		const double t0 = now_ms();
+		hipLaunchKernelGGL(dms_ic_kernel, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, ctx->stream, H, d.seed, d.point_id,
+		                   next_id, (uint32_t) n, d.r_max_rsun * DMS_rSun, B.ics, B.ok);
+		float kms = 0.0f;
+		hipError_t e = run_trajectories(ctx, pt, J, B, n, want_rows, want_recs, &kms);
+		if(e != hipSuccess)
+		{
+			set_err(std::string("trajectory kernel: ") + hipGetErrorString(e));
+			break;
+		}
+		job->kernel_ms += kms;
+		job->launches += 1.0;
+		hsum.resize(n);
+		hipMemcpy(hsum.data(), B.summ, sizeof(DmsSummary) * n, hipMemcpyDeviceToHost);
+		if(want_recs)
+		{
+			hrec.resize(n);
+			hipMemcpy(hrec.data(), B.recs, sizeof(dms_traj_record) * n, hipMemcpyDeviceToHost);
+		}
+		simulated += n;
+		next_id += n;
+		hcls.assign(n, 0);
+		size_t counted = 0;
+		const double elapsed_sec = (now_ms() - t_start) * 1e-3;
+		for(size_t i = 0; i < n; i++)
+		{
+			if(R.stopped || !R.open_round())
+				break;
+			const DmsSummary& s = hsum[i];
+			const bool shift_ok = !(s.flags & DMS_F_SHIFT_FAILED);
+			if(!shift_ok)
+			{
+				C.initial_shift_failures++;
+				C.numerical_failures++;
+			}
+			R.local_total++;
+			C.trajectories++;
+			C.total_scatterings += s.n_scat;
+			C.reason_counts[s.reason]++;
+			C.n_rk_attempts += s.n_rk;
+			C.n_steps_interior += s.n_int;
+			C.n_steps_exterior += s.n_ext;
+			C.n_tau_rejections += s.n_rej;
+			C.n_tv_iterations += s.n_tv;
+			if(shift_ok && is_numerical(s.reason))
+				C.numerical_failures++;
+			if(is_truncation(s.reason))
+				C.computational_truncations++;
+			if(s.flags & DMS_F_CAPTURED)
+			{
+				C.captured++;
+				R.captured++;
+				if(!capture)
+				{
+					if(s.flags & DMS_F_EVENT_OBSERVED)
+						C.complete_evaporation++;
+					else
+						C.invalid_survival_captured++;
+					hcls[i] = 1;
+					/* Build_Evaporation_Record + Make_Compact_Evaporation_Event */
+					if(std::isfinite(s.t_capture_sec))
+					{
+						const bool nie = (s.flags & DMS_F_NUMERICALLY_INVALID_ESCAPE) || s.reason == DMS_ENERGY_DRIFT_ESCAPE;
+						const bool sv = (s.flags & DMS_F_SURVIVAL_VALID) && !nie;
+						const bool eo = sv && (s.flags & DMS_F_EVENT_OBSERVED);
+						const double lu = eo ? (s.t_unbind_sec - s.t_capture_sec) : -1.0;
+						const double lifetime = (std::isfinite(lu) && lu >= 0.0) ? lu : -1.0;
+						if(sv && eo && lifetime >= 0.0)
+						{
+							dms_evap_event E;
+							E.rank = 0;
+							E.pad = 0;
+							E.trajectory_id = C.trajectories;
+							E.completion_wall_time_sec = elapsed_sec;
+							E.lifetime_unbinding_sec = lifetime;
+							job->events.push_back(E);
+						}
+					}
+				}
+			}
+			else if(s.reason == DMS_OUTWARD_ESCAPE)
+			{
+				C.completed_outward_escapes++;
+				if(!capture)
+				{
+					hcls[i] = 2;
+					if(s.flags & DMS_F_FREE)
+						C.free_particles++;
+					else if(s.flags & DMS_F_REFLECTED)
+					{
+						C.reflected_particles++;
+						if(s.flags & DMS_F_REFLECTION_SHIFT_FAILED)
+						{
+							C.final_reflection_shift_failures++;
+							C.numerical_failures++;
+						}
+					}
+				}
+			}
+			if(want_recs)
+				job->records.push_back(hrec[i]);
+			counted = i + 1;
+			R.close_one();
+		}
+		if(want_rows && counted > 0)
+		{
+			hipMemcpyAsync(B.cls, hcls.data(), counted, hipMemcpyHostToDevice, ctx->stream);
+			const uint32_t nblk = (uint32_t) ((counted + TB - 1) / TB);
+			const size_t need = (size_t) nblk * nb;
+			if(need > partial_cap)
+			{
+				hipFree(d_partial);
+				hipFree(d_pcount);
+				if(hipMalloc((void**) &d_partial, sizeof(double) * 8 * need) != hipSuccess
+				   || hipMalloc((void**) &d_pcount, sizeof(unsigned long long) * 2 * need) != hipSuccess)
+				{
+					set_err("hipMalloc of histogram partials failed");
+					break;
+				}
+				partial_cap = need;
+			}
+			hipLaunchKernelGGL(dms_hist_partial_kernel, dim3((unsigned) ((nb + 127) / 128), nblk), dim3(128), 0, ctx->stream,
+			                   B.rows, B.counts, B.cls, (uint32_t) counted, nb, TB, d_partial, d_pcount);
+			hipLaunchKernelGGL(dms_hist_final_kernel, dim3((unsigned) ((nb + 127) / 128)), dim3(128), 0, ctx->stream, d_partial,
+			                   d_pcount, nblk, nb, d_hist, d_counts);
+			if(hipStreamSynchronize(ctx->stream) != hipSuccess)
+			{
+				set_err("histogram reduction failed");
+				break;
+			}
+		}
+		job->other_ms += (now_ms() - t0) - kms;
+		job->done_traj = C.trajectories;
+		job->done_captured = C.captured;
+		job->done_scat = C.total_scatterings;
+	}
+	C.capture_target_overshoot = (R.captured > R.target) ? R.captured - R.target : 0;
+	int stop = R.stop;
+	if(R.captured < R.target && stop == 0)
+		stop = 2;
+	C.early_stop_reason = stop;
+	C.simulated_trajectories = simulated;
+	C.computing_time_sec = (now_ms() - t_start) * 1e-3;
+	if(want_rows && job->status == DMS_OK)
+	{
+		hipMemcpy(job->hist.data(), d_hist, sizeof(double) * 8 * nb, hipMemcpyDeviceToHost);
+		hipMemcpy(job->counts.data(), d_counts, sizeof(unsigned long long) * 2 * nb, hipMemcpyDeviceToHost);
+	}
+	hipFree(d_hist);
+	hipFree(d_counts);
+	hipFree(d_partial);
+	hipFree(d_pcount);
+	B.release();
+	job->finished = 1;
+}
+
+} // namespace
+
+/* ======================= C ABI ============================================= */
+
+extern "C" {
+
+int dms_abi_version(void) { return DMS_ABI_VERSION; }
+
+const char* dms_last_error(const dms_ctx* ctx)
+{
+	if(ctx && !ctx->err.empty())
+		return ctx->err.c_str();
+	return g_last_error.c_str();
+}
+
+double dms_constant(const char* name)
+{
+	static const struct
+	{
+		const char* name;
+		double value;
+	} table[] = {{"cm", DMS_cm}, {"meter", DMS_meter}, {"km", DMS_km}, {"sec", DMS_sec}, {"gram", DMS_gram},
+	             {"kg", DMS_kg}, {"Kelvin", DMS_Kelvin}, {"eV", DMS_eV}, {"MeV", DMS_MeV}, {"GeV", DMS_GeV},
+	             {"mProton", DMS_mProton}, {"mElectron", DMS_mElectron}, {"mNucleon", DMS_mNucleon},
+	             {"aEM", DMS_aEM}, {"G_Newton", DMS_G_Newton}, {"mSun", DMS_mSun}, {"rSun", DMS_rSun},
+	             {"AU", DMS_AU}, {"pb", 1.0e-36 * DMS_cm * DMS_cm}};
+	if(name == nullptr)
+		return dm_nan();
+	for(const auto& e : table)
+		if(strcmp(e.name, name) == 0)
+			return e.value;
+	return dm_nan();
+}
+
+int dms_load_solar_file(const char* path, double* rows_out, size_t capacity_rows, size_t* n_rows)
+{
+	if(path == nullptr || n_rows == nullptr)
+		return fail(nullptr, DMS_ERR_ARG, "dms_load_solar_file: null argument");
+	FILE* f = fopen(path, "r");
+	if(f == nullptr)
+		return fail(nullptr, DMS_ERR_ARG, std::string("cannot open solar model file ") + path);
+	char line[4096];
+	size_t rows = 0;
+	int lineno = 0;
+	while(fgets(line, sizeof(line), f))
+	{
+		if(lineno++ < 20)
+			continue;
+		double v[dms::kColumns];
+		int k = 0;
+		char* p = line;
+		while(k < dms::kColumns)
+		{
+			char* end = nullptr;
+			const double x = strtod(p, &end);
+			if(end == p)
+				break;
+			v[k++] = x;
+			p = end;
+		}
+		if(k != dms::kColumns)
+			continue;
+		if(rows_out != nullptr && rows < capacity_rows)
+			memcpy(rows_out + rows * dms::kColumns, v, sizeof(v));
+		rows++;
+	}
+	fclose(f);
+	*n_rows = rows;
+	return DMS_OK;
+}
+
+int dms_create(dms_ctx** out, int device, const double* raw_rows, size_t n_rows)
+{
+	if(out == nullptr)
+		return fail(nullptr, DMS_ERR_ARG, "dms_create: null output handle");
+	*out = nullptr;
+	dms_ctx* ctx = new dms_ctx();
+	const std::string msg = dms::build_solar_tables(raw_rows, n_rows, ctx->T);
+	if(!msg.empty())
+	{
+		delete ctx;
+		return fail(nullptr, DMS_ERR_ARG, "dms_create: " + msg);
+	}
+	ctx->device = device;
+	if(device >= 0)
+	{
+		int count = 0;
+		if(hipGetDeviceCount(&count) != hipSuccess || device >= count)
+		{
+			delete ctx;
+			return fail(nullptr, DMS_ERR_DEVICE, "dms_create: no such HIP device");
+		}
+		auto cleanup = [&](const char* what) {
+			delete ctx;
+			return fail(nullptr, DMS_ERR_DEVICE, std::string("dms_create: ") + what);
+		};
+		if(hipSetDevice(device) != hipSuccess)
+			return cleanup("hipSetDevice failed");
+		if(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
+			return cleanup("hipStreamCreate failed");
+		hipDeviceProp_t prop;
+		if(hipGetDeviceProperties(&prop, device) != hipSuccess)
+			return cleanup("hipGetDeviceProperties failed");
+		ctx->n_cu = prop.multiProcessorCount;
+		int blocks = 0;
+		if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, dms_traj_kernel, DMS_TRAJ_BLOCK, 0) != hipSuccess)
+			blocks = 1;
+		ctx->traj_blocks_per_cu = blocks > 0 ? blocks : 1;
+		const dms::SolarTables& T = ctx->T;
+		if(dev_alloc_copy(&ctx->d_x, T.x.data(), T.x.size()) != hipSuccess
+		   || dev_alloc_copy(&ctx->d_mass, T.mass.data(), T.mass.size()) != hipSuccess
+		   || dev_alloc_copy(&ctx->d_temp, T.temp.data(), T.temp.size()) != hipSuccess
+		   || dev_alloc_copy(&ctx->d_vesc2, T.vesc2.data(), T.vesc2.size()) != hipSuccess
+		   || dev_alloc_copy(&ctx->d_dens, T.dens.data(), T.dens.size()) != hipSuccess
+		   || dev_alloc_copy(&ctx->d_pref, T.iso_pref.data(), T.iso_pref.size()) != hipSuccess
+		   || dev_alloc_copy(&ctx->d_imass, T.iso_mass.data(), T.iso_mass.size()) != hipSuccess
+		   || dev_alloc_copy(&ctx->d_Z, T.iso_Z.data(), T.iso_Z.size()) != hipSuccess)
+			return cleanup("device table upload failed");
+		ctx->dev = T.view();
+		ctx->dev.x = ctx->d_x;
+		ctx->dev.mass = ctx->d_mass;
+		ctx->dev.temp = ctx->d_temp;
+		ctx->dev.vesc2 = ctx->d_vesc2;
+		ctx->dev.dens = ctx->d_dens;
+		ctx->dev.iso_pref = ctx->d_pref;
+		ctx->dev.iso_mass = ctx->d_imass;
+		ctx->dev.iso_Z = ctx->d_Z;
+	}
+	*out = ctx;
+	return DMS_OK;
+}
+
+void dms_destroy(dms_ctx* ctx)
+{
+	if(ctx == nullptr)
+		return;
+	if(ctx->device >= 0)
+	{
+		hipFree(ctx->d_x);
+		hipFree(ctx->d_mass);
+		hipFree(ctx->d_temp);
+		hipFree(ctx->d_vesc2);
+		hipFree(ctx->d_dens);
+		hipFree(ctx->d_pref);
+		hipFree(ctx->d_imass);
+		hipFree(ctx->d_Z);
+		if(ctx->stream)
+			hipStreamDestroy(ctx->stream);
+	}
+	delete ctx;
+}
+
+int dms_solar_eval(const dms_ctx* ctx, int which, const double* r, double* out, size_t n)
+{
+	if(ctx == nullptr || (n > 0 && (r == nullptr || out == nullptr)))
+		return fail(nullptr, DMS_ERR_ARG, "dms_solar_eval: null argument");
+	const DmsSolarView V = ctx->T.view();
+	for(size_t i = 0; i < n; i++)
+	{
+		const double x = r[i];
+		double res;
+		if(which == 0)
+			res = dms_mass(V, x);
+		else if(which == 1)
+			res = dms_temperature(V, x);
+		else if(which == 2)
+			res = dms_local_escape_speed(V, x);
+		else if(which == 3)
+			res = ctx->T.mass_density(x);
+		else if(which == 4 || (which >= 100 && which - 100 < V.n_iso))
+		{
+			const int k = (which == 4) ? DMS_MAX_TARGETS - 1 : which - 100;
+			double xx = x;
+			if(!dm_isfinite(xx))
+				res = dm_nan();
+			else
+			{
+				if(xx < 0.0)
+					xx = 0.0;
+				if(xx > DMS_rSun)
+					res = 0.0;
+				else
+				{
+					const int j = dms_interval(V, xx);
+					res = dms_density_at(V, j, xx - V.x[j], k);
+				}
+			}
+		}
+		else
+			return fail(nullptr, DMS_ERR_ARG, "dms_solar_eval: unknown quantity");
+		out[i] = res;
+	}
+	return DMS_OK;
+}
+
+int dms_isotopes(const dms_ctx* ctx, int32_t* Z, int32_t* A, double* mass, double* abundance, size_t capacity, size_t* n)
+{
+	if(ctx == nullptr || n == nullptr)
+		return fail(nullptr, DMS_ERR_ARG, "dms_isotopes: null argument");
+	*n = (size_t) ctx->T.n_iso;
+	for(size_t i = 0; i < *n && i < capacity; i++)
+	{
+		if(Z)
+			Z[i] = ctx->T.iso_Z[i];
+		if(A)
+			A[i] = ctx->T.iso_A[i];
+		if(mass)
+			mass[i] = ctx->T.iso_mass[i];
+		if(abundance)
+			abundance[i] = ctx->T.iso_pref[i];
+	}
+	return DMS_OK;
+}
+
+int dms_point_create(dms_ctx* ctx, const dms_point_desc* desc, dms_point** out)
+{
+	if(ctx == nullptr || desc == nullptr || out == nullptr)
+		return fail(ctx, DMS_ERR_ARG, "dms_point_create: null argument");
+	*out = nullptr;
+	if(ctx->device < 0)
+		return fail(ctx, DMS_ERR_DEVICE, "dms_point_create: context has no GPU");
+	dms_point* pt = new dms_point();
+	pt->ctx = ctx;
+	pt->desc = *desc;
+	pt->desc.host_rate_grid = nullptr;
+	const std::string msg = dms::build_point_params(ctx->T, *desc, pt->P);
+	if(!msg.empty())
+	{
+		delete pt;
+		const bool unsupported = desc->interaction == DMS_SD || desc->low_mass != 1;
+		return fail(ctx, unsupported ? DMS_ERR_UNSUPPORTED : DMS_ERR_ARG, "dms_point_create: " + msg);
+	}
+	const int N = desc->interpolation_points;
+	if(N > 20000)
+	{
+		delete pt;
+		return fail(ctx, DMS_ERR_ARG, "dms_point_create: interpolation_points too large");
+	}
+	pt->N = (N >= 2) ? N : 0;
+	std::lock_guard<std::mutex> lock(ctx->mu);
+	auto bail = [&](const std::string& what) {
+		hipFree(pt->d_sigmaN);
+		hipFree(pt->d_grid);
+		delete pt;
+		return fail(ctx, DMS_ERR_DEVICE, "dms_point_create: " + what);
+	};
+	if(dev_alloc_copy(&pt->d_sigmaN, pt->P.sigma_N, DMS_MAX_TARGETS) != hipSuccess)
+		return bail("sigma upload failed");
+	pt->dev = host_point_view(pt);
+	pt->dev.sigma_N = pt->d_sigmaN;
+	if(pt->N >= 2)
+	{
+		const size_t cells = (size_t) pt->N * (size_t) pt->N;
+		if(hipMalloc((void**) &pt->d_grid, sizeof(double) * cells) != hipSuccess)
+			return bail("grid allocation failed");
+		if(desc->host_rate_grid != nullptr)
+		{
+			if(hipMemcpy(pt->d_grid, desc->host_rate_grid, sizeof(double) * cells, hipMemcpyHostToDevice) != hipSuccess)
+				return bail("grid upload failed");
+		}
+		else
+		{
+			hipLaunchKernelGGL(dms_grid_kernel, dim3((unsigned) ((cells + 255) / 256)), dim3(256), 0, ctx->stream, pt->dev, ctx->dev,
+			                   pt->N, pt->N, DMS_V_MAX, pt->d_grid);
+			if(hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess)
+				return bail("rate-grid kernel failed");
+		}
+		pt->dev.grid = pt->d_grid;
+	}
+	*out = pt;
+	return DMS_OK;
+}
+
+void dms_point_destroy(dms_point* pt)
+{
+	if(pt == nullptr)
+		return;
+	hipFree(pt->d_sigmaN);
+	hipFree(pt->d_grid);
+	delete pt;
+}
+
+int dms_point_rate_grid(const dms_point* pt, double* out, size_t n)
+{
+	if(pt == nullptr || out == nullptr)
+		return fail(nullptr, DMS_ERR_ARG, "dms_point_rate_grid: null argument");
+	if(pt->N < 2)
+		return fail(pt->ctx, DMS_ERR_STATE, "dms_point_rate_grid: point has no rate grid");
+	const size_t cells = (size_t) pt->N * (size_t) pt->N;
+	if(n < cells)
+		return fail(pt->ctx, DMS_ERR_ARG, "dms_point_rate_grid: output too small");
+	DMS_HIP(pt->ctx, hipMemcpy(out, pt->d_grid, sizeof(double) * cells, hipMemcpyDeviceToHost));
+	return DMS_OK;
+}
+
+int dms_point_cross_sections(const dms_point* pt, double* sigma_p_cm2, double* sigma_e_cm2)
+{
+	if(pt == nullptr)
+		return fail(nullptr, DMS_ERR_ARG, "dms_point_cross_sections: null argument");
+	if(sigma_p_cm2)
+		*sigma_p_cm2 = pt->P.sigma_p / (DMS_cm * DMS_cm);
+	if(sigma_e_cm2)
+		*sigma_e_cm2 = pt->P.sigma_e / (DMS_cm * DMS_cm);
+	return DMS_OK;
+}
+
+int dms_point_rates(const dms_point* pt, int which, const double* r, const double* v, double* out, size_t n)
+{
+	if(pt == nullptr || (n > 0 && (r == nullptr || v == nullptr || out == nullptr)))
+		return fail(nullptr, DMS_ERR_ARG, "dms_point_rates: null argument");
+	if(which < 0 || which > 2 + pt->ctx->T.n_iso)
+		return fail(pt->ctx, DMS_ERR_ARG, "dms_point_rates: unknown rate");
+	if(n == 0)
+		return DMS_OK;
+	dms_ctx* ctx = pt->ctx;
+	std::lock_guard<std::mutex> lock(ctx->mu);
+	double *dr = nullptr, *dv = nullptr, *dout = nullptr;
+	DMS_HIP(ctx, hipMalloc((void**) &dr, sizeof(double) * n));
+	DMS_HIP(ctx, hipMalloc((void**) &dv, sizeof(double) * n));
+	DMS_HIP(ctx, hipMalloc((void**) &dout, sizeof(double) * n));
+	hipMemcpy(dr, r, sizeof(double) * n, hipMemcpyHostToDevice);
+	hipMemcpy(dv, v, sizeof(double) * n, hipMemcpyHostToDevice);
+	hipLaunchKernelGGL(dms_rates_kernel, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, ctx->stream, pt->dev, ctx->dev, which, dr, dv,
+	                   dout, n);
+	hipError_t e = hipGetLastError();
+	if(e == hipSuccess)
+		e = hipStreamSynchronize(ctx->stream);
+	if(e == hipSuccess)
+		e = hipMemcpy(out, dout, sizeof(double) * n, hipMemcpyDeviceToHost);
+	hipFree(dr);
+	hipFree(dv);
+	hipFree(dout);
+	if(e != hipSuccess)
+		return fail(ctx, DMS_ERR_DEVICE, std::string("dms_point_rates: ") + hipGetErrorString(e));
+	return DMS_OK;
+}
+
+int dms_initial_conditions(dms_ctx* ctx, const dms_halo* halo, uint32_t seed, uint32_t point_id, uint64_t first_id, size_t n,
+                           double r_final_rsun, dms_event* out, int32_t* shift_ok)
+{
+	if(ctx == nullptr || halo == nullptr || (n > 0 && out == nullptr))
+		return fail(ctx, DMS_ERR_ARG, "dms_initial_conditions: null argument");
+	if(ctx->device < 0)
+		return fail(ctx, DMS_ERR_DEVICE, "dms_initial_conditions: context has no GPU");
+	const std::string hm = check_halo(*halo);
+	if(!hm.empty())
+		return fail(ctx, DMS_ERR_ARG, "dms_initial_conditions: " + hm);
+	if(n == 0)
+		return DMS_OK;
+	if(n > 0xffffffffu)
+		return fail(ctx, DMS_ERR_ARG, "dms_initial_conditions: too many events");
+	std::lock_guard<std::mutex> lock(ctx->mu);
+	const DmsHaloView H = halo_view(ctx, *halo);
+	dms_event* d_ev = nullptr;
+	int32_t* d_ok = nullptr;
+	DMS_HIP(ctx, hipMalloc((void**) &d_ev, sizeof(dms_event) * n));
+	DMS_HIP(ctx, hipMalloc((void**) &d_ok, sizeof(int32_t) * n));
+	const double R = (r_final_rsun > 0.0) ? r_final_rsun * DMS_rSun : 0.0;
+	hipLaunchKernelGGL(dms_ic_kernel, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, ctx->stream, H, seed, point_id, first_id,
+	                   (uint32_t) n, R, d_ev, d_ok);
+	hipError_t e = hipGetLastError();
+	if(e == hipSuccess)
+		e = hipStreamSynchronize(ctx->stream);
+	if(e == hipSuccess)
+		e = hipMemcpy(out, d_ev, sizeof(dms_event) * n, hipMemcpyDeviceToHost);
+	if(e == hipSuccess && shift_ok)
+		e = hipMemcpy(shift_ok, d_ok, sizeof(int32_t) * n, hipMemcpyDeviceToHost);
+	hipFree(d_ev);
+	hipFree(d_ok);
+	if(e != hipSuccess)
+		return fail(ctx, DMS_ERR_DEVICE, std::string("dms_initial_conditions: ") + hipGetErrorString(e));
+	return DMS_OK;
+}
+
+int dms_simulate_events(dms_point* pt, const dms_job_desc* job, const dms_event* initial, size_t n, dms_traj_record* records,
+                        double* hist_rows)
+{
+	if(pt == nullptr || job == nullptr || (n > 0 && (initial == nullptr || records == nullptr)))
+		return fail(pt ? pt->ctx : nullptr, DMS_ERR_ARG, "dms_simulate_events: null argument");
+	dms_ctx* ctx = pt->ctx;
+	const std::string jm = check_job(*job);
+	if(!jm.empty())
+		return fail(ctx, DMS_ERR_ARG, "dms_simulate_events: " + jm);
+	if(n == 0)
+		return DMS_OK;
+	if(n > 0x7fffffffu)
+		return fail(ctx, DMS_ERR_ARG, "dms_simulate_events: too many events");
+	for(size_t i = 0; i < n; i++)
+	{
+		/* Simulate's argument check, src/Simulation_Trajectory.cpp:1278-1283 */
+		const double r = std::sqrt(initial[i].r[0] * initial[i].r[0] + initial[i].r[1] * initial[i].r[1] + initial[i].r[2] * initial[i].r[2]);
+		const double v = std::sqrt(initial[i].v[0] * initial[i].v[0] + initial[i].v[1] * initial[i].v[1] + initial[i].v[2] * initial[i].v[2]);
+		if(!std::isfinite(initial[i].t) || initial[i].t < 0.0 || !std::isfinite(r) || r <= 0.0 || !std::isfinite(v) || v <= 0.0)
+			return fail(ctx, DMS_ERR_ARG, "dms_simulate_events: initial state is invalid (Trajectory_Simulator::Simulate)");
+	}
+	std::lock_guard<std::mutex> lock(ctx->mu);
+	const DmsJobView J = job_view(*job);
+	const bool rows = hist_rows != nullptr && !J.terminate_on_capture;
+	ChunkBuffers B;
+	if(ensure(B, n, J.nbins, rows, true) != hipSuccess)
+	{
+		B.release();
+		return fail(ctx, DMS_ERR_NOMEM, "dms_simulate_events: device allocation failed");
+	}
+	hipMemcpy(B.ics, initial, sizeof(dms_event) * n, hipMemcpyHostToDevice);
+	hipMemset(B.ok, 0xff, sizeof(int32_t) * n);
+	float ms = 0.0f;
+	hipError_t e = run_trajectories(ctx, pt, J, B, n, rows, true, &ms);
+	if(e == hipSuccess)
+		e = hipMemcpy(records, B.recs, sizeof(dms_traj_record) * n, hipMemcpyDeviceToHost);
+	if(e == hipSuccess && hist_rows)
+	{
+		if(rows)
+			e = hipMemcpy(hist_rows, B.rows, sizeof(double) * 2 * (size_t) J.nbins * n, hipMemcpyDeviceToHost);
+		else
+			memset(hist_rows, 0, sizeof(double) * 2 * (size_t) J.nbins * n);
+	}
+	B.release();
+	if(e != hipSuccess)
+		return fail(ctx, DMS_ERR_DEVICE, std::string("dms_simulate_events: ") + hipGetErrorString(e));
+	return DMS_OK;
+}
+
+int dms_job_submit(dms_point* pt, const dms_job_desc* desc, dms_job** out)
+{
+	if(pt == nullptr || desc == nullptr || out == nullptr)
+		return fail(pt ? pt->ctx : nullptr, DMS_ERR_ARG, "dms_job_submit: null argument");
+	*out = nullptr;
+	std::string m = check_job(*desc);
+	if(m.empty())
+		m = check_halo(desc->halo);
+	if(!m.empty())
+		return fail(pt->ctx, DMS_ERR_ARG, "dms_job_submit: " + m);
+	if(desc->max_trajectories == 0 && desc->capture_target == 0)
+		return fail(pt->ctx, DMS_ERR_ARG, "dms_job_submit: a job needs max_trajectories or capture_target");
+	dms_job* job = new dms_job();
+	job->pt = pt;
+	job->desc = *desc;
+	memset(&job->C, 0, sizeof(job->C));
+	job->worker = std::thread(run_job, job);
+	*out = job;
+	return DMS_OK;
+}
+
+int dms_job_poll(dms_job* job, dms_job_status* st)
+{
+	if(job == nullptr || st == nullptr)
+		return fail(nullptr, DMS_ERR_ARG, "dms_job_poll: null argument");
+	st->trajectories_done = job->done_traj.load();
+	st->captured = job->done_captured.load();
+	st->scatterings = job->done_scat.load();
+	st->done = job->finished.load();
+	st->pad = 0;
+	return DMS_OK;
+}
+
+int dms_job_wait(dms_job* job, dms_job_status* st)
+{
+	if(job == nullptr)
+		return fail(nullptr, DMS_ERR_ARG, "dms_job_wait: null argument");
+	if(job->worker.joinable())
+		job->worker.join();
+	if(st)
+		dms_job_poll(job, st);
+	if(job->status != DMS_OK)
+		return fail(job->pt->ctx, job->status, "job failed: " + job->err);
+	return DMS_OK;
+}
+
+int dms_job_read(dms_job* job, dms_counters* counters, double* hist, uint64_t* counts, size_t n_bins, dms_evap_event* events,
+                 size_t event_capacity, size_t* n_events, dms_traj_record* records, size_t record_capacity, size_t* n_records)
+{
+	if(job == nullptr)
+		return fail(nullptr, DMS_ERR_ARG, "dms_job_read: null job");
+	if(!job->finished.load())
+		return fail(job->pt->ctx, DMS_ERR_STATE, "dms_job_read: job has not finished (call dms_job_wait)");
+	if(job->status != DMS_OK)
+		return fail(job->pt->ctx, job->status, "dms_job_read: job failed: " + job->err);
+	if(counters)
+		*counters = job->C;
+	const size_t nb = (size_t) job_view(job->desc).nbins;
+	if(hist || counts)
+	{
+		if(n_bins != nb)
+			return fail(job->pt->ctx, DMS_ERR_ARG, "dms_job_read: n_bins does not match the job");
+		if(hist)
+			memcpy(hist, job->hist.data(), sizeof(double) * 8 * nb);
+		if(counts)
+			for(size_t i = 0; i < 2 * nb; i++)
+				counts[i] = job->counts[i];
+	}
+	if(n_events)
+		*n_events = job->events.size();
+	if(events)
+		for(size_t i = 0; i < job->events.size() && i < event_capacity; i++)
+			events[i] = job->events[i];
+	if(n_records)
+		*n_records = job->records.size();
+	if(records)
+		for(size_t i = 0; i < job->records.size() && i < record_capacity; i++)
+			records[i] = job->records[i];
+	return DMS_OK;
+}
+
+int dms_job_timing(const dms_job* job, double* trajectory_kernel_ms, double* launches, double* other_ms)
+{
+	if(job == nullptr)
+		return fail(nullptr, DMS_ERR_ARG, "dms_job_timing: null job");
+	if(trajectory_kernel_ms)
+		*trajectory_kernel_ms = job->kernel_ms;
+	if(launches)
+		*launches = job->launches;
+	if(other_ms)
+		*other_ms = job->other_ms;
+	return DMS_OK;
+}
+
+void dms_job_destroy(dms_job* job)
+{
+	if(job == nullptr)
+		return;
+	if(job->worker.joinable())
+		job->worker.join();
+	delete job;
+}
+
+int dms_probe_math(dms_ctx* ctx, int fn, const double* x, const double* y, double* out, size_t n)
+{
+	if(ctx == nullptr || (n > 0 && (x == nullptr || out == nullptr)) || fn < 0 || fn > 7)
+		return fail(ctx, DMS_ERR_ARG, "dms_probe_math: bad argument");
+	if(ctx->device < 0)
+		return fail(ctx, DMS_ERR_DEVICE, "dms_probe_math: context has no GPU");
+	if(n == 0)
+		return DMS_OK;
+	std::lock_guard<std::mutex> lock(ctx->mu);
+	double *dx = nullptr, *dy = nullptr, *dout = nullptr;
+	DMS_HIP(ctx, hipMalloc((void**) &dx, sizeof(double) * n));
+	DMS_HIP(ctx, hipMalloc((void**) &dy, sizeof(double) * n));
+	DMS_HIP(ctx, hipMalloc((void**) &dout, sizeof(double) * n));
+	hipMemcpy(dx, x, sizeof(double) * n, hipMemcpyHostToDevice);
+	hipMemcpy(dy, y ? y : x, sizeof(double) * n, hipMemcpyHostToDevice);
+	hipLaunchKernelGGL(dms_probe_kernel, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, ctx->stream, fn, dx, dy, dout, n);
+	hipError_t e = hipGetLastError();
+	if(e == hipSuccess)
+		e = hipStreamSynchronize(ctx->stream);
+	if(e == hipSuccess)
+		e = hipMemcpy(out, dout, sizeof(double) * n, hipMemcpyDeviceToHost);
+	hipFree(dx);
+	hipFree(dy);
+	hipFree(dout);
+	if(e != hipSuccess)
+		return fail(ctx, DMS_ERR_DEVICE, std::string("dms_probe_math: ") + hipGetErrorString(e));
+	return DMS_OK;
+}
+
+} /* extern "C" */

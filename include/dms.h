@@ -1,0 +1,308 @@
+/*
+ * dms.h -- C ABI of the MI355X-native solar dark-matter trajectory engine.
+ *
+ * This is the drop-in boundary for the per-trajectory Monte Carlo path of
+ * DaMaSCUS-SUN-EVAP.  In the reference the path sits behind C++ classes
+ * (no FFI): Solar_Model, Trajectory_Simulator and Simulation_Data.  Each entry
+ * point below names the reference interface it replaces (file:line, relative
+ * to the reference repository root).
+ *
+ * Conventions
+ *   - Plain C types only; no exceptions cross this boundary.
+ *   - Return value: DMS_OK (0) on success, a negative DMS_ERR_* code on an API
+ *     or device error; dms_last_error() gives the message.
+ *   - Physical inputs and outputs use the reference's natural units
+ *     (GeV, hbar = c = k_B = 1) unless a field name says otherwise
+ *     (_GeV, _cm2, _kms, _sec, _km, _MeV, _rsun).
+ *   - Per-trajectory numerical trouble is an outcome (dms_reason), not an
+ *     error, exactly as TrajectoryTerminationReason in the reference
+ *     (include/Simulation_Trajectory.hpp:35-48).
+ *   - Caller owns every array it passes; inputs are copied during the call.
+ *     The library owns device memory.  A context is bound to one GPU and may be
+ *     used from several host threads: calls on one context are serialised.
+ */
+#ifndef DMS_H
+#define DMS_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DMS_ABI_VERSION 1
+
+enum dms_status
+{
+	DMS_OK = 0,
+	DMS_ERR_ARG = -1,         /* invalid argument (reference: std::invalid_argument) */
+	DMS_ERR_DEVICE = -2,      /* HIP runtime error */
+	DMS_ERR_NOMEM = -3,       /* device or host allocation failed */
+	DMS_ERR_UNSUPPORTED = -4, /* option outside this build (e.g. SD couplings) */
+	DMS_ERR_STATE = -5        /* call order violated (e.g. read before wait) */
+};
+
+/* TrajectoryTerminationReason, include/Simulation_Trajectory.hpp:35-48 */
+enum dms_reason
+{
+	DMS_UNKNOWN = 0,
+	DMS_OUTWARD_ESCAPE = 1,
+	DMS_SCATTER = 2,
+	DMS_WALL_TIME_LIMIT = 3, /* here: per-trajectory RK-attempt budget exhausted */
+	DMS_MAX_FREE_STEPS = 4,
+	DMS_MAX_SCATTERINGS = 5,
+	DMS_NON_FINITE_STATE = 6,
+	DMS_SPEED_LIMIT = 7,
+	DMS_NUMERICAL_FAILURE = 8,
+	DMS_CAPTURE_MODE = 9,
+	DMS_ENERGY_DRIFT_ESCAPE = 10
+};
+
+enum dms_interaction
+{
+	DMS_SI = 0,         /* obscura::DM_Particle_SI, low-mass mode */
+	DMS_SD = 1,         /* not supported in this build (needs nuclear spin data) */
+	DMS_DARK_PHOTON = 2 /* src/Dark_Photon.cpp */
+};
+
+enum dms_form_factor
+{
+	DMS_FF_CONTACT = 0,
+	DMS_FF_GENERAL = 1
+};
+
+enum dms_mode
+{
+	DMS_MODE_CAPTURE = 0,        /* run_mode "Capture" / capture_mode = true */
+	DMS_MODE_PARAMETER_POINT = 1 /* run_mode "Parameter point" */
+};
+
+enum dms_precision
+{
+	DMS_FP64 = 0, /* plain FP64, bit-exact against the CPU restatement */
+	DMS_MIXED = 1 /* FP32 rates/target sampling/kinematics, FP64 orbit and energy */
+};
+
+/* Event: src/Simulation_Utilities.cpp:47-66 (time, position, velocity). */
+typedef struct
+{
+	double t;
+	double r[3];
+	double v[3];
+} dms_event;
+
+/* obscura Standard_Halo_Model parameters (config keys DM_local_density,
+ * SHM_v0, SHM_vObserver, SHM_vEscape; src/Parameter_Scan.cpp via obscura). */
+typedef struct
+{
+	double rho_GeV_cm3;
+	double v0_kms;
+	double vobs_kms[3];
+	double vesc_kms;
+} dms_halo;
+
+/* DM particle and rate-table request (src/Parameter_Scan.cpp:386-520,
+ * src/Dark_Photon.cpp:123-213, src/Solar_Model.cpp:395-465). */
+typedef struct
+{
+	double mass_GeV;
+	double sigma_p_cm2;        /* DM_cross_section_nucleon (SI); ignored for DP */
+	double sigma_e_cm2;        /* DM_cross_section_electron */
+	int32_t interaction;       /* dms_interaction */
+	int32_t form_factor;       /* dms_form_factor */
+	double mediator_mass_MeV;  /* DP "General" only */
+	double fn_over_fp;         /* SI neutron/proton coupling ratio (1 = isospin conserving) */
+	int32_t low_mass;          /* DM_light; only 1 is supported */
+	int32_t interpolation_points; /* N for an N x N rate grid; < 2 disables the grid */
+	const double* host_rate_grid; /* optional N*N radius-major grid to upload instead of building it */
+} dms_point_desc;
+
+/* One batch job: Simulation_Data::Generate_Data (include/Data_Generation.hpp:134). */
+typedef struct
+{
+	int32_t mode;                 /* dms_mode */
+	int32_t precision;            /* dms_precision */
+	uint64_t max_trajectories;    /* 0 = unlimited (config key max_trajectories) */
+	uint64_t capture_target;      /* sample_size; 0 = run exactly max_trajectories */
+	uint32_t seed;                /* fixed_seed (Philox key word 0) */
+	uint32_t point_id;            /* Philox key word 1 */
+	uint64_t first_trajectory_id; /* id of the first trajectory (counter words 0-1) */
+	uint64_t max_scatterings;     /* maximum_number_of_scatterings */
+	uint64_t max_free_steps;      /* maximum_time_steps of Trajectory_Simulator */
+	uint64_t rk_attempt_budget;   /* wall-time analogue per trajectory; 0 = none */
+	double r_max_rsun;            /* initial_and_final_radius / rSun (2.0 in main.cpp:72) */
+	int32_t n_bins;               /* radial bins over [0, 2 R_sun); 2000 in the reference */
+	int32_t record_trajectories;  /* keep one dms_traj_record per trajectory */
+	dms_halo halo;
+} dms_job_desc;
+
+/* Per-trajectory outcome (Trajectory_Result + TrajectoryBincount scalars,
+ * include/Simulation_Trajectory.hpp:53-114). */
+typedef struct
+{
+	dms_event initial;   /* after the hyperbolic Kepler shift to R_max */
+	dms_event final;
+	uint64_t trajectory_id;
+	uint64_t n_scatterings;
+	int32_t reason;      /* dms_reason */
+	uint32_t flags;      /* DMS_F_* */
+	double t_capture_sec;
+	double t_last_bound_sec;
+	double t_final_unbinding_scatter_sec; /* NaN when unset */
+	double t_boundary_escape_sec;         /* NaN when unset */
+	double t_termination_sec;
+	double r_first_negative_km;
+	double E_first_negative_eV;
+	double dE_first_negative_from_prev_eV;
+	double max_free_energy_drift_eV;
+	double max_free_energy_drift_rel;
+	/* work counters (algorithmic-op accounting) */
+	uint64_t n_rk_attempts;     /* RKF45 stage evaluations (inner attempts) */
+	uint64_t n_steps_interior;  /* accepted steps with an optical-depth evaluation */
+	uint64_t n_steps_exterior;  /* accepted steps outside the Sun */
+	uint64_t n_tau_rejections;  /* optical-depth rejections */
+	uint64_t n_tv_iterations;   /* Romano-Walsh rejection iterations */
+} dms_traj_record;
+
+#define DMS_F_CAPTURED 0x1u
+#define DMS_F_EVENT_OBSERVED 0x2u
+#define DMS_F_BOUNDARY_ESCAPE_OBSERVED 0x4u
+#define DMS_F_SURVIVAL_VALID 0x8u
+#define DMS_F_NUMERICALLY_INVALID_ESCAPE 0x10u
+#define DMS_F_TRUNCATED 0x20u
+#define DMS_F_SHIFT_FAILED 0x40u     /* initial Hyperbolic_Kepler_Shift failed */
+#define DMS_F_FREE 0x80u             /* Trajectory_Result::Particle_Free */
+#define DMS_F_REFLECTED 0x100u       /* Trajectory_Result::Particle_Reflected */
+#define DMS_F_REFLECTION_SHIFT_FAILED 0x200u /* Hyperbolic_Kepler_Shift to 1 AU failed */
+
+/* Run counters (src/Data_Generation.cpp:437-448, 974-1024). */
+typedef struct
+{
+	uint64_t trajectories;
+	uint64_t captured;
+	uint64_t completed_outward_escapes;
+	uint64_t free_particles;
+	uint64_t reflected_particles;
+	uint64_t complete_evaporation;
+	uint64_t invalid_survival_captured;
+	uint64_t initial_shift_failures;
+	uint64_t final_reflection_shift_failures;
+	uint64_t numerical_failures;
+	uint64_t computational_truncations;
+	uint64_t total_scatterings;
+	uint64_t sync_rounds;
+	uint64_t final_round_trajectories;
+	uint64_t capture_target_overshoot;
+	int32_t early_stop_reason;    /* SimulationStopReason, include/Data_Generation.hpp:56-63 */
+	int32_t sync_interval;        /* normal_mode_mpi_sync_interval analogue */
+	double computing_time_sec;    /* device time of the generate loop */
+	uint64_t reason_counts[11];   /* trajectories per dms_reason */
+	uint64_t n_rk_attempts, n_steps_interior, n_steps_exterior, n_tau_rejections, n_tv_iterations;
+	uint64_t simulated_trajectories; /* including speculative ones past the stop point */
+} dms_counters;
+
+/* CompactEvaporationEvent, include/Data_Generation.hpp:48-54 */
+typedef struct
+{
+	int32_t rank;
+	int32_t pad;
+	uint64_t trajectory_id;
+	double completion_wall_time_sec;
+	double lifetime_unbinding_sec;
+} dms_evap_event;
+
+typedef struct
+{
+	uint64_t trajectories_done;
+	uint64_t captured;
+	uint64_t scatterings;
+	int32_t done;
+	int32_t pad;
+} dms_job_status;
+
+typedef struct dms_ctx dms_ctx;
+typedef struct dms_point dms_point;
+typedef struct dms_job dms_job;
+
+/* ---- library / context ------------------------------------------------- */
+
+int dms_abi_version(void);
+
+/* Solar_Model::Solar_Model + Import_Raw_Data (src/Solar_Model.cpp:43-166).
+ * raw_rows: n_rows x 35 data rows of model_agss09.dat in file units
+ * (M/M_sun, r/R_sun, T[K], rho[g/cm^3], P, L, 29 mass fractions).
+ * device < 0 creates a host-only context (tables and CPU queries only). */
+int dms_create(dms_ctx** ctx, int device, const double* raw_rows, size_t n_rows);
+void dms_destroy(dms_ctx* ctx);
+const char* dms_last_error(const dms_ctx* ctx);
+
+/* Parse the data rows of a model_agss09.dat-format file (20 header lines). */
+int dms_load_solar_file(const char* path, double* rows_out, size_t capacity_rows, size_t* n_rows);
+
+/* Solar_Model queries, host-side tables (src/Solar_Model.cpp:168-261).
+ * which: 0 Mass, 1 Temperature, 2 Local_Escape_Speed, 3 Mass_Density,
+ *        4 Number_Density_Electron, 100+i Number_Density_Nucleus(i). */
+int dms_solar_eval(const dms_ctx* ctx, int which, const double* r, double* out, size_t n);
+/* Target isotope list (src/Solar_Model.cpp:143-163). */
+int dms_isotopes(const dms_ctx* ctx, int32_t* Z, int32_t* A, double* mass, double* abundance, size_t capacity, size_t* n);
+/* Named constant of the unit system (e.g. "rSun", "km", "sec"). */
+double dms_constant(const char* name);
+
+/* ---- parameter points ---------------------------------------------------- */
+
+/* DM particle + Solar_Model::Interpolate_Total_DM_Scattering_Rate
+ * (src/Solar_Model.cpp:395-465): builds the radius-major N x N grid on the GPU
+ * (or uploads host_rate_grid). */
+int dms_point_create(dms_ctx* ctx, const dms_point_desc* desc, dms_point** point);
+void dms_point_destroy(dms_point* point);
+/* Copy the rate grid (N*N doubles, radius-major) to host. */
+int dms_point_rate_grid(const dms_point* point, double* out, size_t n);
+/* Effective proton/electron cross sections in cm^2 (DP derives sigma_p). */
+int dms_point_cross_sections(const dms_point* point, double* sigma_p_cm2, double* sigma_e_cm2);
+/* Solar_Model::Total_DM_Scattering_Rate (+_Computed, per-target rates) evaluated
+ * on the device for n (r, v) pairs.  which: 0 total (grid when enabled),
+ * 1 computed, 2 electron, 3+i nucleus i. */
+int dms_point_rates(const dms_point* point, int which, const double* r, const double* v, double* out, size_t n);
+
+/* ---- trajectory batches --------------------------------------------------- */
+
+/* Initial_Conditions + Hyperbolic_Kepler_Shift (src/Simulation_Utilities.cpp:176-321,
+ * src/Data_Generation.cpp:747-748) on the device for ids first_id .. first_id+n-1. */
+int dms_initial_conditions(dms_ctx* ctx, const dms_halo* halo, uint32_t seed, uint32_t point_id,
+                           uint64_t first_id, size_t n, double r_final_rsun,
+                           dms_event* out, int32_t* shift_ok);
+
+/* Trajectory_Simulator::Simulate (src/Simulation_Trajectory.cpp:1276-1400) for a
+ * batch of given initial events; trajectory i uses Philox counter id first_id+i.
+ * hist_rows (optional): n x 2 x n_bins doubles (dt then v2dt per trajectory). */
+int dms_simulate_events(dms_point* point, const dms_job_desc* job, const dms_event* initial, size_t n,
+                        dms_traj_record* records, double* hist_rows);
+
+/* ---- jobs (Simulation_Data::Generate_Data, src/Data_Generation.cpp:488-972) -- */
+
+int dms_job_submit(dms_point* point, const dms_job_desc* desc, dms_job** job);
+int dms_job_poll(dms_job* job, dms_job_status* status);
+int dms_job_wait(dms_job* job, dms_job_status* status);
+/* hist: 8 x n_bins doubles in the order of bincount.txt columns
+ *   cap_dt, cap_v2dt, cap_dt_sq, cap_v2dt_sq, nc_dt, nc_v2dt, nc_dt_sq, nc_v2dt_sq
+ * (src/Data_Generation.cpp:789-795,816-822); counts: 2 x n_bins visit counts
+ * (captured, not captured).  records (optional) need capacity >= trajectories. */
+int dms_job_read(dms_job* job, dms_counters* counters, double* hist, uint64_t* counts, size_t n_bins,
+                 dms_evap_event* events, size_t event_capacity, size_t* n_events,
+                 dms_traj_record* records, size_t record_capacity, size_t* n_records);
+/* Device timing of the last job: total kernel time and per-kernel average. */
+int dms_job_timing(const dms_job* job, double* trajectory_kernel_ms, double* launches, double* other_ms);
+void dms_job_destroy(dms_job* job);
+
+/* ---- diagnostics ----------------------------------------------------------- */
+
+/* Evaluate the deterministic elementary functions on the device for n inputs.
+ * fn: 0 sqrt, 1 div (x/y), 2 exp, 3 log, 4 sin, 5 cos, 6 erf, 7 acos. */
+int dms_probe_math(dms_ctx* ctx, int fn, const double* x, const double* y, double* out, size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

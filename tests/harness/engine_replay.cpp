@@ -1,0 +1,215 @@
+/*
+ * Host replay of the trajectory kernel's per-lane phase functions
+ * (TEST INFRASTRUCTURE).
+ *
+ * Runs dms_lane_advance_a / finish_scatter / finish / start / advance_b from
+ * damascus-earth-evap_amd/csrc/dms_engine.h one lane at a time on the CPU, with
+ * the wave-cooperative Sample_Target replaced by its serial form
+ * (dms_choose_target), the IC kernel's dms_initial_condition and the grid
+ * kernel's dms_rate_computed.  The CPU tests compare its output with the
+ * oracle bit for bit, which checks the kernel code paths without a GPU.
+ */
+#include <cstring>
+#include <vector>
+
+#include "../../damascus-earth-evap_amd/csrc/dms_engine.h"
+#include "../../damascus-earth-evap_amd/csrc/dms_halo.h"
+#include "../../damascus-earth-evap_amd/csrc/dms_host.hpp"
+
+struct rp_ctx
+{
+	dms::SolarTables T;
+};
+
+struct rp_point
+{
+	rp_ctx* ctx;
+	dms::PointParams P;
+	std::vector<double> grid;
+	int N;
+	DmsPointView view;
+};
+
+extern "C" {
+
+rp_ctx* rp_create(const double* rows, size_t n_rows)
+{
+	rp_ctx* c = new rp_ctx();
+	if(!dms::build_solar_tables(rows, n_rows, c->T).empty())
+	{
+		delete c;
+		return nullptr;
+	}
+	return c;
+}
+
+void rp_free(rp_ctx* c) { delete c; }
+
+double rp_solar_eval(rp_ctx* c, int which, double r)
+{
+	const DmsSolarView V = c->T.view();
+	switch(which)
+	{
+		case 0: return dms_mass(V, r);
+		case 1: return dms_temperature(V, r);
+		case 2: return dms_local_escape_speed(V, r);
+		case 3: return c->T.mass_density(r);
+		default:
+		{
+			const int k = (which == 4) ? DMS_MAX_TARGETS - 1 : which - 100;
+			if(!dm_isfinite(r))
+				return dm_nan();
+			if(r < 0.0)
+				r = 0.0;
+			if(r > DMS_rSun)
+				return 0.0;
+			const int j = dms_interval(V, r);
+			return dms_density_at(V, j, r - V.x[j], k);
+		}
+	}
+}
+
+rp_point* rp_point_create(rp_ctx* c, const dms_point_desc* d)
+{
+	rp_point* p = new rp_point();
+	p->ctx = c;
+	if(!dms::build_point_params(c->T, *d, p->P).empty())
+	{
+		delete p;
+		return nullptr;
+	}
+	p->N = d->interpolation_points >= 2 ? d->interpolation_points : 0;
+	DmsPointView& v = p->view;
+	v.mass = p->P.mass;
+	v.mu_e = p->P.mu_e;
+	v.interaction = p->P.interaction;
+	v.ff = p->P.ff;
+	v.m_dark_photon = p->P.m_dark_photon;
+	v.q_reference = p->P.q_reference;
+	v.sigma_e = p->P.sigma_e;
+	v.sigma_N = p->P.sigma_N;
+	v.use_grid = p->N >= 2;
+	v.Nr = p->N;
+	v.Nv = p->N;
+	v.inv_dr = p->N >= 2 ? (double) (p->N - 1) / DMS_rSun : 0.0;
+	v.inv_dv = p->N >= 2 ? (double) (p->N - 1) / DMS_V_MAX : 0.0;
+	v.vmax = p->N >= 2 ? DMS_V_MAX : 0.0;
+	v.grid = nullptr;
+	if(p->N >= 2)
+	{
+		if(d->host_rate_grid)
+			p->grid.assign(d->host_rate_grid, d->host_rate_grid + (size_t) p->N * p->N);
+		else
+		{
+			/* same node construction as dms_grid_kernel */
+			p->grid.resize((size_t) p->N * p->N);
+			const DmsSolarView S = c->T.view();
+			const double dv = (DMS_V_MAX - 0.0) / (double) (p->N - 1);
+			for(int i = 0; i < p->N; i++)
+				for(int j = 0; j < p->N; j++)
+				{
+					const double radius = DMS_rSun * (double) i / (double) (p->N - 1);
+					const double speed = 0.0 + j * dv;
+					p->grid[(size_t) i * p->N + j] = dms_rate_computed(v, S, radius, speed);
+				}
+		}
+		v.grid = p->grid.data();
+	}
+	return p;
+}
+
+void rp_point_free(rp_point* p) { delete p; }
+
+double rp_rate(rp_point* p, int which, double r, double v)
+{
+	const DmsSolarView S = p->ctx->T.view();
+	if(which == 0)
+		return dms_rate_total(p->view, S, r, v);
+	if(which == 1)
+		return dms_rate_computed(p->view, S, r, v);
+	if(which == 2)
+		return dms_rate_target(p->view, S, r, v, DMS_MAX_TARGETS - 1);
+	return dms_rate_target(p->view, S, r, v, which - 3);
+}
+
+int rp_grid(rp_point* p, double* out, size_t n)
+{
+	if(n < p->grid.size())
+		return -1;
+	memcpy(out, p->grid.data(), sizeof(double) * p->grid.size());
+	return 0;
+}
+
+void rp_initial_conditions(rp_ctx* c, const dms_halo* h, uint32_t seed, uint32_t point_id, uint64_t first_id, size_t n,
+                           double r_final_rsun, dms_event* out, int32_t* ok)
+{
+	DmsHaloView H;
+	dms_halo_init(H, *h);
+	const DmsSolarView V = c->T.view();
+	H.vescR = dms_local_escape_speed(V, DMS_rSun);
+	H.vesc_asym = dms_local_escape_speed(V, 1000.0 * DMS_AU);
+	H.norm = dms_shm_moment(H, 1) + H.vescR * H.vescR * dms_shm_moment(H, -1);
+	for(size_t i = 0; i < n; i++)
+	{
+		dm_rng rng;
+		dm_rng_init(&rng, seed, point_id, first_id + i, DMS_STREAM_IC);
+		dms_initial_condition(H, &rng, &out[i]);
+		ok[i] = (r_final_rsun > 0.0) ? (dms_kepler_shift(out[i].r, out[i].v, r_final_rsun * DMS_rSun) ? 1 : 0) : 1;
+	}
+}
+
+/* Serial replay of dms_traj_kernel for n given initial events. */
+int rp_simulate(rp_point* p, const dms_job_desc* job, const dms_event* ics, size_t n, dms_traj_record* recs, double* rows)
+{
+	DmsLaneCtx c;
+	c.S = p->ctx->T.view();
+	c.P = p->view;
+	DmsJobView& J = c.J;
+	J.max_time_steps = job->max_free_steps;
+	J.max_scatterings = job->max_scatterings;
+	J.rk_budget = job->rk_attempt_budget;
+	J.max_distance = job->r_max_rsun * DMS_rSun;
+	J.terminate_on_capture = (job->mode == DMS_MODE_CAPTURE);
+	J.nbins = job->n_bins > 0 ? job->n_bins : 2000;
+	J.bin_width_km = (2.0 * DMS_R_SUN_KM) / J.nbins;
+	J.seed = job->seed;
+	J.point_id = job->point_id;
+	J.first_id = job->first_trajectory_id;
+	c.R.rows = J.terminate_on_capture ? nullptr : rows;
+	c.R.counts = nullptr;
+	if(rows)
+		memset(rows, 0, sizeof(double) * 2 * (size_t) J.nbins * n);
+	double rates[DMS_MAX_TARGETS];
+	for(size_t i = 0; i < n; i++)
+	{
+		DmsLane L;
+		memset(&L, 0, sizeof(L));
+		L.cur_bin = -1;
+		L.slot = (uint32_t) i;
+		dms_lane_start(L, c, ics[i], true, &recs[i], nullptr);
+		dms_lane_advance_b(L, c);
+		while(L.phase != PH_NEED)
+		{
+			dms_lane_advance_a(L, c);
+			if(L.phase == PH_SCAT)
+			{
+				for(int k = 0; k < DMS_MAX_TARGETS; k++)
+					rates[k] = 0.0;
+				for(int k = 0; k < c.S.n_iso; k++)
+					rates[k] = dms_rate_target(c.P, c.S, L.sc_r, L.sc_v, k);
+				rates[DMS_MAX_TARGETS - 1] = dms_rate_target(c.P, c.S, L.sc_r, L.sc_v, DMS_MAX_TARGETS - 1);
+				L.sc_target = dms_choose_target(rates, c.S.n_iso, L.sc_xi);
+			}
+			dms_lane_finish_scatter(L, c);
+			if(L.phase == PH_FIN)
+			{
+				dms_lane_finish(L, c, ics[i], &recs[i], nullptr);
+				break;
+			}
+			dms_lane_advance_b(L, c);
+		}
+	}
+	return 0;
+}
+
+} /* extern "C" */

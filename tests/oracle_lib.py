@@ -1,0 +1,169 @@
+"""ctypes loader for the CPU oracle (oracle/liboracle.so).
+
+Test infrastructure: the oracle is the checker, never the product.  Struct
+layouts come from the product's binding of include/dms.h.
+"""
+import ctypes as C
+import pathlib
+import subprocess
+import sys
+
+import numpy as np
+
+REPO = pathlib.Path(__file__).resolve().parent.parent
+sys.path.insert(0, str(REPO / "damascus-earth-evap_amd"))
+import dmsun  # noqa: E402
+
+ORACLE_DIR = REPO / "oracle"
+ORACLE_SO = ORACLE_DIR / "liboracle.so"
+_P = C.POINTER
+vp = C.c_void_p
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", str(ORACLE_DIR)], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not ORACLE_SO.exists():
+            build()
+        L = C.CDLL(str(ORACLE_SO))
+        sig = {
+            "or_constant": (C.c_double, [C.c_char_p]),
+            "or_solar_create": (vp, [_P(C.c_double), C.c_size_t]),
+            "or_solar_free": (None, [vp]),
+            "or_solar_eval": (C.c_double, [vp, C.c_int, C.c_double]),
+            "or_solar_n_isotopes": (C.c_int, [vp]),
+            "or_solar_isotope": (None, [vp, C.c_int, _P(C.c_int32), _P(C.c_int32), _P(C.c_double),
+                                        _P(C.c_double)]),
+            "or_thermal_averaged_relative_speed": (C.c_double, [C.c_double, C.c_double, C.c_double]),
+            "or_point_create": (vp, [vp, _P(dmsun.PointDesc), C.c_int]),
+            "or_point_free": (None, [vp]),
+            "or_point_rate": (C.c_double, [vp, C.c_int, C.c_double, C.c_double]),
+            "or_point_grid": (C.c_int, [vp, _P(C.c_double), C.c_size_t]),
+            "or_point_cross_sections": (None, [vp, _P(C.c_double), _P(C.c_double)]),
+            "or_shm_pdf_speed": (C.c_double, [_P(dmsun.Halo), C.c_double]),
+            "or_shm_average_speed": (C.c_double, [_P(dmsun.Halo)]),
+            "or_shm_inverse_average_speed": (C.c_double, [_P(dmsun.Halo)]),
+            "or_dm_entering_rate": (C.c_double, [vp, _P(dmsun.Halo), C.c_double]),
+            "or_initial_conditions": (C.c_int, [vp, _P(dmsun.Halo), C.c_uint32, C.c_uint32, C.c_uint64,
+                                                C.c_size_t, C.c_double, _P(dmsun.Event), _P(C.c_int32)]),
+            "or_hyperbolic_kepler_shift": (C.c_int, [_P(dmsun.Event), C.c_double]),
+            "or_bound_kepler_return": (C.c_int, [_P(dmsun.Event), _P(dmsun.Event)]),
+            "or_prop_create": (vp, [_P(dmsun.Event)]),
+            "or_prop_free": (None, [vp]),
+            "or_prop_step_solar": (C.c_int, [vp, vp]),
+            "or_prop_step_mass": (C.c_int, [vp, C.c_double]),
+            "or_prop_state": (None, [vp, _P(C.c_double), _P(C.c_double), _P(C.c_double), _P(C.c_double),
+                                     _P(dmsun.Event)]),
+            "or_prop_set_time_step": (None, [vp, C.c_double]),
+            "or_simulate_events": (C.c_int, [vp, _P(dmsun.JobDesc), _P(dmsun.Event), C.c_size_t,
+                                             _P(dmsun.TrajRecord), _P(C.c_double), _P(C.c_uint32), C.c_int]),
+            "or_scatter": (C.c_int, [vp, _P(dmsun.Event), C.c_uint32, C.c_uint32, C.c_uint64]),
+            "or_probe_math": (None, [C.c_int, _P(C.c_double), _P(C.c_double), _P(C.c_double), C.c_size_t]),
+            "or_run_job": (C.c_int, [vp, _P(dmsun.JobDesc), C.c_int, _P(dmsun.Counters), _P(C.c_double),
+                                     _P(C.c_uint64), _P(dmsun.EvapEvent), C.c_size_t, _P(C.c_size_t),
+                                     _P(dmsun.TrajRecord), C.c_size_t, _P(C.c_size_t)]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def const(name):
+    return lib().or_constant(name.encode())
+
+
+class Solar:
+    def __init__(self, rows=None):
+        if rows is None:
+            rows = dmsun.load_solar_rows()
+        self.rows = np.ascontiguousarray(rows)
+        self.h = lib().or_solar_create(dmsun.dptr(self.rows), self.rows.shape[0])
+        assert self.h
+
+    def __del__(self):
+        try:
+            lib().or_solar_free(self.h)
+        except Exception:
+            pass
+
+    def eval(self, which, r):
+        return lib().or_solar_eval(self.h, which, r)
+
+    def isotopes(self):
+        n = lib().or_solar_n_isotopes(self.h)
+        out = []
+        for i in range(n):
+            Z, A, m, ab = C.c_int32(), C.c_int32(), C.c_double(), C.c_double()
+            lib().or_solar_isotope(self.h, i, C.byref(Z), C.byref(A), C.byref(m), C.byref(ab))
+            out.append((Z.value, A.value, m.value, ab.value))
+        return out
+
+
+class Point:
+    def __init__(self, solar, desc, threads=8):
+        self.solar = solar
+        self.desc = desc
+        self.h = lib().or_point_create(solar.h, C.byref(desc), threads)
+        assert self.h, "or_point_create failed"
+
+    def __del__(self):
+        try:
+            lib().or_point_free(self.h)
+        except Exception:
+            pass
+
+    def rate(self, which, r, v):
+        return lib().or_point_rate(self.h, which, r, v)
+
+    def grid(self):
+        n = self.desc.interpolation_points
+        out = np.empty(n * n)
+        assert lib().or_point_grid(self.h, dmsun.dptr(out), out.size) == 0
+        return out.reshape(n, n)
+
+    def simulate_events(self, job, events, want_rows=False, threads=8):
+        n = len(events)
+        recs = dmsun.record_array(n)
+        rows = np.zeros((n, 2, job.n_bins)) if want_rows else None
+        counts = np.zeros((n, job.n_bins), dtype=np.uint32) if want_rows else None
+        st = lib().or_simulate_events(self.h, C.byref(job), events, n, recs,
+                                      dmsun.dptr(rows) if want_rows else None,
+                                      counts.ctypes.data_as(_P(C.c_uint32)) if want_rows else None, threads)
+        assert st == 0, st
+        return recs, rows, counts
+
+    def run_job(self, job, threads=8, want_records=True, record_capacity=None):
+        nb = job.n_bins
+        hist = np.zeros((8, nb))
+        counts = np.zeros((2, nb), dtype=np.uint64)
+        cnt = dmsun.Counters()
+        cap = 1 << 16
+        evs = (dmsun.EvapEvent * cap)()
+        n_ev = C.c_size_t(0)
+        rc = record_capacity if record_capacity is not None else (job.max_trajectories or 100000)
+        recs = dmsun.record_array(rc) if want_records else None
+        n_rec = C.c_size_t(0)
+        st = lib().or_run_job(self.h, C.byref(job), threads, C.byref(cnt), dmsun.dptr(hist),
+                              counts.ctypes.data_as(_P(C.c_uint64)), evs, cap, C.byref(n_ev), recs,
+                              rc if want_records else 0, C.byref(n_rec))
+        assert st == 0, st
+        return dict(counters=cnt, hist=hist, counts=counts, events=list(evs)[:min(cap, n_ev.value)],
+                    records=recs[:n_rec.value] if want_records else None)
+
+
+def initial_conditions(solar, halo, seed, point_id, first_id, n, r_final_rsun=2.0):
+    ev = (dmsun.Event * n)()
+    ok = np.zeros(n, dtype=np.int32)
+    assert lib().or_initial_conditions(solar.h, C.byref(halo), seed, point_id, first_id, n, r_final_rsun, ev,
+                                       ok.ctypes.data_as(_P(C.c_int32))) == 0
+    return ev, ok

@@ -1,0 +1,142 @@
+"""GPU parity: the HIP kernels, called through the C ABI, against the CPU oracle.
+
+Bar: integer, index and classification results identical; FP64 results
+bit-identical (the kernels and the oracle evaluate the same IEEE operation
+sequence); ensemble histograms, which the device sums in a different but fixed
+order, within 1e-12 relative.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+import dmsun
+import oracle_lib
+
+pytestmark = pytest.mark.gpu
+
+HIST_RTOL = 1e-12
+# Per-trajectory RKF45-attempt budget (the reference's wall-time guard analogue);
+# far above the longest trajectory of these cases, it only bounds a runaway lane.
+RK_BUDGET = 20_000_000
+
+
+def rec_bytes(recs):
+    return np.ctypeslib.as_array(recs).view(np.uint8).reshape(len(recs), -1)
+
+
+def first_difference(a, b):
+    for i in range(len(a)):
+        for f, _ in dmsun.TrajRecord._fields_:
+            x, y = getattr(a[i], f), getattr(b[i], f)
+            if hasattr(x, "_fields_"):
+                x = (x.t, tuple(x.r), tuple(x.v))
+                y = (y.t, tuple(y.r), tuple(y.v))
+            if str(x) != str(y):
+                return i, f, x, y
+    return None
+
+
+def test_device_math_matches_host(gpu_ctx):
+    rng = np.random.default_rng(7)
+    n = 200000
+    cases = {
+        0: (rng.uniform(0, 1e30, n), None, "sqrt"),
+        1: (rng.uniform(-1e3, 1e3, n), rng.uniform(0.1, 1e3, n), "div"),
+        2: (rng.uniform(-740, 709, n), None, "exp"),
+        3: (np.exp(rng.uniform(-700, 700, n)), None, "log"),
+        4: (rng.uniform(-1e4, 1e4, n), None, "sin"),
+        5: (rng.uniform(-1e4, 1e4, n), None, "cos"),
+        6: (rng.uniform(-8, 8, n), None, "erf"),
+        7: (rng.uniform(-1, 1, n), None, "acos"),
+    }
+    for fn, (x, y, name) in cases.items():
+        dev = gpu_ctx.probe_math(fn, x, y)
+        host = host_detmath(fn, x, y)
+        assert np.array_equal(dev.view(np.uint64), host.view(np.uint64)), name
+
+
+def host_detmath(fn, x, y):
+    """Host evaluation of the same deterministic functions, via the oracle build."""
+    import oracle_lib as O
+    L = O.lib()
+    out = np.empty_like(x)
+    yy = y if y is not None else x
+    L.or_probe_math(fn, dmsun.dptr(np.ascontiguousarray(x)), dmsun.dptr(np.ascontiguousarray(yy)), dmsun.dptr(out), x.size)
+    return out
+
+
+@pytest.mark.parametrize("desc_args", [
+    dict(mass_GeV=1.0, sigma_p_cm2=1e-35, interpolation_points=200),
+    dict(mass_GeV=0.1, sigma_p_cm2=0.0, sigma_e_cm2=1e-37, interaction=dmsun.DARK_PHOTON,
+         form_factor=dmsun.FF_GENERAL, mediator_mass_MeV=0.01, interpolation_points=120),
+])
+def test_rate_grid_bit_identical(gpu_ctx, oracle_solar, desc_args):
+    desc = dmsun.make_point(**desc_args)
+    g_dev = gpu_ctx.point(desc).rate_grid()
+    g_cpu = oracle_lib.Point(oracle_solar, desc).grid()
+    assert np.array_equal(g_dev.view(np.uint64), g_cpu.view(np.uint64))
+
+
+def test_initial_conditions_bit_identical(gpu_ctx, oracle_solar):
+    halo = dmsun.default_halo((11.1, 232.2, 7.3))
+    n = 4096
+    ev_d, ok_d = gpu_ctx.initial_conditions(halo, 20260710, 3, 1000, n)
+    ev_c, ok_c = oracle_lib.initial_conditions(oracle_solar, halo, 20260710, 3, 1000, n)
+    a = np.ctypeslib.as_array(ev_d).view(np.float64)
+    b = np.ctypeslib.as_array(ev_c).view(np.float64)
+    assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
+    assert np.array_equal(ok_d, ok_c)
+
+
+CASES = [
+    ("capture_si_1GeV", dmsun.MODE_CAPTURE, dict(mass_GeV=1.0, sigma_p_cm2=1e-35, interpolation_points=1000), 2048, 1000),
+    ("point_si_0.3GeV", dmsun.MODE_PARAMETER_POINT, dict(mass_GeV=0.3, sigma_p_cm2=1e-36, interpolation_points=1000), 512,
+     10**14),
+    ("point_dp_general", dmsun.MODE_PARAMETER_POINT,
+     dict(mass_GeV=0.1, sigma_p_cm2=0.0, sigma_e_cm2=1e-37, interaction=dmsun.DARK_PHOTON, form_factor=dmsun.FF_GENERAL,
+          mediator_mass_MeV=0.01, interpolation_points=300), 24, 10**14),
+]
+
+
+@pytest.mark.parametrize("name,mode,desc_args,n,max_scat", CASES, ids=[c[0] for c in CASES])
+def test_trajectories_bit_identical(gpu_ctx, oracle_solar, name, mode, desc_args, n, max_scat):
+    desc = dmsun.make_point(**desc_args)
+    job = dmsun.make_job(mode=mode, max_trajectories=n, max_scatterings=max_scat, n_bins=1000,
+                         rk_attempt_budget=RK_BUDGET)
+    ev, ok = gpu_ctx.initial_conditions(job.halo, job.seed, job.point_id, 0, n)
+    print(f"case {name}: n={n}", flush=True)
+    recs, rows = gpu_ctx.point(desc).simulate_events(job, ev, want_rows=True)
+    orecs, orows, _ = oracle_lib.Point(oracle_solar, desc).simulate_events(job, ev, want_rows=True)
+    same = (rec_bytes(recs) == rec_bytes(orecs)).all(axis=1)
+    assert same.all(), f"{(~same).sum()} of {n} differ; first: {first_difference(recs, orecs)}"
+    assert np.array_equal(rows.view(np.uint64), orows.view(np.uint64))
+
+
+@pytest.mark.parametrize("mode,desc_args,kw", [
+    (dmsun.MODE_CAPTURE, dict(mass_GeV=1.0, sigma_p_cm2=1e-35, interpolation_points=1000), dict(capture_target=300)),
+    (dmsun.MODE_PARAMETER_POINT, dict(mass_GeV=0.3, sigma_p_cm2=1e-36, interpolation_points=1000),
+     dict(max_trajectories=3000, max_scatterings=10**14, n_bins=1000)),
+])
+def test_job_matches_oracle(gpu_ctx, oracle_solar, mode, desc_args, kw):
+    desc = dmsun.make_point(**desc_args)
+    job = dmsun.make_job(mode=mode, rk_attempt_budget=RK_BUDGET, **{"max_trajectories": 0, **kw})
+    if mode == dmsun.MODE_PARAMETER_POINT:
+        job.capture_target = 0
+    res = gpu_ctx.point(desc).run_job(job)
+    ores = oracle_lib.Point(oracle_solar, desc).run_job(job, record_capacity=200000)
+    c, oc = res["counters"], ores["counters"]
+    for f, _ in dmsun.Counters._fields_:
+        if f in ("computing_time_sec", "simulated_trajectories"):
+            continue
+        a, b = getattr(c, f), getattr(oc, f)
+        if f == "reason_counts":
+            a, b = list(a), list(b)
+        assert a == b, f
+    scale = np.maximum(np.abs(ores["hist"]), 1e-300)
+    assert np.all(np.abs(res["hist"] - ores["hist"]) <= HIST_RTOL * scale)
+    assert np.array_equal(res["counts"], ores["counts"])
+    ev_g = [(e.trajectory_id, e.lifetime_unbinding_sec) for e in res["events"]]
+    ev_c = [(e.trajectory_id, e.lifetime_unbinding_sec) for e in ores["events"]]
+    assert ev_g == ev_c
