@@ -1,0 +1,169 @@
+#!/usr/bin/env python3
+"""Throughput benchmark of the solar dark-matter trajectory path on one MI355X.
+
+Workload (one "step"): a Parameter-point job of the reference
+(Simulation_Data::Generate_Data with run_mode "Parameter point"): SI contact
+DM-nucleon, m = 0.3 GeV, sigma_p = 1e-36 cm^2, DM_light, 10^6 trajectories
+from the Standard Halo Model (v0 = 220, v_esc = 544, |v_sun| = 248 km/s,
+rho = 0.4 GeV/cm^3) injected at 1000 AU and shifted to 2 R_sun, AGSS09 solar
+model, 1000 x 1000 rate grid, 1000 time-weighted radial bins over [0, 2 R_sun),
+evaporation times of valid unbinding events.  Every step simulates a new set of
+10^6 trajectory ids (Philox counter blocks), so the work is never cached.
+
+value = trajectories/s of the whole job stream (initial conditions, trajectory
+kernel, run accounting and histogram sums), device-resident inputs.  The K timed
+steps are K such jobs (distinct trajectory ids) submitted together with
+dms_job_submit_batch, so they share one persistent launch and the long bound
+orbits of one job overlap the bulk of the others.  The latency of one job on its
+own (one launch of 10^6 trajectories, bounded by its longest trajectory) is
+reported beside it as single_job.
+
+roofline: the trajectory kernel is FP64-VALU bound (no dense contraction, tiny
+HBM traffic).  achieved = algorithmic FP64 operations of the launch (frozen
+per-event weights x the kernel's own event counters) / average launch time from
+HIP events on the kernel's stream.
+
+cpu_baseline: the CPU restatement (oracle/liboracle.so, one trajectory stream
+per host thread) on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import pathlib
+import sys
+import time
+
+ROOT = pathlib.Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "damascus-earth-evap_amd"))
+
+# Algorithmic FP64 operations per event (hand count of the reference formulas:
+# +, -, *, /, sqrt = 1; each transcendental = 1).  See DESIGN.md, "Algorithmic work".
+FLOP_WEIGHTS = {
+    "rk_attempt": 266.0,        # one RKF45 attempt: 6 stages, 6 Mass() splines, step control
+    "step_interior": 412.0,     # step prep + optical depth with 4 pieces (Parameter point)
+    "step_interior_capture": 281.0,  # step prep + optical depth with 2 pieces (Capture)
+    "step_exterior": 194.0,     # step prep + boundary handling outside the Sun
+    "scatter": 3000.0,          # Sample_Target (64 thermal-averaged rates) + kinematics
+    "trajectory": 150.0,        # trajectory start/finish bookkeeping
+}
+PEAK_FP64_TFLOPS = 78.6
+
+
+def algorithmic_flops(c, capture_mode):
+    w = FLOP_WEIGHTS
+    segs = c.total_scatterings + c.trajectories
+    return (w["rk_attempt"] * c.n_rk_attempts
+            + (w["step_interior_capture"] if capture_mode else w["step_interior"]) * c.n_steps_interior
+            + w["step_exterior"] * c.n_steps_exterior
+            + w["scatter"] * c.total_scatterings
+            + w["trajectory"] * c.trajectories + 110.0 * segs)
+
+
+def make_workload(dmsun, n_traj, first_id):
+    desc = dmsun.make_point(0.3, 1e-36, interpolation_points=1000)
+    job = dmsun.make_job(mode=dmsun.MODE_PARAMETER_POINT, max_trajectories=n_traj, capture_target=0,
+                         max_scatterings=10**14, n_bins=1000, first_trajectory_id=first_id, record=0,
+                         halo=dmsun.default_halo((0.0, 248.0, 0.0)))
+    return desc, job
+
+
+def cpu_baseline(sample, threads):
+    sys.path.insert(0, str(ROOT / "tests"))
+    import dmsun
+    import oracle_lib
+
+    solar = oracle_lib.Solar()
+    desc, job = make_workload(dmsun, sample, 0)
+    pt = oracle_lib.Point(solar, desc, threads=threads)
+    t0 = time.perf_counter()
+    res = pt.run_job(job, threads=threads, want_records=False)
+    dt = time.perf_counter() - t0
+    c = res["counters"]
+    return dict(value=c.trajectories / dt, unit="trajectories/s", cores=threads, kind="port",
+                sample=f"{sample} trajectories of the same workload (ids 0..{sample - 1}), "
+                       f"{threads} threads, {dt:.2f} s; scatterings/s {c.total_scatterings / dt:.4g}")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--trajectories", type=int, default=1_000_000)
+    ap.add_argument("--cpu-sample", type=int, default=100_000)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(trajectories are independent: see DESIGN.md for how the path would shard).", file=sys.stderr)
+        sys.exit(2)
+
+    import dmsun
+
+    ctx = dmsun.Context(device=0)
+    desc, _ = make_workload(dmsun, args.trajectories, 0)
+    pt = ctx.point(desc)
+    single = None
+    for w in range(args.warmup):
+        _, job = make_workload(dmsun, args.trajectories, (1000 + w) * args.trajectories)
+        t = time.perf_counter()
+        res = pt.run_job(job, want_records=False)
+        dt = time.perf_counter() - t
+        single = dict(trajectories_per_s=res["counters"].trajectories / dt, seconds=dt,
+                      kernel_ms=res["kernel_ms"], trajectories=res["counters"].trajectories)
+        print(f"warmup {w} (one job alone): {dt:.2f} s, kernel {res['kernel_ms']:.1f} ms", file=sys.stderr, flush=True)
+
+    jobs = [make_workload(dmsun, args.trajectories, k * args.trajectories)[1] for k in range(args.steps)]
+    t0 = time.perf_counter()
+    results = pt.run_batch(jobs, want_records=False)
+    wall = time.perf_counter() - t0
+    tot_traj = tot_scat = tot_cap = 0
+    flops = 0.0
+    for k, res in enumerate(results):
+        c = res["counters"]
+        tot_traj += c.trajectories
+        tot_scat += c.total_scatterings
+        tot_cap += c.captured
+        flops += algorithmic_flops(c, False)
+        print(f"step {k}: {c.trajectories} trajectories, {c.total_scatterings} scatterings, "
+              f"{c.captured} captured", file=sys.stderr, flush=True)
+    kern_ms = results[0]["kernel_ms"]
+    launches = results[0]["launches"]
+    print(f"timed: {args.steps} jobs in one launch, kernel {kern_ms:.1f} ms, wall {wall:.2f} s", file=sys.stderr,
+          flush=True)
+
+    avg_launch_s = (kern_ms / launches) * 1e-3
+    achieved = flops / launches / avg_launch_s / 1e12
+    out = {
+        "metric": "trajectories/s (FP64 GPU kernel; scatterings/s alongside)",
+        "value": tot_traj / wall,
+        "unit": "trajectories/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (Standard Halo Model initial conditions from a Philox stream; AGSS09 solar model)",
+        "config": {"workload": "parameter point: SI contact m=0.3 GeV sigma_p=1e-36 cm^2, 1e6 trajectories per "
+                               "step (one job), 1000 radial bins, 1000x1000 rate grid, precision fp64; "
+                               "the timed steps share one persistent launch",
+                   "trajectories_per_step": args.trajectories},
+        "single_job": single,
+        "scatterings_per_s": tot_scat / wall,
+        "capture_fraction": tot_cap / max(1, tot_traj),
+        "kernel_ms_per_step": kern_ms / args.steps,
+        "roofline": {"bound": "valu-fp64", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "kernel": "dms_traj_kernel", "avg_launch_ms": avg_launch_s * 1e3},
+    }
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        out["cpu_baseline"] = cpu_baseline(args.cpu_sample, threads)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -72,7 +72,11 @@ struct DmsProp
 struct DmsLane
 {
 	int phase;
-	uint32_t slot;
+	uint32_t slot;  /* trajectory slot in the launch */
+	uint32_t row;   /* histogram row (slot, or the lane in per-lane mode) */
+	uint32_t job;   /* job of the slot */
+	uint32_t n_touched;
+	uint64_t id;    /* trajectory id (Philox counter) */
 	dm_rng rng;
 	double ev_t, ev_r[3], ev_v[3];
 	uint64_t n_scat;
@@ -98,17 +102,28 @@ struct DmsLane
 	double pb_time, pb_radius, pb_phi, pb_vr, pb_ts;
 	double eb_t, eb_r[3], eb_v[3];
 	double r_before, v_before, rate_before;
+	int eb_valid; /* eb/r_before/v_before already hold Event_In_3D of the current propagator */
 	int inner_iter, rk_ok;
+	double last_E; /* capture energy of the last committed event (NaN when not computed) */
 	/* scatter hand-off */
 	double sc_r, sc_v, sc_xi;
 	int sc_target;
 };
 
 /* Rows written per trajectory: dt[nbins], v2dt[nbins] (FP64), counts[nbins] (u32). */
+/* Per-trajectory radial histogram storage.  Per-trajectory mode: one row per
+ * trajectory slot, summed after the launch.  Per-lane mode: one scratch row per
+ * resident lane plus the list of bins the current trajectory touched; at the end
+ * of a trajectory its bins are added (FP64 atomics) to the sums of its class and
+ * the scratch is cleared for the next trajectory of the lane. */
 struct DmsRows
 {
-	double* rows;     /* n x 2 x nbins or NULL */
-	uint32_t* counts; /* n x nbins or NULL */
+	double* rows;       /* [row][2][nbins] or NULL */
+	uint32_t* counts;   /* [row][nbins] or NULL */
+	uint16_t* touched;  /* [row][nbins] touched-bin list (per-lane mode) */
+	int per_lane;
+	double* hist;       /* per-lane mode: [job][8][nbins] class sums */
+	unsigned long long* hcounts; /* per-lane mode: [job][2][nbins] visit counts */
 };
 
 DMS_HD double dms_rk45_sanitized(double step)
@@ -567,6 +582,108 @@ DMS_HD double dms_rate_at_fraction(const DmsPointView& P, const DmsSolarView& S,
 	return dms_rate_total(P, S, radius, dm_sqrt(dm_max(0.0, vs)));
 }
 
+/* Scattering_Rate_At_Fraction (:327-345) for n fractions of one step.  Per
+ * element the arithmetic is that of dms_rate_at_fraction / dms_rate_total; the
+ * bilinear grid reads of all elements are issued before any is used. */
+DMS_HD void dms_rates_at_fractions(const DmsPointView& P, const DmsSolarView& S, const double* br, const double* bv,
+                                   const double* ar, const double* av, const double* fr, int n, bool use_cached,
+                                   double cached_rate, double* out)
+{
+	double rf[DMS_MAX_OPTICAL_DEPTH_PIECES + 1], vf[DMS_MAX_OPTICAL_DEPTH_PIECES + 1];
+	size_t lo[DMS_MAX_OPTICAL_DEPTH_PIECES + 1];
+	int mode[DMS_MAX_OPTICAL_DEPTH_PIECES + 1]; /* 0 done, 1 grid, 2 computed */
+	double rad[DMS_MAX_OPTICAL_DEPTH_PIECES + 1], spd[DMS_MAX_OPTICAL_DEPTH_PIECES + 1];
+#pragma unroll
+	for(int i = 0; i < DMS_MAX_OPTICAL_DEPTH_PIECES + 1; i++)
+	{
+		mode[i] = 0;
+		if(i >= n)
+			continue;
+		if(i == 0 && use_cached)
+		{
+			out[i] = cached_rate;
+			continue;
+		}
+		const double f = dm_clamp_unit(fr[i]);
+		double rs = 0.0, vs = 0.0;
+		for(int c = 0; c < 3; c++)
+		{
+			const double pos = br[c] + f * (ar[c] - br[c]);
+			const double vel = bv[c] + f * (av[c] - bv[c]);
+			rs += pos * pos;
+			vs += vel * vel;
+		}
+		double r = dm_sqrt(dm_max(0.0, rs));
+		if(r >= DMS_rSun)
+		{
+			out[i] = 0.0;
+			continue;
+		}
+		double v = dm_sqrt(dm_max(0.0, vs));
+		/* Total_DM_Scattering_Rate */
+		if(!dm_isfinite(r) || !dm_isfinite(v))
+		{
+			out[i] = dm_nan();
+			continue;
+		}
+		if(r < 0.0)
+			r = 0.0;
+		if(v < 0.0)
+			v = 0.0;
+		rad[i] = r;
+		spd[i] = v;
+		mode[i] = 2;
+		if(P.use_grid && v <= P.vmax)
+		{
+			/* Total_DM_Scattering_Rate_Interpolated up to the table reads */
+			if(r > DMS_rSun || r < 0.0 || v < 0.0)
+			{
+				out[i] = 0.0;
+				mode[i] = 0;
+				continue;
+			}
+			const double rc = r * P.inv_dr;
+			const double vc = v * P.inv_dv;
+			if(!dm_isfinite(rc) || !dm_isfinite(vc) || rc < 0.0 || vc < 0.0)
+				continue; /* computed */
+			unsigned ri = (unsigned) rc;
+			if(ri > (unsigned) (P.Nr - 2))
+				ri = (unsigned) (P.Nr - 2);
+			unsigned vi = (unsigned) vc;
+			if(vi > (unsigned) (P.Nv - 2))
+				vi = (unsigned) (P.Nv - 2);
+			rf[i] = rc - ri;
+			vf[i] = vc - vi;
+			lo[i] = (size_t) ri * (size_t) P.Nv + vi;
+			mode[i] = 1;
+		}
+	}
+	double g[DMS_MAX_OPTICAL_DEPTH_PIECES + 1][4];
+#pragma unroll
+	for(int i = 0; i < DMS_MAX_OPTICAL_DEPTH_PIECES + 1; i++)
+		if(mode[i] == 1)
+		{
+			const double* q = P.grid + lo[i];
+			g[i][0] = q[0];
+			g[i][1] = q[1];
+			g[i][2] = q[P.Nv];
+			g[i][3] = q[P.Nv + 1];
+		}
+#pragma unroll
+	for(int i = 0; i < DMS_MAX_OPTICAL_DEPTH_PIECES + 1; i++)
+	{
+		if(mode[i] == 1)
+		{
+			const double lower = g[i][0] + vf[i] * (g[i][1] - g[i][0]);
+			const double upper = g[i][2] + vf[i] * (g[i][3] - g[i][2]);
+			const double val = lower + rf[i] * (upper - lower);
+			out[i] = dm_isfinite(val) ? dm_max(0.0, val) : dm_nan();
+		}
+		else if(mode[i] == 2)
+			out[i] = dms_rate_computed(P, S, rad[i], spd[i]);
+	}
+}
+
 /* Solve_Linear_Rate_Optical_Depth_Fraction, :416-464 */
 DMS_HD double dms_solve_linear_rate(double rs, double re, double dt, double target_tau)
 {
@@ -631,9 +748,11 @@ DMS_HD bool dms_uncaptured_bound(const DmsLane& L, const DmsSolarView& S, double
 DMS_HD int dms_update_capture_state(DmsLane& L, const DmsSolarView& S, double dm_mass, int terminate_on_capture,
                                     double radius, double speed, double time, int allow_new)
 {
+	L.last_E = dm_nan();
 	if(terminate_on_capture && !allow_new && !L.captured)
 		return 0;
 	const double E = dms_capture_energy_eV(S, dm_mass, radius, speed);
+	L.last_E = E;
 	double dE = dm_nan();
 	if(dm_isfinite(L.prev_cap_E))
 		dE = E - L.prev_cap_E;
@@ -688,12 +807,12 @@ DMS_HD void dms_bin_flush(DmsLane& L, const DmsRows& R, int nbins)
 		return;
 	if(R.rows)
 	{
-		double* row = R.rows + (size_t) L.slot * 2 * (size_t) nbins;
+		double* row = R.rows + (size_t) L.row * 2 * (size_t) nbins;
 		row[L.cur_bin] = L.acc_dt;
 		row[nbins + L.cur_bin] = L.acc_v2;
 	}
 	if(R.counts)
-		R.counts[(size_t) L.slot * (size_t) nbins + L.cur_bin] = L.acc_cnt;
+		R.counts[(size_t) L.row * (size_t) nbins + L.cur_bin] = L.acc_cnt;
 	L.cur_bin = -1;
 }
 
@@ -719,7 +838,7 @@ DMS_HD void dms_accumulate_bincount(DmsLane& L, const DmsRows& R, const DmsJobVi
 		L.cur_bin = bin;
 		if(R.rows)
 		{
-			const double* row = R.rows + (size_t) L.slot * 2 * (size_t) J.nbins;
+			const double* row = R.rows + (size_t) L.row * 2 * (size_t) J.nbins;
 			L.acc_dt = row[bin];
 			L.acc_v2 = row[J.nbins + bin];
 		}
@@ -728,7 +847,9 @@ DMS_HD void dms_accumulate_bincount(DmsLane& L, const DmsRows& R, const DmsJobVi
 			L.acc_dt = 0.0;
 			L.acc_v2 = 0.0;
 		}
-		L.acc_cnt = R.counts ? R.counts[(size_t) L.slot * (size_t) J.nbins + bin] : 0u;
+		L.acc_cnt = R.counts ? R.counts[(size_t) L.row * (size_t) J.nbins + bin] : 0u;
+		if(R.per_lane && L.acc_cnt == 0u)
+			R.touched[(size_t) L.row * (size_t) J.nbins + L.n_touched++] = (uint16_t) bin;
 	}
 	const double v2dt = v2 * dt_sec;
 	L.acc_dt += dt_sec;
@@ -769,6 +890,7 @@ DMS_HD int dms_segment_start(DmsLane& L, const DmsSolarView& S, double dm_mass, 
 		return DMS_NUMERICAL_FAILURE;
 	L.time_origin = L.ev_t;
 	dms_prop_init(L.prop, 0.0, L.ev_r, L.ev_v);
+	L.eb_valid = 0;
 	L.mlx = -dm_log(dm_max(dm_uniform(&L.rng), DM_DBL_MIN));
 	L.time_steps = 0;
 	L.od_retries = 0;
@@ -790,9 +912,13 @@ DMS_HD int dms_step_prep(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 		L.ev_t = t + L.time_origin;
 		return DMS_WALL_TIME_LIMIT;
 	}
-	dms_prop_event(L.prop, L.eb_t, L.eb_r, L.eb_v);
-	L.r_before = dms_prop_radius(L.prop);
-	L.v_before = dms_prop_speed(L.prop);
+	if(!L.eb_valid)
+	{
+		dms_prop_event(L.prop, L.eb_t, L.eb_r, L.eb_v);
+		L.r_before = dms_prop_radius(L.prop);
+		L.v_before = dms_prop_speed(L.prop);
+		L.eb_valid = 1;
+	}
 	L.rate_before = 0.0;
 	if(L.r_before >= DMS_rSun)
 	{
@@ -880,30 +1006,30 @@ DMS_HD int dms_step_post(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 		if(ie > is && actual_dt > 0.0)
 		{
 			const double span = ie - is;
-			for(int i = 0; i <= od_count; i++)
-				fr[i] = is + (double) i / (double) od_count * span;
-			for(int i = 0; i <= od_count; i++)
-			{
-				if(i == 0 && use_cached)
-					rt[i] = L.rate_before;
-				else
-					rt[i] = dms_rate_at_fraction(P, S, L.eb_r, L.eb_v, ea_r, ea_v, fr[i]);
-				if(!dm_isfinite(rt[i]) || rt[i] < 0.0)
-				{
+#pragma unroll
+			for(int i = 0; i <= DMS_MAX_OPTICAL_DEPTH_PIECES; i++)
+				fr[i] = (i <= od_count) ? is + (double) i / (double) od_count * span : 0.0;
+			/* Scattering_Rate_At_Fraction for all fractions, grid loads issued together */
+			dms_rates_at_fractions(P, S, L.eb_r, L.eb_v, ea_r, ea_v, fr, od_count + 1, use_cached, L.rate_before, rt);
+#pragma unroll
+			for(int i = 0; i <= DMS_MAX_OPTICAL_DEPTH_PIECES; i++)
+				if(i <= od_count && (!dm_isfinite(rt[i]) || rt[i] < 0.0))
 					invalid = 1;
-					break;
-				}
-			}
 			if(!invalid)
 			{
 #define DMS_TRAP(s, e) (0.5 * (rt[s] + rt[e]) * (actual_dt * (fr[e] - fr[s])))
 				if(od_count >= 4)
 					tau2 = DMS_TRAP(0, 2) + DMS_TRAP(2, 4);
 				else
-					for(int i = 0; i < od_count; i++)
-						tau2 += DMS_TRAP(i, i + 1);
-				for(int i = 0; i < od_count; i++)
-					if(fr[i + 1] > fr[i])
+				{
+#pragma unroll
+					for(int i = 0; i < DMS_MAX_OPTICAL_DEPTH_PIECES; i++)
+						if(i < od_count)
+							tau2 += DMS_TRAP(i, i + 1);
+				}
+#pragma unroll
+				for(int i = 0; i < DMS_MAX_OPTICAL_DEPTH_PIECES; i++)
+					if(i < od_count && fr[i + 1] > fr[i])
 					{
 						pc++;
 						dtau += DMS_TRAP(i, i + 1);
@@ -939,32 +1065,41 @@ DMS_HD int dms_step_post(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 		}
 		if(dtau >= L.mlx)
 		{
+			/* locate the scattering piece; pieces with end > start only */
 			double remaining = L.mlx;
-			int k = 0;
-			for(int i = 0; i < od_count; i++)
+			int hit = -1;
+#pragma unroll
+			for(int i = 0; i < DMS_MAX_OPTICAL_DEPTH_PIECES; i++)
 			{
-				if(!(fr[i + 1] > fr[i]))
-					continue;
-				const double ps = fr[i], pe = fr[i + 1];
-				const double piece_dt = actual_dt * (pe - ps);
-				const double piece_tau = 0.5 * (rt[i] + rt[i + 1]) * piece_dt;
-				k++;
-				if(piece_tau >= remaining)
+				if(hit < 0 && i < od_count && fr[i + 1] > fr[i])
 				{
-					const double lf = dms_solve_linear_rate(rt[i], rt[i + 1], piece_dt, remaining);
-					const double fraction = ps + lf * (pe - ps);
-					dms_interpolate_event(L.eb_t, L.eb_r, L.eb_v, ea_t, ea_r, ea_v, fraction, acc_t, acc_r, acc_v);
-					scattering = 1;
-					break;
+					const double piece_dt = actual_dt * (fr[i + 1] - fr[i]);
+					const double piece_tau = 0.5 * (rt[i] + rt[i + 1]) * piece_dt;
+					if(piece_tau >= remaining)
+						hit = i;
+					else
+						remaining -= piece_tau;
 				}
-				remaining -= piece_tau;
 			}
-			(void) k;
-			if(!scattering)
+			double fraction = ie;
+			if(hit >= 0)
 			{
-				dms_interpolate_event(L.eb_t, L.eb_r, L.eb_v, ea_t, ea_r, ea_v, ie, acc_t, acc_r, acc_v);
-				scattering = 1;
+				double ps = 0.0, pe = 0.0, rs0 = 0.0, re0 = 0.0;
+#pragma unroll
+				for(int i = 0; i < DMS_MAX_OPTICAL_DEPTH_PIECES; i++)
+					if(i == hit)
+					{
+						ps = fr[i];
+						pe = fr[i + 1];
+						rs0 = rt[i];
+						re0 = rt[i + 1];
+					}
+				const double piece_dt = actual_dt * (pe - ps);
+				const double lf = dms_solve_linear_rate(rs0, re0, piece_dt, remaining);
+				fraction = ps + lf * (pe - ps);
 			}
+			dms_interpolate_event(L.eb_t, L.eb_r, L.eb_v, ea_t, ea_r, ea_v, fraction, acc_t, acc_r, acc_v);
+			scattering = 1;
 		}
 		else
 			L.mlx -= dtau;
@@ -1011,6 +1146,7 @@ DMS_HD int dms_step_post(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 				/* reset_propagator_at_absolute_event, :814-822 */
 				L.time_origin = L.ev_t;
 				dms_prop_init(L.prop, 0.0, L.ev_r, L.ev_v);
+				L.eb_valid = 0;
 				if(!J.terminate_on_capture && L.prev_time_sec >= 0.0)
 					L.prev_time_sec = 0.0;
 				return -2;
@@ -1026,14 +1162,29 @@ DMS_HD int dms_step_post(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 		L.ev_r[c] = acc_r[c];
 		L.ev_v[c] = acc_v[c];
 	}
-	if(dms_uncaptured_bound(L, S, P.mass, L.ev_r, L.ev_v))
-		return DMS_NUMERICAL_FAILURE;
+	{
+		/* abort_if_uncaptured_bound on the committed event: the energy was just
+		 * computed for the same (|r|, |v|) by Update_Capture_State */
+		const double E = dm_isnan(L.last_E) ? dms_capture_energy_eV(S, P.mass, dms_norm(L.ev_r), dms_norm(L.ev_v)) : L.last_E;
+		if(!L.captured && !(dm_isfinite(E) && E >= 0.0))
+			return DMS_NUMERICAL_FAILURE;
+	}
 	if(J.terminate_on_capture && captured_now)
 		return DMS_CAPTURE_MODE;
 	if(reflection)
 		return DMS_OUTWARD_ESCAPE;
 	if(scattering)
 		return -3;
+	/* the next step starts from this propagator state: Event_In_3D is ea */
+	L.eb_t = ea_t;
+	for(int c = 0; c < 3; c++)
+	{
+		L.eb_r[c] = ea_r[c];
+		L.eb_v[c] = ea_v[c];
+	}
+	L.r_before = r_after;
+	L.v_before = v_after;
+	L.eb_valid = 1;
 	return -2;
 #undef DMS_RETURN_AT
 }
@@ -1207,6 +1358,7 @@ DMS_HD void dms_trajectory_begin(DmsLane& L, const DmsSolarView& S, double dm_ma
 	L.acc_dt = 0.0;
 	L.acc_v2 = 0.0;
 	L.acc_cnt = 0;
+	L.n_touched = 0;
 	L.rk_all = 0;
 	L.n_int = 0;
 	L.n_ext = 0;
@@ -1411,6 +1563,44 @@ DMS_HD void dms_shift_failure_record(uint64_t id, const dms_event& ic, dms_traj_
 }
 
 
+/* Per-lane mode: add the finished trajectory's bins to the sums of its class
+ * (1 captured, 2 completed outward escape, 0 none) and clear the scratch row. */
+DMS_HD void dms_flush_lane_rows(DmsLane& L, const DmsRows& R, int nbins, int cls)
+{
+	double* row = R.rows + (size_t) L.row * 2 * (size_t) nbins;
+	uint32_t* cnt = R.counts + (size_t) L.row * (size_t) nbins;
+	const uint16_t* list = R.touched + (size_t) L.row * (size_t) nbins;
+	double* h = R.hist + (size_t) L.job * 8 * (size_t) nbins + (cls == 2 ? 4 * (size_t) nbins : 0);
+	unsigned long long* hc = R.hcounts + (size_t) L.job * 2 * (size_t) nbins + (cls == 2 ? (size_t) nbins : 0);
+	for(uint32_t k = 0; k < L.n_touched; k++)
+	{
+		const int b = list[k];
+		const double dt = row[b];
+		const double v2 = row[nbins + b];
+		const uint32_t cn = cnt[b];
+		if(cls != 0)
+		{
+#if defined(__HIP_DEVICE_COMPILE__)
+			unsafeAtomicAdd(h + b, dt);
+			unsafeAtomicAdd(h + nbins + b, v2);
+			unsafeAtomicAdd(h + 2 * nbins + b, dt * dt);
+			unsafeAtomicAdd(h + 3 * nbins + b, v2 * v2);
+			atomicAdd(hc + b, (unsigned long long) cn);
+#else
+			h[b] += dt;
+			h[nbins + b] += v2;
+			h[2 * nbins + b] += dt * dt;
+			h[3 * nbins + b] += v2 * v2;
+			hc[b] += cn;
+#endif
+		}
+		row[b] = 0.0;
+		row[nbins + b] = 0.0;
+		cnt[b] = 0u;
+	}
+	L.n_touched = 0;
+}
+
 /* ---- phase drivers shared by the kernel and host replays ------------------ */
 struct DmsLaneCtx
 {
@@ -1468,7 +1658,17 @@ DMS_HD void dms_lane_finish_scatter(DmsLane& L, const DmsLaneCtx& c)
 DMS_HD void dms_lane_finish(DmsLane& L, const DmsLaneCtx& c, const dms_event& ic, dms_traj_record* rec, DmsSummary* sum)
 {
 	dms_bin_flush(L, c.R, c.J.nbins);
-	dms_trajectory_end(L, c.S, c.P.mass, c.J, c.J.first_id + L.slot, ic, rec, sum);
+	uint32_t flags = 0;
+	int reason = 0;
+	{
+		DmsSummary tmp;
+		DmsSummary* s = sum ? sum : &tmp;
+		dms_trajectory_end(L, c.S, c.P.mass, c.J, L.id, ic, rec, s);
+		flags = s->flags;
+		reason = s->reason;
+	}
+	if(c.R.per_lane && L.n_touched > 0)
+		dms_flush_lane_rows(L, c.R, c.J.nbins, (flags & DMS_F_CAPTURED) ? 1 : ((reason == DMS_OUTWARD_ESCAPE) ? 2 : 0));
 	L.phase = PH_NEED;
 }
 
@@ -1476,11 +1676,11 @@ DMS_HD void dms_lane_start(DmsLane& L, const DmsLaneCtx& c, const dms_event& ic,
 {
 	if(!shift_ok)
 	{
-		dms_shift_failure_record(c.J.first_id + L.slot, ic, rec, sum);
+		dms_shift_failure_record(L.id, ic, rec, sum);
 		L.phase = PH_NEED;
 		return;
 	}
-	dms_trajectory_begin(L, c.S, c.P.mass, c.J, c.J.first_id + L.slot, ic);
+	dms_trajectory_begin(L, c.S, c.P.mass, c.J, L.id, ic);
 	L.phase = PH_SEG;
 }
 

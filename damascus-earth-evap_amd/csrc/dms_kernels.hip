@@ -34,8 +34,27 @@ __device__ inline double shfl_d(double x, int src) { return __shfl(x, src); }
 __global__ void __launch_bounds__(DMS_TRAJ_BLOCK) dms_traj_kernel(DmsTrajArgs a)
 {
 	const int lane = threadIdx.x & 63;
+	/* Stage the profile nodes and the Mass and escape-speed cubics in LDS: the
+	 * RKF45 stages read Mass() six times per attempt in a dependent chain, so
+	 * their latency sets the speed of a lone long trajectory. */
+	extern __shared__ __attribute__((aligned(32))) unsigned char dms_lds[];
+	const int nn = a.S.n;
+	double* lx = reinterpret_cast<double*>(dms_lds);
+	DmsCubic* lmass = reinterpret_cast<DmsCubic*>(dms_lds + dms_lds_mass_offset(nn));
+	DmsCubic* lvesc = lmass + (nn - 1);
+	for(int i = threadIdx.x; i < nn; i += blockDim.x)
+		lx[i] = a.S.x[i];
+	for(int i = threadIdx.x; i < nn - 1; i += blockDim.x)
+	{
+		lmass[i] = a.S.mass[i];
+		lvesc[i] = a.S.vesc2[i];
+	}
+	__syncthreads();
 	DmsLaneCtx c;
 	c.S = a.S;
+	c.S.x = lx;
+	c.S.mass = lmass;
+	c.S.vesc2 = lvesc;
 	c.P = a.P;
 	c.J = a.J;
 	c.R = a.R;
@@ -43,7 +62,12 @@ __global__ void __launch_bounds__(DMS_TRAJ_BLOCK) dms_traj_kernel(DmsTrajArgs a)
 	L.phase = PH_NEED;
 	L.cur_bin = -1;
 	L.slot = 0;
+	L.row = 0;
+	L.job = 0;
+	L.id = 0;
+	L.n_touched = 0;
 	L.sc_target = -2;
+	const uint32_t lane_gid = blockIdx.x * blockDim.x + threadIdx.x;
 	const int n_iso = a.S.n_iso;
 	for(;;)
 	{
@@ -111,6 +135,12 @@ __global__ void __launch_bounds__(DMS_TRAJ_BLOCK) dms_traj_kernel(DmsTrajArgs a)
 				if(idx < a.n)
 				{
 					L.slot = idx;
+					uint32_t j = 0;
+					while(j + 1 < a.n_jobs && a.job_start[j + 1] <= idx)
+						j++;
+					L.job = j;
+					L.id = a.job_first_id[j] + (uint64_t) (idx - a.job_start[j]);
+					L.row = a.R.per_lane ? lane_gid : idx;
 					L.phase = PH_START;
 				}
 				else

@@ -9,6 +9,10 @@
 
 #define DMS_TRAJ_BLOCK 256
 
+/* Dynamic LDS of dms_traj_kernel: nodes x[n], then Mass and v_esc^2 cubics. */
+__host__ __device__ inline size_t dms_lds_mass_offset(int n) { return ((size_t) n * sizeof(double) + 31) & ~(size_t) 31; }
+__host__ __device__ inline size_t dms_lds_bytes(int n) { return dms_lds_mass_offset(n) + 2 * (size_t) (n - 1) * sizeof(DmsCubic); }
+
 struct DmsTrajArgs
 {
 	DmsSolarView S;
@@ -21,6 +25,11 @@ struct DmsTrajArgs
 	DmsSummary* summ;
 	DmsRows R;
 	unsigned int* next;
+	/* jobs sharing the launch: slots [job_start[j], job_start[j+1]) belong to job j,
+	 * whose trajectory ids start at job_first_id[j] */
+	uint32_t n_jobs;
+	const uint32_t* job_start;
+	const uint64_t* job_first_id;
 };
 
 __global__ void dms_traj_kernel(DmsTrajArgs a);

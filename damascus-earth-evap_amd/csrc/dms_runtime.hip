@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -35,6 +36,7 @@ struct dms_ctx
 	dms::SolarTables T;
 	int n_cu = 0;
 	int traj_blocks_per_cu = 0;
+	size_t traj_lds = 0;
 	/* device copies of the solar tables */
 	double* d_x = nullptr;
 	DmsCubic *d_mass = nullptr, *d_temp = nullptr, *d_vesc2 = nullptr, *d_dens = nullptr;
@@ -54,11 +56,23 @@ struct dms_point
 	DmsPointView dev;
 };
 
+struct dms_job;
+
+struct dms_batch
+{
+	std::vector<dms_job*> jobs;
+	std::thread worker;
+	std::mutex mu;
+	bool joined = false;
+};
+
 struct dms_job
 {
 	dms_point* pt = nullptr;
 	dms_job_desc desc;
-	std::thread worker;
+	std::shared_ptr<dms_batch> batch;
+	bool needs_rerun = false;
+	double batch_size = 1.0;
 	std::atomic<uint64_t> done_traj{0}, done_captured{0}, done_scat{0};
 	std::atomic<int> finished{0};
 	int status = DMS_OK;
@@ -260,9 +274,38 @@ int traj_grid(const dms_ctx* ctx, size_t n)
 	return (int) (g > 0 ? g : 1);
 }
 
-/* Launch the trajectory kernel on ids [first, first+n) with buffers B. Returns the kernel time. */
+/* Jobs sharing one trajectory launch (slot ranges and first trajectory ids). */
+struct LaunchJobs
+{
+	std::vector<uint32_t> start;
+	std::vector<uint64_t> first_id;
+	uint32_t* d_start = nullptr;
+	uint64_t* d_first = nullptr;
+
+	hipError_t upload()
+	{
+		hipError_t e;
+		if((e = hipMalloc((void**) &d_start, sizeof(uint32_t) * start.size())) != hipSuccess)
+			return e;
+		if((e = hipMalloc((void**) &d_first, sizeof(uint64_t) * first_id.size())) != hipSuccess)
+			return e;
+		if((e = hipMemcpy(d_start, start.data(), sizeof(uint32_t) * start.size(), hipMemcpyHostToDevice)) != hipSuccess)
+			return e;
+		return hipMemcpy(d_first, first_id.data(), sizeof(uint64_t) * first_id.size(), hipMemcpyHostToDevice);
+	}
+	void release()
+	{
+		hipFree(d_start);
+		hipFree(d_first);
+		d_start = nullptr;
+		d_first = nullptr;
+	}
+};
+
+/* Launch the trajectory kernel over n slots.  rows: per-trajectory rows in B;
+ * lane_rows: per-lane scratch mode (R prepared by the caller). */
 hipError_t run_trajectories(dms_ctx* ctx, const dms_point* pt, const DmsJobView& J, ChunkBuffers& B, size_t n, bool rows,
-                            bool recs, float* ms)
+                            bool recs, const LaunchJobs& LJ, const DmsRows* lane_rows, float* ms)
 {
 	DmsTrajArgs a;
 	a.S = ctx->dev;
@@ -273,13 +316,25 @@ hipError_t run_trajectories(dms_ctx* ctx, const dms_point* pt, const DmsJobView&
 	a.n = (uint32_t) n;
 	a.records = recs ? B.recs : nullptr;
 	a.summ = B.summ;
-	a.R.rows = rows ? B.rows : nullptr;
-	a.R.counts = rows ? B.counts : nullptr;
+	if(lane_rows)
+		a.R = *lane_rows;
+	else
+	{
+		a.R.rows = rows ? B.rows : nullptr;
+		a.R.counts = rows ? B.counts : nullptr;
+		a.R.touched = nullptr;
+		a.R.per_lane = 0;
+		a.R.hist = nullptr;
+		a.R.hcounts = nullptr;
+	}
 	a.next = B.next;
+	a.n_jobs = (uint32_t) LJ.start.size();
+	a.job_start = LJ.d_start;
+	a.job_first_id = LJ.d_first;
 	hipError_t e;
 	if((e = hipMemsetAsync(B.next, 0, sizeof(unsigned int), ctx->stream)) != hipSuccess)
 		return e;
-	if(rows)
+	if(rows && !lane_rows)
 	{
 		if((e = hipMemsetAsync(B.rows, 0, sizeof(double) * 2 * (size_t) J.nbins * n, ctx->stream)) != hipSuccess)
 			return e;
@@ -289,8 +344,9 @@ hipError_t run_trajectories(dms_ctx* ctx, const dms_point* pt, const DmsJobView&
 	hipEvent_t e0, e1;
 	hipEventCreate(&e0);
 	hipEventCreate(&e1);
+	(void) hipGetLastError();
 	hipEventRecord(e0, ctx->stream);
-	hipLaunchKernelGGL(dms_traj_kernel, dim3(traj_grid(ctx, n)), dim3(DMS_TRAJ_BLOCK), 0, ctx->stream, a);
+	hipLaunchKernelGGL(dms_traj_kernel, dim3(traj_grid(ctx, n)), dim3(DMS_TRAJ_BLOCK), ctx->traj_lds, ctx->stream, a);
 	e = hipGetLastError();
 	hipEventRecord(e1, ctx->stream);
 	if(e == hipSuccess)
@@ -301,6 +357,20 @@ hipError_t run_trajectories(dms_ctx* ctx, const dms_point* pt, const DmsJobView&
 		*ms = t;
 	hipEventDestroy(e0);
 	hipEventDestroy(e1);
+	return e;
+}
+
+/* single-job launch over slots [0, n) with ids first_id .. */
+hipError_t run_trajectories_single(dms_ctx* ctx, const dms_point* pt, const DmsJobView& J, ChunkBuffers& B, size_t n, bool rows,
+                                   bool recs, uint64_t first_id, float* ms)
+{
+	LaunchJobs LJ;
+	LJ.start.push_back(0);
+	LJ.first_id.push_back(first_id);
+	hipError_t e = LJ.upload();
+	if(e == hipSuccess)
+		e = run_trajectories(ctx, pt, J, B, n, rows, recs, LJ, nullptr, ms);
+	LJ.release();
 	return e;
 }
 
@@ -325,8 +395,21 @@ struct RoundReplay
 	int stop = 0;
 	dms_counters* C;
 
+	void init(const dms_job_desc& d, const dms_point* pt, dms_counters* counters)
+	{
+		C = counters;
+		capture = (d.mode == DMS_MODE_CAPTURE);
+		target = d.capture_target == 0 ? UINT64_MAX : d.capture_target;
+		unlimited = d.max_trajectories == 0;
+		max_traj = unlimited ? UINT64_MAX : d.max_trajectories;
+		interval = capture ? 32 : dms::normal_mode_sync_interval(pt->P.sigma_p / (DMS_cm * DMS_cm));
+		C->sync_interval = (int32_t) interval;
+	}
+
 	bool open_round()
 	{
+		if(stopped)
+			return false;
 		if(left_in_round > 0)
 			return true;
 		if(!(captured < target))
@@ -371,13 +454,103 @@ struct RoundReplay
 			}
 		}
 	}
+
+	void finish()
+	{
+		C->capture_target_overshoot = (captured > target) ? captured - target : 0;
+		int s = stop;
+		if(captured < target && s == 0)
+			s = 2;
+		C->early_stop_reason = s;
+	}
 };
 
-void run_job(dms_job* job)
+/* simulate_one_trajectory accounting (src/Data_Generation.cpp:764-850) of one
+ * summary.  Returns the histogram class: 1 captured, 2 completed escape, 0 none. */
+int account_trajectory(dms_job* job, RoundReplay& R, const DmsSummary& s, double elapsed_sec)
+{
+	dms_counters& C = job->C;
+	const bool capture = R.capture;
+	int cls = 0;
+	const bool shift_ok = !(s.flags & DMS_F_SHIFT_FAILED);
+	if(!shift_ok)
+	{
+		C.initial_shift_failures++;
+		C.numerical_failures++;
+	}
+	R.local_total++;
+	C.trajectories++;
+	C.total_scatterings += s.n_scat;
+	C.reason_counts[s.reason]++;
+	C.n_rk_attempts += s.n_rk;
+	C.n_steps_interior += s.n_int;
+	C.n_steps_exterior += s.n_ext;
+	C.n_tau_rejections += s.n_rej;
+	C.n_tv_iterations += s.n_tv;
+	if(shift_ok && is_numerical(s.reason))
+		C.numerical_failures++;
+	if(is_truncation(s.reason))
+		C.computational_truncations++;
+	if(s.flags & DMS_F_CAPTURED)
+	{
+		C.captured++;
+		R.captured++;
+		if(!capture)
+		{
+			if(s.flags & DMS_F_EVENT_OBSERVED)
+				C.complete_evaporation++;
+			else
+				C.invalid_survival_captured++;
+			cls = 1;
+			/* Build_Evaporation_Record + Make_Compact_Evaporation_Event (:118-176) */
+			if(std::isfinite(s.t_capture_sec))
+			{
+				const bool nie = (s.flags & DMS_F_NUMERICALLY_INVALID_ESCAPE) || s.reason == DMS_ENERGY_DRIFT_ESCAPE;
+				const bool sv = (s.flags & DMS_F_SURVIVAL_VALID) && !nie;
+				const bool eo = sv && (s.flags & DMS_F_EVENT_OBSERVED);
+				const double lu = eo ? (s.t_unbind_sec - s.t_capture_sec) : -1.0;
+				const double lifetime = (std::isfinite(lu) && lu >= 0.0) ? lu : -1.0;
+				if(sv && eo && lifetime >= 0.0)
+				{
+					dms_evap_event E;
+					E.rank = 0;
+					E.pad = 0;
+					E.trajectory_id = C.trajectories;
+					E.completion_wall_time_sec = elapsed_sec;
+					E.lifetime_unbinding_sec = lifetime;
+					job->events.push_back(E);
+				}
+			}
+		}
+	}
+	else if(s.reason == DMS_OUTWARD_ESCAPE)
+	{
+		C.completed_outward_escapes++;
+		if(!capture)
+		{
+			cls = 2;
+			if(s.flags & DMS_F_FREE)
+				C.free_particles++;
+			else if(s.flags & DMS_F_REFLECTED)
+			{
+				C.reflected_particles++;
+				if(s.flags & DMS_F_REFLECTION_SHIFT_FAILED)
+				{
+					C.final_reflection_shift_failures++;
+					C.numerical_failures++;
+				}
+			}
+		}
+	}
+	return cls;
+}
+
+/* Chunked job: speculative id-ordered chunks with per-trajectory rows, so that
+ * capture-target runs count exactly the trajectories the reference would. */
+void run_job_chunked(dms_job* job)
 {
 	dms_point* pt = job->pt;
 	dms_ctx* ctx = pt->ctx;
-	std::lock_guard<std::mutex> lock(ctx->mu);
 	const dms_job_desc& d = job->desc;
 	const DmsJobView J = job_view(d);
 	const bool capture = (d.mode == DMS_MODE_CAPTURE);
@@ -386,16 +559,12 @@ void run_job(dms_job* job)
 	const int nb = J.nbins;
 	dms_counters& C = job->C;
 	memset(&C, 0, sizeof(C));
+	job->events.clear();
+	job->records.clear();
 	job->hist.assign((size_t) 8 * nb, 0.0);
 	job->counts.assign((size_t) 2 * nb, 0ull);
 	RoundReplay R;
-	R.C = &C;
-	R.capture = capture;
-	R.target = d.capture_target == 0 ? UINT64_MAX : d.capture_target;
-	R.unlimited = d.max_trajectories == 0;
-	R.max_traj = R.unlimited ? UINT64_MAX : d.max_trajectories;
-	R.interval = capture ? 32 : dms::normal_mode_sync_interval(pt->P.sigma_p / (DMS_cm * DMS_cm));
-	C.sync_interval = (int32_t) R.interval;
+	R.init(d, pt, &C);
 	const DmsHaloView H = halo_view(ctx, d.halo);
 
 	ChunkBuffers B;
@@ -415,7 +584,6 @@ void run_job(dms_job* job)
 		   || hipMalloc((void**) &d_counts, sizeof(unsigned long long) * 2 * nb) != hipSuccess)
 		{
 			set_err("hipMalloc of histogram accumulators failed");
-			job->finished = 1;
 			return;
 		}
 		hipMemset(d_hist, 0, sizeof(double) * 8 * nb);
@@ -435,13 +603,10 @@ void run_job(dms_job* job)
 	std::vector<DmsSummary> hsum;
 	std::vector<uint8_t> hcls;
 	std::vector<dms_traj_record> hrec;
-	while(!R.stopped && R.open_round())
+	while(R.open_round())
 	{
-		/* how many trajectories to simulate speculatively */
 		uint64_t want;
-		if(R.unlimited && R.target == UINT64_MAX)
-			want = 1 << 20;
-		else if(R.target == UINT64_MAX)
+		if(R.target == UINT64_MAX)
 			want = R.max_traj - R.local_total;
 		else
 		{
@@ -462,10 +627,11 @@ void run_job(dms_job* job)
 			break;
 		}
 		const double t0 = now_ms();
+		(void) hipGetLastError();
 		hipLaunchKernelGGL(dms_ic_kernel, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, ctx->stream, H, d.seed, d.point_id,
 		                   next_id, (uint32_t) n, d.r_max_rsun * DMS_rSun, B.ics, B.ok);
 		float kms = 0.0f;
-		hipError_t e = run_trajectories(ctx, pt, J, B, n, want_rows, want_recs, &kms);
+		hipError_t e = run_trajectories_single(ctx, pt, J, B, n, want_rows, want_recs, next_id, &kms);
 		if(e != hipSuccess)
 		{
 			set_err(std::string("trajectory kernel: ") + hipGetErrorString(e));
@@ -487,79 +653,9 @@ void run_job(dms_job* job)
 		const double elapsed_sec = (now_ms() - t_start) * 1e-3;
 		for(size_t i = 0; i < n; i++)
 		{
-			if(R.stopped || !R.open_round())
+			if(!R.open_round())
 				break;
-			const DmsSummary& s = hsum[i];
-			const bool shift_ok = !(s.flags & DMS_F_SHIFT_FAILED);
-			if(!shift_ok)
-			{
-				C.initial_shift_failures++;
-				C.numerical_failures++;
-			}
-			R.local_total++;
-			C.trajectories++;
-			C.total_scatterings += s.n_scat;
-			C.reason_counts[s.reason]++;
-			C.n_rk_attempts += s.n_rk;
-			C.n_steps_interior += s.n_int;
-			C.n_steps_exterior += s.n_ext;
-			C.n_tau_rejections += s.n_rej;
-			C.n_tv_iterations += s.n_tv;
-			if(shift_ok && is_numerical(s.reason))
-				C.numerical_failures++;
-			if(is_truncation(s.reason))
-				C.computational_truncations++;
-			if(s.flags & DMS_F_CAPTURED)
-			{
-				C.captured++;
-				R.captured++;
-				if(!capture)
-				{
-					if(s.flags & DMS_F_EVENT_OBSERVED)
-						C.complete_evaporation++;
-					else
-						C.invalid_survival_captured++;
-					hcls[i] = 1;
-					/* Build_Evaporation_Record + Make_Compact_Evaporation_Event */
-					if(std::isfinite(s.t_capture_sec))
-					{
-						const bool nie = (s.flags & DMS_F_NUMERICALLY_INVALID_ESCAPE) || s.reason == DMS_ENERGY_DRIFT_ESCAPE;
-						const bool sv = (s.flags & DMS_F_SURVIVAL_VALID) && !nie;
-						const bool eo = sv && (s.flags & DMS_F_EVENT_OBSERVED);
-						const double lu = eo ? (s.t_unbind_sec - s.t_capture_sec) : -1.0;
-						const double lifetime = (std::isfinite(lu) && lu >= 0.0) ? lu : -1.0;
-						if(sv && eo && lifetime >= 0.0)
-						{
-							dms_evap_event E;
-							E.rank = 0;
-							E.pad = 0;
-							E.trajectory_id = C.trajectories;
-							E.completion_wall_time_sec = elapsed_sec;
-							E.lifetime_unbinding_sec = lifetime;
-							job->events.push_back(E);
-						}
-					}
-				}
-			}
-			else if(s.reason == DMS_OUTWARD_ESCAPE)
-			{
-				C.completed_outward_escapes++;
-				if(!capture)
-				{
-					hcls[i] = 2;
-					if(s.flags & DMS_F_FREE)
-						C.free_particles++;
-					else if(s.flags & DMS_F_REFLECTED)
-					{
-						C.reflected_particles++;
-						if(s.flags & DMS_F_REFLECTION_SHIFT_FAILED)
-						{
-							C.final_reflection_shift_failures++;
-							C.numerical_failures++;
-						}
-					}
-				}
-			}
+			hcls[i] = (uint8_t) account_trajectory(job, R, hsum[i], elapsed_sec);
 			if(want_recs)
 				job->records.push_back(hrec[i]);
 			counted = i + 1;
@@ -597,11 +693,7 @@ void run_job(dms_job* job)
 		job->done_captured = C.captured;
 		job->done_scat = C.total_scatterings;
 	}
-	C.capture_target_overshoot = (R.captured > R.target) ? R.captured - R.target : 0;
-	int stop = R.stop;
-	if(R.captured < R.target && stop == 0)
-		stop = 2;
-	C.early_stop_reason = stop;
+	R.finish();
 	C.simulated_trajectories = simulated;
 	C.computing_time_sec = (now_ms() - t_start) * 1e-3;
 	if(want_rows && job->status == DMS_OK)
@@ -614,7 +706,182 @@ void run_job(dms_job* job)
 	hipFree(d_partial);
 	hipFree(d_pcount);
 	B.release();
-	job->finished = 1;
+}
+
+/* Fixed-count jobs (max_trajectories > 0, no capture target) of one point
+ * streamed through a single persistent launch: lanes refill across job
+ * boundaries, so the long tail of one job overlaps the bulk of the others.
+ * Histograms use per-lane scratch rows flushed into per-job sums. */
+void run_batch_fixed(std::vector<dms_job*>& jobs)
+{
+	dms_point* pt = jobs[0]->pt;
+	dms_ctx* ctx = pt->ctx;
+	const dms_job_desc& d0 = jobs[0]->desc;
+	const DmsJobView J = job_view(d0);
+	const bool capture = (d0.mode == DMS_MODE_CAPTURE);
+	const bool want_rows = !capture;
+	bool want_recs = false;
+	for(dms_job* j : jobs)
+		want_recs = want_recs || j->desc.record_trajectories != 0;
+	const int nb = J.nbins;
+	const size_t nj = jobs.size();
+	LaunchJobs LJ;
+	size_t n = 0;
+	for(dms_job* j : jobs)
+	{
+		LJ.start.push_back((uint32_t) n);
+		LJ.first_id.push_back(j->desc.first_trajectory_id);
+		n += (size_t) j->desc.max_trajectories;
+		memset(&j->C, 0, sizeof(j->C));
+		j->events.clear();
+		j->records.clear();
+		j->hist.assign((size_t) 8 * nb, 0.0);
+		j->counts.assign((size_t) 2 * nb, 0ull);
+	}
+	auto set_err = [&](const std::string& m) {
+		for(dms_job* j : jobs)
+		{
+			j->status = DMS_ERR_DEVICE;
+			j->err = m;
+		}
+	};
+	const double t_start = now_ms();
+	ChunkBuffers B;
+	DmsRows LR;
+	memset(&LR, 0, sizeof(LR));
+	const size_t lanes = (size_t) traj_grid(ctx, n) * DMS_TRAJ_BLOCK;
+	bool ok = LJ.upload() == hipSuccess && ensure(B, n, nb, false, want_recs) == hipSuccess;
+	if(ok && want_rows)
+	{
+		LR.per_lane = 1;
+		ok = hipMalloc((void**) &LR.rows, sizeof(double) * 2 * (size_t) nb * lanes) == hipSuccess
+		     && hipMalloc((void**) &LR.counts, sizeof(uint32_t) * (size_t) nb * lanes) == hipSuccess
+		     && hipMalloc((void**) &LR.touched, sizeof(uint16_t) * (size_t) nb * lanes) == hipSuccess
+		     && hipMalloc((void**) &LR.hist, sizeof(double) * 8 * (size_t) nb * nj) == hipSuccess
+		     && hipMalloc((void**) &LR.hcounts, sizeof(unsigned long long) * 2 * (size_t) nb * nj) == hipSuccess;
+		if(ok)
+			ok = hipMemsetAsync(LR.rows, 0, sizeof(double) * 2 * (size_t) nb * lanes, ctx->stream) == hipSuccess
+			     && hipMemsetAsync(LR.counts, 0, sizeof(uint32_t) * (size_t) nb * lanes, ctx->stream) == hipSuccess
+			     && hipMemsetAsync(LR.hist, 0, sizeof(double) * 8 * (size_t) nb * nj, ctx->stream) == hipSuccess
+			     && hipMemsetAsync(LR.hcounts, 0, sizeof(unsigned long long) * 2 * (size_t) nb * nj, ctx->stream) == hipSuccess;
+	}
+	if(!ok)
+		set_err("device allocation for a batched launch failed");
+	float kms = 0.0f;
+	if(ok)
+	{
+		const DmsHaloView H = halo_view(ctx, d0.halo);
+		(void) hipGetLastError();
+		for(size_t k = 0; k < nj; k++)
+		{
+			const size_t nk = (size_t) jobs[k]->desc.max_trajectories;
+			hipLaunchKernelGGL(dms_ic_kernel, dim3((unsigned) ((nk + 255) / 256)), dim3(256), 0, ctx->stream, H, d0.seed, d0.point_id,
+			                   jobs[k]->desc.first_trajectory_id, (uint32_t) nk, d0.r_max_rsun * DMS_rSun, B.ics + LJ.start[k],
+			                   B.ok + LJ.start[k]);
+		}
+		hipError_t e = run_trajectories(ctx, pt, J, B, n, want_rows, want_recs, LJ, want_rows ? &LR : nullptr, &kms);
+		if(e != hipSuccess)
+		{
+			set_err(std::string("trajectory kernel: ") + hipGetErrorString(e));
+			ok = false;
+		}
+	}
+	if(ok)
+	{
+		std::vector<DmsSummary> hsum(n);
+		hipMemcpy(hsum.data(), B.summ, sizeof(DmsSummary) * n, hipMemcpyDeviceToHost);
+		std::vector<dms_traj_record> hrec;
+		if(want_recs)
+		{
+			hrec.resize(n);
+			hipMemcpy(hrec.data(), B.recs, sizeof(dms_traj_record) * n, hipMemcpyDeviceToHost);
+		}
+		const double elapsed_sec = (now_ms() - t_start) * 1e-3;
+		for(size_t k = 0; k < nj; k++)
+		{
+			dms_job* job = jobs[k];
+			RoundReplay R;
+			R.init(job->desc, pt, &job->C);
+			const size_t nk = (size_t) job->desc.max_trajectories;
+			size_t counted = 0;
+			for(size_t i = 0; i < nk; i++)
+			{
+				if(!R.open_round())
+					break;
+				account_trajectory(job, R, hsum[LJ.start[k] + i], elapsed_sec);
+				if(job->desc.record_trajectories)
+					job->records.push_back(hrec[LJ.start[k] + i]);
+				counted = i + 1;
+				R.close_one();
+			}
+			R.open_round();
+			R.finish();
+			job->C.simulated_trajectories = nk;
+			job->kernel_ms = kms;
+			job->launches = 1.0;
+			job->batch_size = (double) nj;
+			if(want_rows)
+			{
+				hipMemcpy(job->hist.data(), LR.hist + (size_t) k * 8 * nb, sizeof(double) * 8 * nb, hipMemcpyDeviceToHost);
+				hipMemcpy(job->counts.data(), LR.hcounts + (size_t) k * 2 * nb, sizeof(unsigned long long) * 2 * nb,
+				          hipMemcpyDeviceToHost);
+			}
+			job->done_traj = job->C.trajectories;
+			job->done_captured = job->C.captured;
+			job->done_scat = job->C.total_scatterings;
+			job->needs_rerun = want_rows && counted < nk;
+		}
+		const double total = (now_ms() - t_start) * 1e-3;
+		for(dms_job* job : jobs)
+		{
+			job->C.computing_time_sec = total;
+			job->other_ms = total * 1e3 - kms;
+		}
+	}
+	hipFree(LR.rows);
+	hipFree(LR.counts);
+	hipFree(LR.touched);
+	hipFree(LR.hist);
+	hipFree(LR.hcounts);
+	LJ.release();
+	B.release();
+	/* A run-level fuse stopped a job before its last trajectory: its histogram
+	 * sums hold trajectories past the stop, so recount it exactly (chunked). */
+	for(dms_job* job : jobs)
+		if(job->status == DMS_OK && job->needs_rerun)
+			run_job_chunked(job);
+}
+
+/* Largest slot count of one batched launch (initial conditions and summaries
+ * of every trajectory stay resident: about 150 B each). */
+#define DMS_BATCH_MAX_SLOTS ((uint64_t) 1 << 27)
+
+bool batchable(const dms_job_desc& d)
+{
+	return d.capture_target == 0 && d.max_trajectories > 0 && d.max_trajectories <= DMS_BATCH_MAX_SLOTS
+	       && job_view(d).nbins <= 65536;
+}
+
+bool same_batch_config(const dms_job_desc& a, const dms_job_desc& b)
+{
+	return a.mode == b.mode && a.precision == b.precision && a.seed == b.seed && a.point_id == b.point_id
+	       && a.max_scatterings == b.max_scatterings && a.max_free_steps == b.max_free_steps
+	       && a.rk_attempt_budget == b.rk_attempt_budget && a.r_max_rsun == b.r_max_rsun && a.n_bins == b.n_bins
+	       && memcmp(&a.halo, &b.halo, sizeof(dms_halo)) == 0;
+}
+
+void run_batch_worker(std::shared_ptr<dms_batch> batch)
+{
+	dms_ctx* ctx = batch->jobs[0]->pt->ctx;
+	{
+		std::lock_guard<std::mutex> lock(ctx->mu);
+		if(batch->jobs.size() == 1 && !batchable(batch->jobs[0]->desc))
+			run_job_chunked(batch->jobs[0]);
+		else
+			run_batch_fixed(batch->jobs);
+	}
+	for(dms_job* j : batch->jobs)
+		j->finished = 1;
 }
 
 } // namespace
@@ -721,10 +988,17 @@ int dms_create(dms_ctx** out, int device, const double* raw_rows, size_t n_rows)
 		if(hipGetDeviceProperties(&prop, device) != hipSuccess)
 			return cleanup("hipGetDeviceProperties failed");
 		ctx->n_cu = prop.multiProcessorCount;
+		const size_t lds = dms_lds_bytes(ctx->T.n);
+		int max_lds = 0;
+		if(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess || (size_t) max_lds < lds + 16384)
+			return cleanup("solar profile too large for the LDS-resident Mass/escape-speed tables");
+		if(hipFuncSetAttribute((const void*) dms_traj_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds) != hipSuccess)
+			return cleanup("hipFuncSetAttribute(MaxDynamicSharedMemorySize) failed");
 		int blocks = 0;
-		if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, dms_traj_kernel, DMS_TRAJ_BLOCK, 0) != hipSuccess)
+		if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, dms_traj_kernel, DMS_TRAJ_BLOCK, lds) != hipSuccess)
 			blocks = 1;
 		ctx->traj_blocks_per_cu = blocks > 0 ? blocks : 1;
+		ctx->traj_lds = lds;
 		const dms::SolarTables& T = ctx->T;
 		if(dev_alloc_copy(&ctx->d_x, T.x.data(), T.x.size()) != hipSuccess
 		   || dev_alloc_copy(&ctx->d_mass, T.mass.data(), T.mass.size()) != hipSuccess
@@ -879,6 +1153,7 @@ int dms_point_create(dms_ctx* ctx, const dms_point_desc* desc, dms_point** out)
 		}
 		else
 		{
+			(void) hipGetLastError();
 			hipLaunchKernelGGL(dms_grid_kernel, dim3((unsigned) ((cells + 255) / 256)), dim3(256), 0, ctx->stream, pt->dev, ctx->dev,
 			                   pt->N, pt->N, DMS_V_MAX, pt->d_grid);
 			if(hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess)
@@ -939,6 +1214,7 @@ int dms_point_rates(const dms_point* pt, int which, const double* r, const doubl
 	DMS_HIP(ctx, hipMalloc((void**) &dout, sizeof(double) * n));
 	hipMemcpy(dr, r, sizeof(double) * n, hipMemcpyHostToDevice);
 	hipMemcpy(dv, v, sizeof(double) * n, hipMemcpyHostToDevice);
+	(void) hipGetLastError();
 	hipLaunchKernelGGL(dms_rates_kernel, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, ctx->stream, pt->dev, ctx->dev, which, dr, dv,
 	                   dout, n);
 	hipError_t e = hipGetLastError();
@@ -975,6 +1251,7 @@ int dms_initial_conditions(dms_ctx* ctx, const dms_halo* halo, uint32_t seed, ui
 	DMS_HIP(ctx, hipMalloc((void**) &d_ev, sizeof(dms_event) * n));
 	DMS_HIP(ctx, hipMalloc((void**) &d_ok, sizeof(int32_t) * n));
 	const double R = (r_final_rsun > 0.0) ? r_final_rsun * DMS_rSun : 0.0;
+	(void) hipGetLastError();
 	hipLaunchKernelGGL(dms_ic_kernel, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, ctx->stream, H, seed, point_id, first_id,
 	                   (uint32_t) n, R, d_ev, d_ok);
 	hipError_t e = hipGetLastError();
@@ -1024,7 +1301,7 @@ int dms_simulate_events(dms_point* pt, const dms_job_desc* job, const dms_event*
 	hipMemcpy(B.ics, initial, sizeof(dms_event) * n, hipMemcpyHostToDevice);
 	hipMemset(B.ok, 0xff, sizeof(int32_t) * n);
 	float ms = 0.0f;
-	hipError_t e = run_trajectories(ctx, pt, J, B, n, rows, true, &ms);
+	hipError_t e = run_trajectories_single(ctx, pt, J, B, n, rows, true, J.first_id, &ms);
 	if(e == hipSuccess)
 		e = hipMemcpy(records, B.recs, sizeof(dms_traj_record) * n, hipMemcpyDeviceToHost);
 	if(e == hipSuccess && hist_rows)
@@ -1040,24 +1317,85 @@ int dms_simulate_events(dms_point* pt, const dms_job_desc* job, const dms_event*
 	return DMS_OK;
 }
 
+namespace
+{
+std::string check_submit(const dms_job_desc& d)
+{
+	std::string m = check_job(d);
+	if(m.empty())
+		m = check_halo(d.halo);
+	if(m.empty() && d.max_trajectories == 0 && d.capture_target == 0)
+		m = "a job needs max_trajectories or capture_target";
+	return m;
+}
+
+void start_batch(dms_point* pt, const dms_job_desc* descs, size_t n, dms_job** out)
+{
+	auto batch = std::make_shared<dms_batch>();
+	for(size_t k = 0; k < n; k++)
+	{
+		dms_job* job = new dms_job();
+		job->pt = pt;
+		job->desc = descs[k];
+		job->batch = batch;
+		memset(&job->C, 0, sizeof(job->C));
+		batch->jobs.push_back(job);
+		out[k] = job;
+	}
+	batch->worker = std::thread(run_batch_worker, batch);
+}
+
+void join_batch(dms_job* job)
+{
+	dms_batch* b = job->batch.get();
+	if(b == nullptr)
+		return;
+	std::lock_guard<std::mutex> lock(b->mu);
+	if(!b->joined && b->worker.joinable())
+		b->worker.join();
+	b->joined = true;
+}
+} // namespace
+
 int dms_job_submit(dms_point* pt, const dms_job_desc* desc, dms_job** out)
 {
 	if(pt == nullptr || desc == nullptr || out == nullptr)
 		return fail(pt ? pt->ctx : nullptr, DMS_ERR_ARG, "dms_job_submit: null argument");
 	*out = nullptr;
-	std::string m = check_job(*desc);
-	if(m.empty())
-		m = check_halo(desc->halo);
+	const std::string m = check_submit(*desc);
 	if(!m.empty())
 		return fail(pt->ctx, DMS_ERR_ARG, "dms_job_submit: " + m);
-	if(desc->max_trajectories == 0 && desc->capture_target == 0)
-		return fail(pt->ctx, DMS_ERR_ARG, "dms_job_submit: a job needs max_trajectories or capture_target");
-	dms_job* job = new dms_job();
-	job->pt = pt;
-	job->desc = *desc;
-	memset(&job->C, 0, sizeof(job->C));
-	job->worker = std::thread(run_job, job);
-	*out = job;
+	start_batch(pt, desc, 1, out);
+	return DMS_OK;
+}
+
+int dms_job_submit_batch(dms_point* pt, const dms_job_desc* descs, size_t n, dms_job** out)
+{
+	if(pt == nullptr || descs == nullptr || out == nullptr || n == 0)
+		return fail(pt ? pt->ctx : nullptr, DMS_ERR_ARG, "dms_job_submit_batch: null argument or empty batch");
+	for(size_t k = 0; k < n; k++)
+		out[k] = nullptr;
+	uint64_t total = 0;
+	for(size_t k = 0; k < n; k++)
+	{
+		const std::string m = check_submit(descs[k]);
+		if(!m.empty())
+			return fail(pt->ctx, DMS_ERR_ARG, "dms_job_submit_batch: job " + std::to_string(k) + ": " + m);
+		if(n > 1 && !batchable(descs[k]))
+			return fail(pt->ctx, DMS_ERR_ARG,
+			            "dms_job_submit_batch: job " + std::to_string(k)
+			                + ": batched jobs need a fixed trajectory count (max_trajectories > 0, capture_target 0) and at most "
+			                + std::to_string(DMS_BATCH_MAX_SLOTS) + " trajectories and 65536 bins");
+		if(n > 1 && !same_batch_config(descs[0], descs[k]))
+			return fail(pt->ctx, DMS_ERR_ARG,
+			            "dms_job_submit_batch: job " + std::to_string(k)
+			                + " differs from job 0 in a field other than max_trajectories, first_trajectory_id or record_trajectories");
+		total += descs[k].max_trajectories;
+	}
+	if(n > 1 && total > DMS_BATCH_MAX_SLOTS)
+		return fail(pt->ctx, DMS_ERR_ARG, "dms_job_submit_batch: the batch holds more than "
+		                                      + std::to_string(DMS_BATCH_MAX_SLOTS) + " trajectories");
+	start_batch(pt, descs, n, out);
 	return DMS_OK;
 }
 
@@ -1077,8 +1415,7 @@ int dms_job_wait(dms_job* job, dms_job_status* st)
 {
 	if(job == nullptr)
 		return fail(nullptr, DMS_ERR_ARG, "dms_job_wait: null argument");
-	if(job->worker.joinable())
-		job->worker.join();
+	join_batch(job);
 	if(st)
 		dms_job_poll(job, st);
 	if(job->status != DMS_OK)
@@ -1138,8 +1475,7 @@ void dms_job_destroy(dms_job* job)
 {
 	if(job == nullptr)
 		return;
-	if(job->worker.joinable())
-		job->worker.join();
+	join_batch(job);
 	delete job;
 }
 
@@ -1158,6 +1494,7 @@ int dms_probe_math(dms_ctx* ctx, int fn, const double* x, const double* y, doubl
 	DMS_HIP(ctx, hipMalloc((void**) &dout, sizeof(double) * n));
 	hipMemcpy(dx, x, sizeof(double) * n, hipMemcpyHostToDevice);
 	hipMemcpy(dy, y ? y : x, sizeof(double) * n, hipMemcpyHostToDevice);
+	(void) hipGetLastError();
 	hipLaunchKernelGGL(dms_probe_kernel, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, ctx->stream, fn, dx, dy, dout, n);
 	hipError_t e = hipGetLastError();
 	if(e == hipSuccess)

@@ -201,6 +201,7 @@ def _declare(lib):
         "dms_simulate_events": (C.c_int, [vp, _P(JobDesc), _P(Event), C.c_size_t, _P(TrajRecord),
                                           _P(C.c_double)]),
         "dms_job_submit": (C.c_int, [vp, _P(JobDesc), _P(vp)]),
+        "dms_job_submit_batch": (C.c_int, [vp, _P(JobDesc), C.c_size_t, _P(vp)]),
         "dms_job_poll": (C.c_int, [vp, _P(JobStatus)]),
         "dms_job_wait": (C.c_int, [vp, _P(JobStatus)]),
         "dms_job_read": (C.c_int, [vp, _P(Counters), _P(C.c_double), _P(C.c_uint64), C.c_size_t,
@@ -221,7 +222,7 @@ def _declare(lib):
 EXPORTED = ["dms_abi_version", "dms_create", "dms_destroy", "dms_last_error", "dms_load_solar_file",
             "dms_solar_eval", "dms_isotopes", "dms_constant", "dms_point_create", "dms_point_destroy",
             "dms_point_rate_grid", "dms_point_cross_sections", "dms_point_rates", "dms_initial_conditions",
-            "dms_simulate_events", "dms_job_submit", "dms_job_poll", "dms_job_wait", "dms_job_read",
+            "dms_simulate_events", "dms_job_submit", "dms_job_submit_batch", "dms_job_poll", "dms_job_wait", "dms_job_read",
             "dms_job_timing", "dms_job_destroy", "dms_probe_math"]
 
 _lib = None
@@ -341,31 +342,50 @@ class Point:
         self.ctx.check(lib().dms_simulate_events(self.h, C.byref(job), events, n, recs, rp), "dms_simulate_events")
         return recs, rows
 
+    def _collect(self, h, job, want_records):
+        L = lib()
+        st = JobStatus()
+        self.ctx.check(L.dms_job_wait(h, C.byref(st)), "dms_job_wait")
+        nb = job.n_bins
+        hist = np.zeros((8, nb), dtype=np.float64)
+        counts = np.zeros((2, nb), dtype=np.uint64)
+        cnt = Counters()
+        n_ev = C.c_size_t(0)
+        n_rec = C.c_size_t(0)
+        L.dms_job_read(h, C.byref(cnt), None, None, 0, None, 0, C.byref(n_ev), None, 0, C.byref(n_rec))
+        evs = (EvapEvent * max(1, n_ev.value))()
+        recs = record_array(max(1, n_rec.value)) if want_records else None
+        self.ctx.check(L.dms_job_read(h, C.byref(cnt), dptr(hist), counts.ctypes.data_as(_P(C.c_uint64)), nb,
+                                      evs, n_ev.value, C.byref(n_ev), recs,
+                                      n_rec.value if want_records else 0, C.byref(n_rec)), "dms_job_read")
+        kt = C.c_double(0)
+        kl = C.c_double(0)
+        ot = C.c_double(0)
+        L.dms_job_timing(h, C.byref(kt), C.byref(kl), C.byref(ot))
+        return dict(counters=cnt, hist=hist, counts=counts, events=list(evs)[:n_ev.value],
+                    records=recs[:n_rec.value] if want_records else None,
+                    kernel_ms=kt.value, launches=kl.value, other_ms=ot.value)
+
     def run_job(self, job, want_records=True):
+        """One Generate_Data job (dms_job_submit + wait + read)."""
         L = lib()
         h = C.c_void_p()
         self.ctx.check(L.dms_job_submit(self.h, C.byref(job), C.byref(h)), "dms_job_submit")
         try:
-            st = JobStatus()
-            self.ctx.check(L.dms_job_wait(h, C.byref(st)), "dms_job_wait")
-            nb = job.n_bins
-            hist = np.zeros((8, nb), dtype=np.float64)
-            counts = np.zeros((2, nb), dtype=np.uint64)
-            cnt = Counters()
-            n_ev = C.c_size_t(0)
-            n_rec = C.c_size_t(0)
-            L.dms_job_read(h, C.byref(cnt), None, None, 0, None, 0, C.byref(n_ev), None, 0, C.byref(n_rec))
-            evs = (EvapEvent * max(1, n_ev.value))()
-            recs = record_array(max(1, n_rec.value)) if want_records else None
-            self.ctx.check(L.dms_job_read(h, C.byref(cnt), dptr(hist), counts.ctypes.data_as(_P(C.c_uint64)), nb,
-                                          evs, n_ev.value, C.byref(n_ev), recs,
-                                          n_rec.value if want_records else 0, C.byref(n_rec)), "dms_job_read")
-            kt = C.c_double(0)
-            kl = C.c_double(0)
-            ot = C.c_double(0)
-            L.dms_job_timing(h, C.byref(kt), C.byref(kl), C.byref(ot))
-            return dict(counters=cnt, hist=hist, counts=counts, events=list(evs)[:n_ev.value],
-                        records=recs[:n_rec.value] if want_records else None,
-                        kernel_ms=kt.value, launches=kl.value, other_ms=ot.value)
+            return self._collect(h, job, want_records)
         finally:
             L.dms_job_destroy(h)
+
+    def run_batch(self, jobs, want_records=False):
+        """Fixed-count jobs of this point in one trajectory launch
+        (dms_job_submit_batch); returns one result dict per job."""
+        L = lib()
+        n = len(jobs)
+        descs = (JobDesc * n)(*jobs)
+        hs = (C.c_void_p * n)()
+        self.ctx.check(L.dms_job_submit_batch(self.h, descs, n, hs), "dms_job_submit_batch")
+        try:
+            return [self._collect(C.c_void_p(hs[k]), jobs[k], want_records) for k in range(n)]
+        finally:
+            for k in range(n):
+                L.dms_job_destroy(C.c_void_p(hs[k]))

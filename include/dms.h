@@ -282,6 +282,17 @@ int dms_simulate_events(dms_point* point, const dms_job_desc* job, const dms_eve
 /* ---- jobs (Simulation_Data::Generate_Data, src/Data_Generation.cpp:488-972) -- */
 
 int dms_job_submit(dms_point* point, const dms_job_desc* desc, dms_job** job);
+/* Several fixed-count jobs of one point (one Generate_Data call per job, as
+ * separate ranks or separate runs would make them) simulated in ONE trajectory
+ * launch, so that the long trajectories of one job overlap the bulk of the
+ * others.  Every job needs max_trajectories > 0, capture_target 0, and the
+ * same descriptor as jobs[0] apart from max_trajectories, first_trajectory_id
+ * and record_trajectories; the batch holds at most 2^27 trajectories and
+ * 65536 bins.  Each job's results equal those of dms_job_submit on the same
+ * descriptor (counters, counts and events exactly; the histogram sums to the
+ * rounding of their summation order).  jobs[k] receives the handle of descs[k];
+ * every handle is waited on, read and destroyed separately. */
+int dms_job_submit_batch(dms_point* point, const dms_job_desc* descs, size_t n, dms_job** jobs);
 int dms_job_poll(dms_job* job, dms_job_status* status);
 int dms_job_wait(dms_job* job, dms_job_status* status);
 /* hist: 8 x n_bins doubles in the order of bincount.txt columns

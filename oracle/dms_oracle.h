@@ -70,6 +70,15 @@ int or_simulate_events(const or_point* P, const dms_job_desc* job, const dms_eve
 /* Trajectory_Simulator::Scatter on one event with an explicit Philox stream. */
 int or_scatter(const or_point* P, dms_event* e, uint32_t seed, uint32_t point_id, uint64_t traj_id);
 
+/* Sampler hooks (closed-form checks): Philox block; n draws of the CM angle
+ * sampler, of Sample_Target, of Sample_Target_Velocity; one New_DM_Velocity. */
+void or_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+void or_sample_angles(const or_point* P, uint32_t seed, int target, double v, size_t n, double* out);
+void or_sample_targets(const or_point* P, uint32_t seed, double r, double v, size_t n, int32_t* out);
+size_t or_sample_target_velocities(uint32_t seed, double temperature, double target_mass, const double* vel_DM, size_t n,
+                                   double* out);
+int or_new_dm_velocity(uint32_t seed, double cos_a, double mDM, double mT, double* vel, const double* vT);
+
 /* Simulation_Data::Generate_Data. */
 void or_probe_math(int fn, const double* x, const double* y, double* out, size_t n);
 int or_run_job(const or_point* P, const dms_job_desc* job, int n_threads, dms_counters* counters,

@@ -1357,3 +1357,44 @@ int or_simulate_one(const or_sim_cfg* cfg, const dms_event* ic, uint32_t seed, u
 	rec->flags = flags;
 	return DMS_OK;
 }
+
+/* ---- sampler hooks for the closed-form checks of the test suite ------------
+ * Each draws n samples from one Philox stream (key {seed, 0}, trajectory 0). */
+void or_philox(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) { dm_philox4x32_10(ctr, key, out); }
+
+void or_sample_angles(const or_point* P, uint32_t seed, int target, double v, size_t n, double* out)
+{
+	dm_rng rng;
+	dm_rng_init(&rng, seed, 0, 0, DMS_STREAM_SIM);
+	for(size_t i = 0; i < n; i++)
+		out[i] = sample_angle(P, &rng, target, v);
+}
+
+void or_sample_targets(const or_point* P, uint32_t seed, double r, double v, size_t n, int32_t* out)
+{
+	dm_rng rng;
+	dm_rng_init(&rng, seed, 0, 0, DMS_STREAM_SIM);
+	for(size_t i = 0; i < n; i++)
+		out[i] = sample_target(P, &rng, r, v);
+}
+
+/* out: 3n components; returns the number of failed draws */
+size_t or_sample_target_velocities(uint32_t seed, double temperature, double target_mass, const double* vel_DM, size_t n,
+                                   double* out)
+{
+	dm_rng rng;
+	dm_rng_init(&rng, seed, 0, 0, DMS_STREAM_SIM);
+	size_t failed = 0;
+	uint64_t it = 0;
+	for(size_t i = 0; i < n; i++)
+		if(!sample_target_velocity(&rng, temperature, target_mass, vel_DM, out + 3 * i, &it))
+			failed++;
+	return failed;
+}
+
+int or_new_dm_velocity(uint32_t seed, double cos_a, double mDM, double mT, double* vel, const double* vT)
+{
+	dm_rng rng;
+	dm_rng_init(&rng, seed, 0, 0, DMS_STREAM_SIM);
+	return new_dm_velocity(&rng, cos_a, mDM, mT, vel, vT);
+}

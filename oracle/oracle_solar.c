@@ -173,7 +173,7 @@ or_solar* or_solar_create(const double* raw_rows, size_t n_rows)
 			last = S->raw[(size_t) (n - 2) * OR_NCOL + c];
 		S->raw[(size_t) (n - 1) * OR_NCOL + c] = last;
 	}
-	double* y = (double*) malloc(sizeof(double) * (size_t) n);
+	double* y = (double*) calloc((size_t) n, sizeof(double));
 	for(int i = 0; i < n; i++)
 		S->r[i] = S->raw[i * OR_NCOL + 1];
 #define OR_COLUMN(col)                           \

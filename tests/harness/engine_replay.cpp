@@ -158,13 +158,39 @@ void rp_initial_conditions(rp_ctx* c, const dms_halo* h, uint32_t seed, uint32_t
 	}
 }
 
-/* Serial replay of dms_traj_kernel for n given initial events. */
-int rp_simulate(rp_point* p, const dms_job_desc* job, const dms_event* ics, size_t n, dms_traj_record* recs, double* rows)
+/* One trajectory through the kernel's phase drivers, with the wave-cooperative
+ * Sample_Target done serially (same per-target formulas, same sum order). */
+static void replay_lane(DmsLane& L, const DmsLaneCtx& c, const dms_event& ic, bool shift_ok, dms_traj_record* rec, DmsSummary* sum)
 {
-	DmsLaneCtx c;
-	c.S = p->ctx->T.view();
-	c.P = p->view;
-	DmsJobView& J = c.J;
+	double rates[DMS_MAX_TARGETS];
+	dms_lane_start(L, c, ic, shift_ok, rec, sum);
+	if(L.phase == PH_NEED)
+		return;
+	dms_lane_advance_b(L, c);
+	while(L.phase != PH_NEED)
+	{
+		dms_lane_advance_a(L, c);
+		if(L.phase == PH_SCAT)
+		{
+			for(int k = 0; k < DMS_MAX_TARGETS; k++)
+				rates[k] = 0.0;
+			for(int k = 0; k < c.S.n_iso; k++)
+				rates[k] = dms_rate_target(c.P, c.S, L.sc_r, L.sc_v, k);
+			rates[DMS_MAX_TARGETS - 1] = dms_rate_target(c.P, c.S, L.sc_r, L.sc_v, DMS_MAX_TARGETS - 1);
+			L.sc_target = dms_choose_target(rates, c.S.n_iso, L.sc_xi);
+		}
+		dms_lane_finish_scatter(L, c);
+		if(L.phase == PH_FIN)
+		{
+			dms_lane_finish(L, c, ic, rec, sum);
+			break;
+		}
+		dms_lane_advance_b(L, c);
+	}
+}
+
+static void fill_job_view(DmsJobView& J, const dms_job_desc* job)
+{
 	J.max_time_steps = job->max_free_steps;
 	J.max_scatterings = job->max_scatterings;
 	J.rk_budget = job->rk_attempt_budget;
@@ -175,39 +201,79 @@ int rp_simulate(rp_point* p, const dms_job_desc* job, const dms_event* ics, size
 	J.seed = job->seed;
 	J.point_id = job->point_id;
 	J.first_id = job->first_trajectory_id;
-	c.R.rows = J.terminate_on_capture ? nullptr : rows;
-	c.R.counts = nullptr;
-	if(rows)
-		memset(rows, 0, sizeof(double) * 2 * (size_t) J.nbins * n);
-	double rates[DMS_MAX_TARGETS];
+}
+
+/* Serial replay of the per-lane histogram mode: nlanes scratch rows reused
+ * round-robin, trajectories finished in id order, class sums flushed at each
+ * trajectory end (Parameter-point jobs).  hist: 8 x nbins, hcounts: 2 x nbins. */
+int rp_lane_hist(rp_point* p, const dms_job_desc* job, const dms_event* ics, const int32_t* ok, size_t n, size_t nlanes,
+                 double* hist, unsigned long long* hcounts)
+{
+	DmsLaneCtx c;
+	c.S = p->ctx->T.view();
+	c.P = p->view;
+	fill_job_view(c.J, job);
+	const size_t nb = (size_t) c.J.nbins;
+	if(c.J.terminate_on_capture || nlanes == 0 || nb > 65536)
+		return -1;
+	std::vector<double> rows(2 * nb * nlanes, 0.0);
+	std::vector<uint32_t> counts(nb * nlanes, 0u);
+	std::vector<uint16_t> touched(nb * nlanes, 0);
+	memset(hist, 0, sizeof(double) * 8 * nb);
+	memset(hcounts, 0, sizeof(unsigned long long) * 2 * nb);
+	c.R.rows = rows.data();
+	c.R.counts = counts.data();
+	c.R.touched = touched.data();
+	c.R.per_lane = 1;
+	c.R.hist = hist;
+	c.R.hcounts = hcounts;
 	for(size_t i = 0; i < n; i++)
 	{
 		DmsLane L;
 		memset(&L, 0, sizeof(L));
 		L.cur_bin = -1;
 		L.slot = (uint32_t) i;
-		dms_lane_start(L, c, ics[i], true, &recs[i], nullptr);
-		dms_lane_advance_b(L, c);
-		while(L.phase != PH_NEED)
-		{
-			dms_lane_advance_a(L, c);
-			if(L.phase == PH_SCAT)
-			{
-				for(int k = 0; k < DMS_MAX_TARGETS; k++)
-					rates[k] = 0.0;
-				for(int k = 0; k < c.S.n_iso; k++)
-					rates[k] = dms_rate_target(c.P, c.S, L.sc_r, L.sc_v, k);
-				rates[DMS_MAX_TARGETS - 1] = dms_rate_target(c.P, c.S, L.sc_r, L.sc_v, DMS_MAX_TARGETS - 1);
-				L.sc_target = dms_choose_target(rates, c.S.n_iso, L.sc_xi);
-			}
-			dms_lane_finish_scatter(L, c);
-			if(L.phase == PH_FIN)
-			{
-				dms_lane_finish(L, c, ics[i], &recs[i], nullptr);
-				break;
-			}
-			dms_lane_advance_b(L, c);
-		}
+		L.row = (uint32_t) (i % nlanes);
+		L.job = 0;
+		L.id = c.J.first_id + i;
+		DmsSummary s;
+		replay_lane(L, c, ics[i], ok[i] != 0, nullptr, &s);
+	}
+	for(size_t k = 0; k < 2 * nb * nlanes; k++)
+		if(rows[k] != 0.0)
+			return -2; /* scratch not cleared after a flush */
+	for(size_t k = 0; k < nb * nlanes; k++)
+		if(counts[k] != 0u)
+			return -2;
+	return 0;
+}
+
+/* Serial replay of dms_traj_kernel for n given initial events. */
+int rp_simulate(rp_point* p, const dms_job_desc* job, const dms_event* ics, size_t n, dms_traj_record* recs, double* rows)
+{
+	DmsLaneCtx c;
+	c.S = p->ctx->T.view();
+	c.P = p->view;
+	DmsJobView& J = c.J;
+	fill_job_view(J, job);
+	c.R.rows = J.terminate_on_capture ? nullptr : rows;
+	c.R.counts = nullptr;
+	c.R.touched = nullptr;
+	c.R.per_lane = 0;
+	c.R.hist = nullptr;
+	c.R.hcounts = nullptr;
+	if(rows)
+		memset(rows, 0, sizeof(double) * 2 * (size_t) J.nbins * n);
+	for(size_t i = 0; i < n; i++)
+	{
+		DmsLane L;
+		memset(&L, 0, sizeof(L));
+		L.cur_bin = -1;
+		L.slot = (uint32_t) i;
+		L.row = (uint32_t) i;
+		L.job = 0;
+		L.id = J.first_id + i;
+		replay_lane(L, c, ics[i], true, &recs[i], nullptr);
 	}
 	return 0;
 }

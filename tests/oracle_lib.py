@@ -65,6 +65,13 @@ def lib():
             "or_simulate_events": (C.c_int, [vp, _P(dmsun.JobDesc), _P(dmsun.Event), C.c_size_t,
                                              _P(dmsun.TrajRecord), _P(C.c_double), _P(C.c_uint32), C.c_int]),
             "or_scatter": (C.c_int, [vp, _P(dmsun.Event), C.c_uint32, C.c_uint32, C.c_uint64]),
+            "or_philox": (None, [_P(C.c_uint32), _P(C.c_uint32), _P(C.c_uint32)]),
+            "or_sample_angles": (None, [vp, C.c_uint32, C.c_int, C.c_double, C.c_size_t, _P(C.c_double)]),
+            "or_sample_targets": (None, [vp, C.c_uint32, C.c_double, C.c_double, C.c_size_t, _P(C.c_int32)]),
+            "or_sample_target_velocities": (C.c_size_t, [C.c_uint32, C.c_double, C.c_double, _P(C.c_double),
+                                                         C.c_size_t, _P(C.c_double)]),
+            "or_new_dm_velocity": (C.c_int, [C.c_uint32, C.c_double, C.c_double, C.c_double, _P(C.c_double),
+                                             _P(C.c_double)]),
             "or_probe_math": (None, [C.c_int, _P(C.c_double), _P(C.c_double), _P(C.c_double), C.c_size_t]),
             "or_run_job": (C.c_int, [vp, _P(dmsun.JobDesc), C.c_int, _P(dmsun.Counters), _P(C.c_double),
                                      _P(C.c_uint64), _P(dmsun.EvapEvent), C.c_size_t, _P(C.c_size_t),

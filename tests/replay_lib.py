@@ -43,6 +43,8 @@ def lib():
             "rp_grid": (C.c_int, [vp, _P(C.c_double), C.c_size_t]),
             "rp_initial_conditions": (None, [vp, _P(dmsun.Halo), C.c_uint32, C.c_uint32, C.c_uint64, C.c_size_t,
                                              C.c_double, _P(dmsun.Event), _P(C.c_int32)]),
+            "rp_lane_hist": (C.c_int, [vp, _P(dmsun.JobDesc), _P(dmsun.Event), _P(C.c_int32), C.c_size_t,
+                                       C.c_size_t, _P(C.c_double), _P(C.c_uint64)]),
             "rp_simulate": (C.c_int, [vp, _P(dmsun.JobDesc), _P(dmsun.Event), C.c_size_t, _P(dmsun.TrajRecord),
                                       _P(C.c_double)]),
         }
@@ -98,3 +100,14 @@ class ReplayPoint:
         rows = np.zeros((n, 2, job.n_bins)) if want_rows else None
         assert lib().rp_simulate(self.h, C.byref(job), events, n, recs, dmsun.dptr(rows) if want_rows else None) == 0
         return recs, rows
+
+    def lane_hist(self, job, events, ok, nlanes=3):
+        """Class sums through per-lane scratch rows (serial, id order)."""
+        nb = job.n_bins
+        hist = np.zeros((8, nb))
+        counts = np.zeros((2, nb), dtype=np.uint64)
+        ok = np.ascontiguousarray(ok, dtype=np.int32)
+        st = lib().rp_lane_hist(self.h, C.byref(job), events, ok.ctypes.data_as(_P(C.c_int32)), len(events), nlanes,
+                                dmsun.dptr(hist), counts.ctypes.data_as(_P(C.c_uint64)))
+        assert st == 0, st
+        return hist, counts

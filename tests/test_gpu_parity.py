@@ -23,6 +23,8 @@ RK_BUDGET = 20_000_000
 
 
 def rec_bytes(recs):
+    if isinstance(recs, list):
+        recs = (dmsun.TrajRecord * len(recs))(*recs)
     return np.ctypeslib.as_array(recs).view(np.uint8).reshape(len(recs), -1)
 
 
@@ -126,6 +128,10 @@ def test_job_matches_oracle(gpu_ctx, oracle_solar, mode, desc_args, kw):
         job.capture_target = 0
     res = gpu_ctx.point(desc).run_job(job)
     ores = oracle_lib.Point(oracle_solar, desc).run_job(job, record_capacity=200000)
+    assert_job_equal(res, ores)
+
+
+def assert_job_equal(res, ores):
     c, oc = res["counters"], ores["counters"]
     for f, _ in dmsun.Counters._fields_:
         if f in ("computing_time_sec", "simulated_trajectories"):
@@ -140,3 +146,34 @@ def test_job_matches_oracle(gpu_ctx, oracle_solar, mode, desc_args, kw):
     ev_g = [(e.trajectory_id, e.lifetime_unbinding_sec) for e in res["events"]]
     ev_c = [(e.trajectory_id, e.lifetime_unbinding_sec) for e in ores["events"]]
     assert ev_g == ev_c
+
+
+@pytest.mark.parametrize("mode", [dmsun.MODE_PARAMETER_POINT, dmsun.MODE_CAPTURE])
+def test_batch_matches_oracle_and_single(gpu_ctx, oracle_solar, mode):
+    """Jobs streamed through one launch give each job's own results: equal to
+    the oracle's Generate_Data run and to the job submitted alone."""
+    desc = dmsun.make_point(0.3, 1e-36, interpolation_points=1000)
+    sizes = [(1500, 0), (700, 50_000), (2200, 7_000_000)]
+    jobs = [dmsun.make_job(mode=mode, max_trajectories=n, capture_target=0, max_scatterings=10**14, n_bins=700,
+                           first_trajectory_id=first, rk_attempt_budget=RK_BUDGET) for n, first in sizes]
+    print(f"batch mode={mode} sizes={sizes}", flush=True)
+    pt = gpu_ctx.point(desc)
+    batch = pt.run_batch(jobs, want_records=True)
+    opt = oracle_lib.Point(oracle_solar, desc)
+    for job, res in zip(jobs, batch):
+        ores = opt.run_job(job, record_capacity=job.max_trajectories)
+        assert_job_equal(res, ores)
+        assert np.array_equal(rec_bytes(res["records"]), rec_bytes(ores["records"]))
+        alone = pt.run_job(job, want_records=False)
+        assert_job_equal(alone, res)
+
+
+def test_batch_rejects_mixed_descriptors(gpu_ctx):
+    desc = dmsun.make_point(0.3, 1e-36, interpolation_points=200)
+    a = dmsun.make_job(mode=dmsun.MODE_PARAMETER_POINT, max_trajectories=10, capture_target=0, n_bins=100)
+    b = dmsun.make_job(mode=dmsun.MODE_PARAMETER_POINT, max_trajectories=10, capture_target=0, n_bins=200)
+    with pytest.raises(dmsun.DmsError):
+        gpu_ctx.point(desc).run_batch([a, b])
+    c = dmsun.make_job(mode=dmsun.MODE_CAPTURE, max_trajectories=0, capture_target=5)
+    with pytest.raises(dmsun.DmsError):
+        gpu_ctx.point(desc).run_batch([a, c])
