@@ -7,8 +7,8 @@
  * device (hipcc, gfx950) evaluate them to the same bits, which is what makes
  * trajectory-level replay between the CPU restatement and the FP64 kernel
  * bit-exact.  Accuracy against correctly rounded results is <= 2-3 ulp on the
- * domains the path uses (exp, log, sin, cos, erf, acos); the C unit tests
- * measure it against 60-digit references.
+ * domains the path uses (exp, log, sin, cos, erf, acos), measured by the
+ * unit tests against 50-digit references.
  *
  * The polynomial tables live in detmath_tables.h (generated).
  * This header is valid C99 and HIP C++.

@@ -1277,6 +1277,8 @@ int dms_simulate_events(dms_point* pt, const dms_job_desc* job, const dms_event*
 	const std::string jm = check_job(*job);
 	if(!jm.empty())
 		return fail(ctx, DMS_ERR_ARG, "dms_simulate_events: " + jm);
+	if(job->precision == DMS_MIXED)
+		return fail(ctx, DMS_ERR_UNSUPPORTED, "dms_simulate_events: the mixed-precision kernel is not available in this build");
 	if(n == 0)
 		return DMS_OK;
 	if(n > 0x7fffffffu)
@@ -1326,7 +1328,14 @@ std::string check_submit(const dms_job_desc& d)
 		m = check_halo(d.halo);
 	if(m.empty() && d.max_trajectories == 0 && d.capture_target == 0)
 		m = "a job needs max_trajectories or capture_target";
+	if(m.empty() && d.precision == DMS_MIXED)
+		m = "the mixed-precision kernel is not available in this build";
 	return m;
+}
+
+int submit_status(const dms_job_desc& d)
+{
+	return d.precision == DMS_MIXED ? DMS_ERR_UNSUPPORTED : DMS_ERR_ARG;
 }
 
 void start_batch(dms_point* pt, const dms_job_desc* descs, size_t n, dms_job** out)
@@ -1364,7 +1373,7 @@ int dms_job_submit(dms_point* pt, const dms_job_desc* desc, dms_job** out)
 	*out = nullptr;
 	const std::string m = check_submit(*desc);
 	if(!m.empty())
-		return fail(pt->ctx, DMS_ERR_ARG, "dms_job_submit: " + m);
+		return fail(pt->ctx, submit_status(*desc), "dms_job_submit: " + m);
 	start_batch(pt, desc, 1, out);
 	return DMS_OK;
 }
@@ -1380,7 +1389,7 @@ int dms_job_submit_batch(dms_point* pt, const dms_job_desc* descs, size_t n, dms
 	{
 		const std::string m = check_submit(descs[k]);
 		if(!m.empty())
-			return fail(pt->ctx, DMS_ERR_ARG, "dms_job_submit_batch: job " + std::to_string(k) + ": " + m);
+			return fail(pt->ctx, submit_status(descs[k]), "dms_job_submit_batch: job " + std::to_string(k) + ": " + m);
 		if(n > 1 && !batchable(descs[k]))
 			return fail(pt->ctx, DMS_ERR_ARG,
 			            "dms_job_submit_batch: job " + std::to_string(k)
