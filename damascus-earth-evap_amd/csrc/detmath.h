@@ -175,6 +175,28 @@ DM_FN double dm_sin(double x)
 	}
 }
 
+/* sin(x) and cos(x) from one reduction: the same bits as dm_sin and dm_cos. */
+DM_FN void dm_sincos(double x, double* sn, double* cs)
+{
+	if(!dm_isfinite(x))
+	{
+		*sn = dm_nan();
+		*cs = dm_nan();
+		return;
+	}
+	int q;
+	const double r = dm_reduce_pio2(x, &q);
+	const double s = dm_sin_kernel(r);
+	const double c = dm_cos_kernel(r);
+	switch(q)
+	{
+		case 0: *sn = s; *cs = c; break;
+		case 1: *sn = c; *cs = -s; break;
+		case 2: *sn = -s; *cs = -c; break;
+		default: *sn = -c; *cs = s; break;
+	}
+}
+
 DM_FN double dm_cos(double x)
 {
 	if(!dm_isfinite(x))

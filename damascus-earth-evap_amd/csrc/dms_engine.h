@@ -103,6 +103,7 @@ struct DmsLane
 	double eb_t, eb_r[3], eb_v[3];
 	double r_before, v_before, rate_before;
 	int eb_valid; /* eb/r_before/v_before already hold Event_In_3D of the current propagator */
+	int rb_cached; /* rate_before already holds Total_DM_Scattering_Rate(r_before, v_before) */
 	int inner_iter, rk_ok;
 	double last_E; /* capture energy of the last committed event (NaN when not computed) */
 	/* scatter hand-off */
@@ -230,8 +231,8 @@ DMS_HD void dms_prop_event(const DmsProp& p, double& t, double* r, double* v)
 		}
 		return;
 	}
-	const double cos_phi = dm_cos(p.phi);
-	const double sin_phi = dm_sin(p.phi);
+	double sin_phi, cos_phi;
+	dm_sincos(p.phi, &sin_phi, &cos_phi);
 	const double tangential = p.J / safe_radius;
 	const double vx = p.vr * cos_phi - tangential * sin_phi;
 	const double vy = p.vr * sin_phi + tangential * cos_phi;
@@ -582,48 +583,63 @@ DMS_HD double dms_rate_at_fraction(const DmsPointView& P, const DmsSolarView& S,
 	return dms_rate_total(P, S, radius, dm_sqrt(dm_max(0.0, vs)));
 }
 
-/* Scattering_Rate_At_Fraction (:327-345) for n fractions of one step.  Per
- * element the arithmetic is that of dms_rate_at_fraction / dms_rate_total; the
- * bilinear grid reads of all elements are issued before any is used. */
+/* Scattering_Rate_At_Fraction (:327-345) for n fractions of one step, plus
+ * optionally Total_DM_Scattering_Rate (src/Solar_Model.cpp:305-322) at one more
+ * (xr, xv) -- the rate the next step's preparation needs at the end point.
+ * Per element the arithmetic is that of dms_rate_at_fraction / dms_rate_total;
+ * the bilinear grid reads of all elements are issued before any is used. */
 DMS_HD void dms_rates_at_fractions(const DmsPointView& P, const DmsSolarView& S, const double* br, const double* bv,
                                    const double* ar, const double* av, const double* fr, int n, bool use_cached,
-                                   double cached_rate, double* out)
+                                   double cached_rate, double* out, bool extra, double xr, double xv, double* xout)
 {
-	double rf[DMS_MAX_OPTICAL_DEPTH_PIECES + 1], vf[DMS_MAX_OPTICAL_DEPTH_PIECES + 1];
-	size_t lo[DMS_MAX_OPTICAL_DEPTH_PIECES + 1];
-	int mode[DMS_MAX_OPTICAL_DEPTH_PIECES + 1]; /* 0 done, 1 grid, 2 computed */
-	double rad[DMS_MAX_OPTICAL_DEPTH_PIECES + 1], spd[DMS_MAX_OPTICAL_DEPTH_PIECES + 1];
+	constexpr int NQ = DMS_MAX_OPTICAL_DEPTH_PIECES + 2; /* fractions, then the extra query */
+	double rf[NQ], vf[NQ];
+	size_t lo[NQ];
+	int mode[NQ]; /* 0 done, 1 grid, 2 computed */
+	double rad[NQ], spd[NQ], res[NQ];
 #pragma unroll
-	for(int i = 0; i < DMS_MAX_OPTICAL_DEPTH_PIECES + 1; i++)
+	for(int i = 0; i < NQ; i++)
 	{
 		mode[i] = 0;
-		if(i >= n)
-			continue;
-		if(i == 0 && use_cached)
+		res[i] = 0.0;
+		double r, v;
+		if(i == NQ - 1)
 		{
-			out[i] = cached_rate;
-			continue;
+			if(!extra)
+				continue;
+			r = xr;
+			v = xv;
 		}
-		const double f = dm_clamp_unit(fr[i]);
-		double rs = 0.0, vs = 0.0;
-		for(int c = 0; c < 3; c++)
+		else
 		{
-			const double pos = br[c] + f * (ar[c] - br[c]);
-			const double vel = bv[c] + f * (av[c] - bv[c]);
-			rs += pos * pos;
-			vs += vel * vel;
+			if(i >= n)
+				continue;
+			if(i == 0 && use_cached)
+			{
+				res[i] = cached_rate;
+				continue;
+			}
+			const double f = dm_clamp_unit(fr[i]);
+			double rs = 0.0, vs = 0.0;
+			for(int c = 0; c < 3; c++)
+			{
+				const double pos = br[c] + f * (ar[c] - br[c]);
+				const double vel = bv[c] + f * (av[c] - bv[c]);
+				rs += pos * pos;
+				vs += vel * vel;
+			}
+			r = dm_sqrt(dm_max(0.0, rs));
+			if(r >= DMS_rSun)
+			{
+				res[i] = 0.0;
+				continue;
+			}
+			v = dm_sqrt(dm_max(0.0, vs));
 		}
-		double r = dm_sqrt(dm_max(0.0, rs));
-		if(r >= DMS_rSun)
-		{
-			out[i] = 0.0;
-			continue;
-		}
-		double v = dm_sqrt(dm_max(0.0, vs));
 		/* Total_DM_Scattering_Rate */
 		if(!dm_isfinite(r) || !dm_isfinite(v))
 		{
-			out[i] = dm_nan();
+			res[i] = dm_nan();
 			continue;
 		}
 		if(r < 0.0)
@@ -638,7 +654,7 @@ DMS_HD void dms_rates_at_fractions(const DmsPointView& P, const DmsSolarView& S,
 			/* Total_DM_Scattering_Rate_Interpolated up to the table reads */
 			if(r > DMS_rSun || r < 0.0 || v < 0.0)
 			{
-				out[i] = 0.0;
+				res[i] = 0.0;
 				mode[i] = 0;
 				continue;
 			}
@@ -658,9 +674,9 @@ DMS_HD void dms_rates_at_fractions(const DmsPointView& P, const DmsSolarView& S,
 			mode[i] = 1;
 		}
 	}
-	double g[DMS_MAX_OPTICAL_DEPTH_PIECES + 1][4];
+	double g[NQ][4];
 #pragma unroll
-	for(int i = 0; i < DMS_MAX_OPTICAL_DEPTH_PIECES + 1; i++)
+	for(int i = 0; i < NQ; i++)
 		if(mode[i] == 1)
 		{
 			const double* q = P.grid + lo[i];
@@ -670,18 +686,24 @@ DMS_HD void dms_rates_at_fractions(const DmsPointView& P, const DmsSolarView& S,
 			g[i][3] = q[P.Nv + 1];
 		}
 #pragma unroll
-	for(int i = 0; i < DMS_MAX_OPTICAL_DEPTH_PIECES + 1; i++)
+	for(int i = 0; i < NQ; i++)
 	{
 		if(mode[i] == 1)
 		{
 			const double lower = g[i][0] + vf[i] * (g[i][1] - g[i][0]);
 			const double upper = g[i][2] + vf[i] * (g[i][3] - g[i][2]);
 			const double val = lower + rf[i] * (upper - lower);
-			out[i] = dm_isfinite(val) ? dm_max(0.0, val) : dm_nan();
+			res[i] = dm_isfinite(val) ? dm_max(0.0, val) : dm_nan();
 		}
 		else if(mode[i] == 2)
-			out[i] = dms_rate_computed(P, S, rad[i], spd[i]);
+			res[i] = dms_rate_computed(P, S, rad[i], spd[i]);
 	}
+#pragma unroll
+	for(int i = 0; i < NQ - 1; i++)
+		if(i < n)
+			out[i] = res[i];
+	if(extra)
+		*xout = res[NQ - 1];
 }
 
 /* Solve_Linear_Rate_Optical_Depth_Fraction, :416-464 */
@@ -918,7 +940,9 @@ DMS_HD int dms_step_prep(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 		L.r_before = dms_prop_radius(L.prop);
 		L.v_before = dms_prop_speed(L.prop);
 		L.eb_valid = 1;
+		L.rb_cached = 0;
 	}
+	const double cached_rate = L.rate_before;
 	L.rate_before = 0.0;
 	if(L.r_before >= DMS_rSun)
 	{
@@ -935,7 +959,7 @@ DMS_HD int dms_step_prep(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 	else
 	{
 		L.prop.ts = dms_rk45_sanitized(L.prop.ts);
-		L.rate_before = dms_rate_total(P, S, L.r_before, L.v_before);
+		L.rate_before = L.rb_cached ? cached_rate : dms_rate_total(P, S, L.r_before, L.v_before);
 		if(!dm_isfinite(L.rate_before) || L.rate_before < 0.0)
 		{
 			double t;
@@ -991,6 +1015,8 @@ DMS_HD int dms_step_post(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 	const double od_limit = fast ? DMS_CAPTURE_MODE_MAX_OPTICAL_DEPTH_STEP : DMS_MAX_OPTICAL_DEPTH_STEP;
 	const int od_count = fast ? DMS_CAPTURE_MODE_OPTICAL_DEPTH_PIECES : DMS_MAX_OPTICAL_DEPTH_PIECES;
 	int scattering = 0, reflection = 0;
+	bool want_next = false;
+	double rate_next = 0.0;
 	double acc_t = ea_t, acc_r[3] = {ea_r[0], ea_r[1], ea_r[2]}, acc_v[3] = {ea_v[0], ea_v[1], ea_v[2]};
 	double is = 0.0, ie = 0.0;
 	if(dms_interior_interval(L.eb_r, ea_r, L.r_before, r_after, is, ie))
@@ -1010,7 +1036,9 @@ DMS_HD int dms_step_post(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 			for(int i = 0; i <= DMS_MAX_OPTICAL_DEPTH_PIECES; i++)
 				fr[i] = (i <= od_count) ? is + (double) i / (double) od_count * span : 0.0;
 			/* Scattering_Rate_At_Fraction for all fractions, grid loads issued together */
-			dms_rates_at_fractions(P, S, L.eb_r, L.eb_v, ea_r, ea_v, fr, od_count + 1, use_cached, L.rate_before, rt);
+			want_next = r_after < DMS_rSun;
+			dms_rates_at_fractions(P, S, L.eb_r, L.eb_v, ea_r, ea_v, fr, od_count + 1, use_cached, L.rate_before, rt,
+			                       want_next, r_after, v_after, &rate_next);
 #pragma unroll
 			for(int i = 0; i <= DMS_MAX_OPTICAL_DEPTH_PIECES; i++)
 				if(i <= od_count && (!dm_isfinite(rt[i]) || rt[i] < 0.0))
@@ -1054,6 +1082,8 @@ DMS_HD int dms_step_post(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 			L.prop.phi = L.pb_phi;
 			L.prop.vr = L.pb_vr;
 			L.prop.ts = L.pb_ts;
+			/* the step is retried from the same state: rate_before stays valid */
+			L.rb_cached = 1;
 			double factor = DMS_RK45_MAX_STEP_FACTOR;
 			if(dtau > od_limit && dtau > 0.0)
 				factor = dm_min(factor, 0.8 * od_limit / dtau);
@@ -1185,6 +1215,9 @@ DMS_HD int dms_step_post(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 	L.r_before = r_after;
 	L.v_before = v_after;
 	L.eb_valid = 1;
+	/* Total_DM_Scattering_Rate at the new r_before, read with this step's gathers */
+	L.rb_cached = want_next ? 1 : 0;
+	L.rate_before = rate_next;
 	return -2;
 #undef DMS_RETURN_AT
 }

@@ -80,6 +80,35 @@ DMS_HD int dms_interval(const DmsSolarView& S, double x)
 
 DMS_HD double dms_cubic(const DmsCubic& c, double t) { return ((c.a * t + c.b) * t + c.c) * t + c.d; }
 
+/* Cubic piece j of table T evaluated at x, for x in [x_0, x_{n-1}] (finite).
+ * The interval guess from the uniform spacing is almost always exact, so the
+ * node pair and the coefficients of the guessed interval are read together
+ * (one memory round trip on the dependent chain); a wrong guess falls back to
+ * dms_interval.  The result is that of dms_cubic(T[j], x - x_j) with j from
+ * dms_interval in every case. */
+DMS_HD double dms_spline_at(const DmsSolarView& S, const DmsCubic* T, double x)
+{
+	const int n = S.n;
+	const double t = (x - S.x1) * S.inv_h;
+	int j;
+	if(!(t >= 0.0))
+		j = 0;
+	else if(t >= (double) n)
+		j = n - 2;
+	else
+		j = (int) t + 1;
+	if(j > n - 2)
+		j = n - 2;
+	const double xj = S.x[j];
+	const double xn = S.x[j + 1];
+	const DmsCubic c = T[j];
+	const bool ok = (j == n - 2 || !(xn <= x)) && (j == 0 || !(xj > x));
+	if(ok)
+		return dms_cubic(c, x - xj);
+	const int k = dms_interval(S, x);
+	return dms_cubic(T[k], x - S.x[k]);
+}
+
 /* Solar_Model::Mass, src/Solar_Model.cpp:168-179 */
 DMS_HD double dms_mass(const DmsSolarView& S, double r)
 {
@@ -89,8 +118,7 @@ DMS_HD double dms_mass(const DmsSolarView& S, double r)
 		r = 0.0;
 	if(r > DMS_rSun)
 		return DMS_mSun;
-	const int j = dms_interval(S, r);
-	return 1.0 * dms_cubic(S.mass[j], r - S.x[j]);
+	return 1.0 * dms_spline_at(S, S.mass, r);
 }
 
 /* Solar_Model::Temperature, src/Solar_Model.cpp:194-202 */
@@ -113,8 +141,7 @@ DMS_HD double dms_local_escape_speed(const DmsSolarView& S, double r)
 		r = 0.0;
 	if(r > DMS_rSun)
 		return dm_sqrt(2 * DMS_G_Newton * DMS_mSun / r);
-	const int j = dms_interval(S, r);
-	return dm_sqrt(1.0 * dms_cubic(S.vesc2[j], r - S.x[j]));
+	return dm_sqrt(1.0 * dms_spline_at(S, S.vesc2, r));
 }
 
 /* Number density of target k (k < n_iso: isotope, k == 63: electrons) at a
