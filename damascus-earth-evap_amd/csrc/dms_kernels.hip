@@ -118,7 +118,11 @@ __global__ void __launch_bounds__(DMS_TRAJ_BLOCK) dms_traj_kernel(DmsTrajArgs a)
 		}
 		dms_lane_finish_scatter(L, c);
 		if(L.phase == PH_FIN)
+		{
 			dms_lane_finish(L, c, a.ics[L.slot], a.records ? a.records + L.slot : nullptr, a.summ ? a.summ + L.slot : nullptr);
+			if(a.job_done)
+				atomicAdd(a.job_done + L.job, 1u);
+		}
 		/* lane refill: one atomic per wave */
 		const unsigned long long nmask = __ballot(L.phase == PH_NEED);
 		if(nmask)
@@ -148,8 +152,12 @@ __global__ void __launch_bounds__(DMS_TRAJ_BLOCK) dms_traj_kernel(DmsTrajArgs a)
 			}
 		}
 		if(L.phase == PH_START)
+		{
 			dms_lane_start(L, c, a.ics[L.slot], a.shift_ok == nullptr || a.shift_ok[L.slot] != 0,
 			               a.records ? a.records + L.slot : nullptr, a.summ ? a.summ + L.slot : nullptr);
+			if(L.phase == PH_NEED && a.job_done) /* initial shift failed: recorded, done */
+				atomicAdd(a.job_done + L.job, 1u);
+		}
 		dms_lane_advance_b(L, c);
 		if(__ballot(L.phase != PH_DONE) == 0)
 			break;

@@ -30,6 +30,9 @@ struct DmsTrajArgs
 	uint32_t n_jobs;
 	const uint32_t* job_start;
 	const uint64_t* job_first_id;
+	/* finished trajectories per job (progress, read by the host while the
+	 * launch runs); may be null */
+	unsigned int* job_done;
 };
 
 __global__ void dms_traj_kernel(DmsTrajArgs a);

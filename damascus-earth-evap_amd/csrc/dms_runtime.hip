@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -31,6 +32,7 @@ struct dms_ctx
 {
 	int device = -1;
 	hipStream_t stream = nullptr;
+	hipStream_t progress_stream = nullptr; /* progress reads while a launch runs */
 	std::mutex mu;
 	std::string err;
 	dms::SolarTables T;
@@ -281,6 +283,7 @@ struct LaunchJobs
 	std::vector<uint64_t> first_id;
 	uint32_t* d_start = nullptr;
 	uint64_t* d_first = nullptr;
+	unsigned int* d_done = nullptr;
 
 	hipError_t upload()
 	{
@@ -288,6 +291,10 @@ struct LaunchJobs
 		if((e = hipMalloc((void**) &d_start, sizeof(uint32_t) * start.size())) != hipSuccess)
 			return e;
 		if((e = hipMalloc((void**) &d_first, sizeof(uint64_t) * first_id.size())) != hipSuccess)
+			return e;
+		if((e = hipMalloc((void**) &d_done, sizeof(unsigned int) * start.size())) != hipSuccess)
+			return e;
+		if((e = hipMemset(d_done, 0, sizeof(unsigned int) * start.size())) != hipSuccess)
 			return e;
 		if((e = hipMemcpy(d_start, start.data(), sizeof(uint32_t) * start.size(), hipMemcpyHostToDevice)) != hipSuccess)
 			return e;
@@ -297,15 +304,21 @@ struct LaunchJobs
 	{
 		hipFree(d_start);
 		hipFree(d_first);
+		hipFree(d_done);
 		d_start = nullptr;
 		d_first = nullptr;
+		d_done = nullptr;
 	}
 };
 
 /* Launch the trajectory kernel over n slots.  rows: per-trajectory rows in B;
  * lane_rows: per-lane scratch mode (R prepared by the caller). */
+/* Called about every 20 ms while a launch runs with the finished-trajectory
+ * count of each job of the launch. */
+using ProgressFn = std::function<void(const std::vector<unsigned int>&)>;
+
 hipError_t run_trajectories(dms_ctx* ctx, const dms_point* pt, const DmsJobView& J, ChunkBuffers& B, size_t n, bool rows,
-                            bool recs, const LaunchJobs& LJ, const DmsRows* lane_rows, float* ms)
+                            bool recs, const LaunchJobs& LJ, const DmsRows* lane_rows, float* ms, const ProgressFn& progress = nullptr)
 {
 	DmsTrajArgs a;
 	a.S = ctx->dev;
@@ -331,6 +344,7 @@ hipError_t run_trajectories(dms_ctx* ctx, const dms_point* pt, const DmsJobView&
 	a.n_jobs = (uint32_t) LJ.start.size();
 	a.job_start = LJ.d_start;
 	a.job_first_id = LJ.d_first;
+	a.job_done = LJ.d_done;
 	hipError_t e;
 	if((e = hipMemsetAsync(B.next, 0, sizeof(unsigned int), ctx->stream)) != hipSuccess)
 		return e;
@@ -349,6 +363,19 @@ hipError_t run_trajectories(dms_ctx* ctx, const dms_point* pt, const DmsJobView&
 	hipLaunchKernelGGL(dms_traj_kernel, dim3(traj_grid(ctx, n)), dim3(DMS_TRAJ_BLOCK), ctx->traj_lds, ctx->stream, a);
 	e = hipGetLastError();
 	hipEventRecord(e1, ctx->stream);
+	if(e == hipSuccess && progress && LJ.d_done)
+	{
+		/* heartbeat: read the per-job finished counters on a second stream */
+		std::vector<unsigned int> done(LJ.start.size(), 0u);
+		while(hipEventQuery(e1) == hipErrorNotReady)
+		{
+			if(hipMemcpyAsync(done.data(), LJ.d_done, sizeof(unsigned int) * done.size(), hipMemcpyDeviceToHost,
+			                  ctx->progress_stream) == hipSuccess
+			   && hipStreamSynchronize(ctx->progress_stream) == hipSuccess)
+				progress(done);
+			std::this_thread::sleep_for(std::chrono::milliseconds(20));
+		}
+	}
 	if(e == hipSuccess)
 		e = hipEventSynchronize(e1);
 	float t = 0.0f;
@@ -362,14 +389,14 @@ hipError_t run_trajectories(dms_ctx* ctx, const dms_point* pt, const DmsJobView&
 
 /* single-job launch over slots [0, n) with ids first_id .. */
 hipError_t run_trajectories_single(dms_ctx* ctx, const dms_point* pt, const DmsJobView& J, ChunkBuffers& B, size_t n, bool rows,
-                                   bool recs, uint64_t first_id, float* ms)
+                                   bool recs, uint64_t first_id, float* ms, const ProgressFn& progress = nullptr)
 {
 	LaunchJobs LJ;
 	LJ.start.push_back(0);
 	LJ.first_id.push_back(first_id);
 	hipError_t e = LJ.upload();
 	if(e == hipSuccess)
-		e = run_trajectories(ctx, pt, J, B, n, rows, recs, LJ, nullptr, ms);
+		e = run_trajectories(ctx, pt, J, B, n, rows, recs, LJ, nullptr, ms, progress);
 	LJ.release();
 	return e;
 }
@@ -631,7 +658,9 @@ void run_job_chunked(dms_job* job)
 		hipLaunchKernelGGL(dms_ic_kernel, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, ctx->stream, H, d.seed, d.point_id,
 		                   next_id, (uint32_t) n, d.r_max_rsun * DMS_rSun, B.ics, B.ok);
 		float kms = 0.0f;
-		hipError_t e = run_trajectories_single(ctx, pt, J, B, n, want_rows, want_recs, next_id, &kms);
+		const uint64_t base_done = C.trajectories;
+		hipError_t e = run_trajectories_single(ctx, pt, J, B, n, want_rows, want_recs, next_id, &kms,
+		                                       [job, base_done](const std::vector<unsigned int>& d) { job->done_traj = base_done + d[0]; });
 		if(e != hipSuccess)
 		{
 			set_err(std::string("trajectory kernel: ") + hipGetErrorString(e));
@@ -779,7 +808,11 @@ void run_batch_fixed(std::vector<dms_job*>& jobs)
 			                   jobs[k]->desc.first_trajectory_id, (uint32_t) nk, d0.r_max_rsun * DMS_rSun, B.ics + LJ.start[k],
 			                   B.ok + LJ.start[k]);
 		}
-		hipError_t e = run_trajectories(ctx, pt, J, B, n, want_rows, want_recs, LJ, want_rows ? &LR : nullptr, &kms);
+		hipError_t e = run_trajectories(ctx, pt, J, B, n, want_rows, want_recs, LJ, want_rows ? &LR : nullptr, &kms,
+		                                [&jobs](const std::vector<unsigned int>& d) {
+			                                for(size_t k = 0; k < jobs.size() && k < d.size(); k++)
+				                                jobs[k]->done_traj = d[k];
+		                                });
 		if(e != hipSuccess)
 		{
 			set_err(std::string("trajectory kernel: ") + hipGetErrorString(e));
@@ -982,7 +1015,8 @@ int dms_create(dms_ctx** out, int device, const double* raw_rows, size_t n_rows)
 		};
 		if(hipSetDevice(device) != hipSuccess)
 			return cleanup("hipSetDevice failed");
-		if(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess)
+		if(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess
+		   || hipStreamCreateWithFlags(&ctx->progress_stream, hipStreamNonBlocking) != hipSuccess)
 			return cleanup("hipStreamCreate failed");
 		hipDeviceProp_t prop;
 		if(hipGetDeviceProperties(&prop, device) != hipSuccess)
@@ -1039,6 +1073,8 @@ void dms_destroy(dms_ctx* ctx)
 		hipFree(ctx->d_Z);
 		if(ctx->stream)
 			hipStreamDestroy(ctx->stream);
+		if(ctx->progress_stream)
+			hipStreamDestroy(ctx->progress_stream);
 	}
 	delete ctx;
 }
