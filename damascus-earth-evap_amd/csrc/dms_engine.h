@@ -470,6 +470,13 @@ DMS_HD bool dms_interior_interval(const double* br, const double* ar, double r_b
 {
 	const bool before_inside = r_before < DMS_rSun;
 	const bool after_inside = r_after < DMS_rSun;
+	if(before_inside && after_inside)
+	{
+		/* the chord's crossings are not used in this case */
+		start = 0.0;
+		end = 1.0;
+		return true;
+	}
 	double first = 0.0, second = 0.0;
 	bool has = false;
 	{
@@ -502,12 +509,6 @@ DMS_HD bool dms_interior_interval(const double* br, const double* ar, double r_b
 	const double tol = 1.0e-12;
 	const bool f_in = first >= -tol && first <= 1.0 + tol;
 	const bool s_in = second >= -tol && second <= 1.0 + tol;
-	if(before_inside && after_inside)
-	{
-		start = 0.0;
-		end = 1.0;
-		return true;
-	}
 	bool need_bisection = false;
 	if(!before_inside && after_inside)
 	{
@@ -592,34 +593,31 @@ DMS_HD void dms_rates_at_fractions(const DmsPointView& P, const DmsSolarView& S,
                                    const double* ar, const double* av, const double* fr, int n, bool use_cached,
                                    double cached_rate, double* out, bool extra, double xr, double xv, double* xout)
 {
+	/* Written without branches on the common path (selects only), so that the
+	 * slots' dependent chains interleave in one instruction stream; slots that
+	 * need the direct 64-target sum (no table, v > v_max, non-finite table
+	 * coordinates) take a separate branch afterwards. */
 	constexpr int NQ = DMS_MAX_OPTICAL_DEPTH_PIECES + 2; /* fractions, then the extra query */
-	double rf[NQ], vf[NQ];
-	size_t lo[NQ];
-	int mode[NQ]; /* 0 done, 1 grid, 2 computed */
-	double rad[NQ], spd[NQ], res[NQ];
+	double res[NQ], rad[NQ], spd[NQ], rf[NQ], vf[NQ];
+	unsigned lo[NQ];
+	bool grid[NQ], comp[NQ];
+	const bool use_grid = P.use_grid != 0;
 #pragma unroll
 	for(int i = 0; i < NQ; i++)
 	{
-		mode[i] = 0;
-		res[i] = 0.0;
+		const bool is_extra = (i == NQ - 1);
+		const bool active = is_extra ? extra : (i < n);
+		const bool cached = !is_extra && i == 0 && use_cached;
 		double r, v;
-		if(i == NQ - 1)
+		bool outside = false; /* Scattering_Rate_At_Fraction: radius >= R_sun gives 0 */
+		if(is_extra)
 		{
-			if(!extra)
-				continue;
 			r = xr;
 			v = xv;
 		}
 		else
 		{
-			if(i >= n)
-				continue;
-			if(i == 0 && use_cached)
-			{
-				res[i] = cached_rate;
-				continue;
-			}
-			const double f = dm_clamp_unit(fr[i]);
+			const double f = dm_clamp_unit(fr[i < n ? i : 0]);
 			double rs = 0.0, vs = 0.0;
 			for(int c = 0; c < 3; c++)
 			{
@@ -629,74 +627,67 @@ DMS_HD void dms_rates_at_fractions(const DmsPointView& P, const DmsSolarView& S,
 				vs += vel * vel;
 			}
 			r = dm_sqrt(dm_max(0.0, rs));
-			if(r >= DMS_rSun)
-			{
-				res[i] = 0.0;
-				continue;
-			}
 			v = dm_sqrt(dm_max(0.0, vs));
+			outside = r >= DMS_rSun;
 		}
 		/* Total_DM_Scattering_Rate */
-		if(!dm_isfinite(r) || !dm_isfinite(v))
-		{
-			res[i] = dm_nan();
-			continue;
-		}
-		if(r < 0.0)
-			r = 0.0;
-		if(v < 0.0)
-			v = 0.0;
+		const bool finite = dm_isfinite(r) && dm_isfinite(v);
+		r = (r < 0.0) ? 0.0 : r;
+		v = (v < 0.0) ? 0.0 : v;
 		rad[i] = r;
 		spd[i] = v;
-		mode[i] = 2;
-		if(P.use_grid && v <= P.vmax)
-		{
-			/* Total_DM_Scattering_Rate_Interpolated up to the table reads */
-			if(r > DMS_rSun || r < 0.0 || v < 0.0)
-			{
-				res[i] = 0.0;
-				mode[i] = 0;
-				continue;
-			}
-			const double rc = r * P.inv_dr;
-			const double vc = v * P.inv_dv;
-			if(!dm_isfinite(rc) || !dm_isfinite(vc) || rc < 0.0 || vc < 0.0)
-				continue; /* computed */
-			unsigned ri = (unsigned) rc;
-			if(ri > (unsigned) (P.Nr - 2))
-				ri = (unsigned) (P.Nr - 2);
-			unsigned vi = (unsigned) vc;
-			if(vi > (unsigned) (P.Nv - 2))
-				vi = (unsigned) (P.Nv - 2);
-			rf[i] = rc - ri;
-			vf[i] = vc - vi;
-			lo[i] = (size_t) ri * (size_t) P.Nv + vi;
-			mode[i] = 1;
-		}
+		const bool table = use_grid && v <= P.vmax;
+		/* Total_DM_Scattering_Rate_Interpolated */
+		const bool zero_interp = table && (r > DMS_rSun || r < 0.0 || v < 0.0);
+		const double rc = r * P.inv_dr;
+		const double vc = v * P.inv_dv;
+		const bool coords_ok = dm_isfinite(rc) && dm_isfinite(vc) && !(rc < 0.0) && !(vc < 0.0);
+		const bool live = active && !cached && !outside && finite;
+		grid[i] = live && table && !zero_interp && coords_ok;
+		comp[i] = live && !zero_interp && !(table && coords_ok);
+		const double rcs = coords_ok ? dm_min(rc, (double) (P.Nr - 2)) : 0.0;
+		const double vcs = coords_ok ? dm_min(vc, (double) (P.Nv - 2)) : 0.0;
+		const unsigned ri = (unsigned) rcs;
+		const unsigned vi = (unsigned) vcs;
+		rf[i] = rc - ri;
+		vf[i] = vc - vi;
+		lo[i] = grid[i] ? ri * (unsigned) P.Nv + vi : 0u;
+		res[i] = cached ? cached_rate : ((!finite && !outside) ? dm_nan() : 0.0);
 	}
-	double g[NQ][4];
+	if(use_grid)
+	{
+		/* all table reads in one batch (slots without a lookup read node 0) */
+		const unsigned nv = (unsigned) P.Nv;
+		double g[NQ][4];
 #pragma unroll
-	for(int i = 0; i < NQ; i++)
-		if(mode[i] == 1)
+		for(int i = 0; i < NQ; i++)
 		{
 			const double* q = P.grid + lo[i];
 			g[i][0] = q[0];
 			g[i][1] = q[1];
-			g[i][2] = q[P.Nv];
-			g[i][3] = q[P.Nv + 1];
+			g[i][2] = q[nv];
+			g[i][3] = q[nv + 1];
 		}
 #pragma unroll
-	for(int i = 0; i < NQ; i++)
-	{
-		if(mode[i] == 1)
+		for(int i = 0; i < NQ; i++)
 		{
 			const double lower = g[i][0] + vf[i] * (g[i][1] - g[i][0]);
 			const double upper = g[i][2] + vf[i] * (g[i][3] - g[i][2]);
 			const double val = lower + rf[i] * (upper - lower);
-			res[i] = dm_isfinite(val) ? dm_max(0.0, val) : dm_nan();
+			const double gv = dm_isfinite(val) ? dm_max(0.0, val) : dm_nan();
+			res[i] = grid[i] ? gv : res[i];
 		}
-		else if(mode[i] == 2)
-			res[i] = dms_rate_computed(P, S, rad[i], spd[i]);
+	}
+	bool any_comp = false;
+#pragma unroll
+	for(int i = 0; i < NQ; i++)
+		any_comp = any_comp || comp[i];
+	if(any_comp)
+	{
+#pragma unroll
+		for(int i = 0; i < NQ; i++)
+			if(comp[i])
+				res[i] = dms_rate_computed(P, S, rad[i], spd[i]);
 	}
 #pragma unroll
 	for(int i = 0; i < NQ - 1; i++)

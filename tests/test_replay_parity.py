@@ -58,6 +58,9 @@ def rec_bytes_ev(ev):
     (dmsun.MODE_PARAMETER_POINT, dict(mass_GeV=0.3, sigma_p_cm2=1e-36, interpolation_points=300), 120, 10**14),
     (dmsun.MODE_PARAMETER_POINT, dict(mass_GeV=0.1, sigma_p_cm2=0.0, sigma_e_cm2=1e-37,
                                       interaction=dmsun.DARK_PHOTON, interpolation_points=100), 12, 10**14),
+    # no rate table: every rate is the direct 64-target sum
+    (dmsun.MODE_PARAMETER_POINT, dict(mass_GeV=0.3, sigma_p_cm2=1e-36, interpolation_points=0), 20, 10**14),
+    (dmsun.MODE_CAPTURE, dict(mass_GeV=1.0, sigma_p_cm2=1e-35, interpolation_points=1), 40, 1000),
 ])
 def test_trajectories_and_rows(replay, oracle_solar, mode, desc_args, n, max_scat):
     desc = dmsun.make_point(**desc_args)
