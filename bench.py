@@ -59,11 +59,11 @@ def algorithmic_flops(c, capture_mode):
             + w["trajectory"] * c.trajectories + 110.0 * segs)
 
 
-def make_workload(dmsun, n_traj, first_id):
+def make_workload(dmsun, n_traj, first_id, precision=0):
     desc = dmsun.make_point(0.3, 1e-36, interpolation_points=1000)
     job = dmsun.make_job(mode=dmsun.MODE_PARAMETER_POINT, max_trajectories=n_traj, capture_target=0,
                          max_scatterings=10**14, n_bins=1000, first_trajectory_id=first_id, record=0,
-                         halo=dmsun.default_halo((0.0, 248.0, 0.0)))
+                         precision=precision, halo=dmsun.default_halo((0.0, 248.0, 0.0)))
     return desc, job
 
 
@@ -84,6 +84,39 @@ def cpu_baseline(sample, threads):
                        f"{threads} threads, {dt:.2f} s; scatterings/s {c.total_scatterings / dt:.4g}")
 
 
+def measure(dmsun, pt, precision, args):
+    """One untimed job alone (reported as single_job), then the K timed steps as
+    one batch (one launch)."""
+    name = "fp64" if precision == dmsun.FP64 else "mixed"
+    single = None
+    for w in range(args.warmup):
+        _, job = make_workload(dmsun, args.trajectories, (1000 + w) * args.trajectories, precision)
+        t = time.perf_counter()
+        res = pt.run_job(job, want_records=False)
+        dt = time.perf_counter() - t
+        single = dict(trajectories_per_s=res["counters"].trajectories / dt, seconds=dt, kernel_ms=res["kernel_ms"],
+                      trajectories=res["counters"].trajectories)
+        print(f"{name} warmup {w} (one job alone): {dt:.2f} s, kernel {res['kernel_ms']:.1f} ms", file=sys.stderr,
+              flush=True)
+    jobs = [make_workload(dmsun, args.trajectories, k * args.trajectories, precision)[1] for k in range(args.steps)]
+    t0 = time.perf_counter()
+    results = pt.run_batch(jobs, want_records=False)
+    wall = time.perf_counter() - t0
+    out = dict(traj=0, scat=0, cap=0, flops=0.0, wall=wall, single=single, kern_ms=results[0]["kernel_ms"],
+               launches=results[0]["launches"])
+    for k, res in enumerate(results):
+        c = res["counters"]
+        out["traj"] += c.trajectories
+        out["scat"] += c.total_scatterings
+        out["cap"] += c.captured
+        out["flops"] += algorithmic_flops(c, False)
+        print(f"{name} step {k}: {c.trajectories} trajectories, {c.total_scatterings} scatterings, {c.captured} captured",
+              file=sys.stderr, flush=True)
+    print(f"{name} timed: {args.steps} jobs in one launch, kernel {out['kern_ms']:.1f} ms, wall {wall:.2f} s",
+          file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,6 +125,7 @@ def main():
     ap.add_argument("--trajectories", type=int, default=1_000_000)
     ap.add_argument("--cpu-sample", type=int, default=100_000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -103,39 +137,15 @@ def main():
     ctx = dmsun.Context(device=0)
     desc, _ = make_workload(dmsun, args.trajectories, 0)
     pt = ctx.point(desc)
-    single = None
-    for w in range(args.warmup):
-        _, job = make_workload(dmsun, args.trajectories, (1000 + w) * args.trajectories)
-        t = time.perf_counter()
-        res = pt.run_job(job, want_records=False)
-        dt = time.perf_counter() - t
-        single = dict(trajectories_per_s=res["counters"].trajectories / dt, seconds=dt,
-                      kernel_ms=res["kernel_ms"], trajectories=res["counters"].trajectories)
-        print(f"warmup {w} (one job alone): {dt:.2f} s, kernel {res['kernel_ms']:.1f} ms", file=sys.stderr, flush=True)
-
-    jobs = [make_workload(dmsun, args.trajectories, k * args.trajectories)[1] for k in range(args.steps)]
-    t0 = time.perf_counter()
-    results = pt.run_batch(jobs, want_records=False)
-    wall = time.perf_counter() - t0
-    tot_traj = tot_scat = tot_cap = 0
-    flops = 0.0
-    for k, res in enumerate(results):
-        c = res["counters"]
-        tot_traj += c.trajectories
-        tot_scat += c.total_scatterings
-        tot_cap += c.captured
-        flops += algorithmic_flops(c, False)
-        print(f"step {k}: {c.trajectories} trajectories, {c.total_scatterings} scatterings, "
-              f"{c.captured} captured", file=sys.stderr, flush=True)
-    kern_ms = results[0]["kernel_ms"]
-    launches = results[0]["launches"]
-    print(f"timed: {args.steps} jobs in one launch, kernel {kern_ms:.1f} ms, wall {wall:.2f} s", file=sys.stderr,
-          flush=True)
+    fp64 = measure(dmsun, pt, dmsun.FP64, args)
+    mixed = measure(dmsun, pt, dmsun.MIXED, args) if not args.no_mixed else None
+    tot_traj, tot_scat, tot_cap = fp64["traj"], fp64["scat"], fp64["cap"]
+    wall, flops, kern_ms, launches, single = fp64["wall"], fp64["flops"], fp64["kern_ms"], fp64["launches"], fp64["single"]
 
     avg_launch_s = (kern_ms / launches) * 1e-3
     achieved = flops / launches / avg_launch_s / 1e12
     out = {
-        "metric": "trajectories/s (FP64 GPU kernel; scatterings/s alongside)",
+        "metric": "trajectories/s (FP64 GPU kernel; mixed-precision kernel and scatterings/s alongside)",
         "value": tot_traj / wall,
         "unit": "trajectories/s",
         "n_gpus": 1,
@@ -159,6 +169,21 @@ def main():
                      "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
                      "kernel": "dms_traj_kernel", "avg_launch_ms": avg_launch_s * 1e3},
     }
+    if mixed is not None:
+        m_launch_s = mixed["kern_ms"] / mixed["launches"] * 1e-3
+        out["mixed"] = {
+            "value": mixed["traj"] / mixed["wall"], "unit": "trajectories/s",
+            "scatterings_per_s": mixed["scat"] / mixed["wall"],
+            "capture_fraction": mixed["cap"] / max(1, mixed["traj"]),
+            "speedup_vs_fp64": (mixed["traj"] / mixed["wall"]) / out["value"],
+            "single_job": mixed["single"],
+            "single_job_speedup_vs_fp64": (mixed["single"]["trajectories_per_s"] / single["trajectories_per_s"]
+                                           if mixed["single"] and single else None),
+            "kernel": "dms_traj_kernel_mixed", "avg_launch_ms": m_launch_s * 1e3,
+            "achieved_tflops_fp64_equivalent": mixed["flops"] / mixed["launches"] / m_launch_s / 1e12,
+            "precision": "FP32 rate table, optical depth (Kahan), target sampling, kinematics; "
+                         "FP64 RKF45, energy test, time sums",
+        }
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(args.cpu_sample, threads)

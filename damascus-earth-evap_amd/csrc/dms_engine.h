@@ -59,6 +59,7 @@ struct DmsJobView
 	double bin_width_km;
 	uint32_t seed, point_id;
 	uint64_t first_id;
+	int mixed; /* DMS_MIXED: run dms_traj_kernel_mixed */
 };
 
 struct DmsProp
@@ -104,6 +105,8 @@ struct DmsLane
 	double r_before, v_before, rate_before;
 	int eb_valid; /* eb/r_before/v_before already hold Event_In_3D of the current propagator */
 	int rb_cached; /* rate_before already holds Total_DM_Scattering_Rate(r_before, v_before) */
+	/* mixed precision: rate_before in 1/s, remaining optical depth (Kahan pair) */
+	float rate_before_f, mlx_f, mlx_c;
 	int inner_iter, rk_ok;
 	double last_E; /* capture energy of the last committed event (NaN when not computed) */
 	/* scatter hand-off */
@@ -895,6 +898,8 @@ DMS_HD int dms_commit(DmsLane& L, const DmsRows& R, const DmsJobView& J, const D
 
 /* Propagate_Freely entry (:764-805).  Returns -1 to continue with PREP or the
  * termination reason the reference returns before the step loop. */
+#include "dms_engine_mixed.h"
+
 DMS_HD int dms_segment_start(DmsLane& L, const DmsSolarView& S, double dm_mass, const DmsJobView& J)
 {
 	if(dms_outward_escaping(S, L.ev_r, L.ev_v, J.max_distance))
@@ -905,6 +910,8 @@ DMS_HD int dms_segment_start(DmsLane& L, const DmsSolarView& S, double dm_mass, 
 	dms_prop_init(L.prop, 0.0, L.ev_r, L.ev_v);
 	L.eb_valid = 0;
 	L.mlx = -dm_log(dm_max(dm_uniform(&L.rng), DM_DBL_MIN));
+	L.mlx_f = (float) L.mlx;
+	L.mlx_c = 0.0f;
 	L.time_steps = 0;
 	L.od_retries = 0;
 	if(!J.terminate_on_capture && L.prev_time_sec >= 0.0)
@@ -914,6 +921,7 @@ DMS_HD int dms_segment_start(DmsLane& L, const DmsSolarView& S, double dm_mass, 
 
 /* Top of the step loop (:876-918).  Returns -1 when an RKF45 step follows,
  * otherwise the termination reason. */
+template <bool MX = false>
 DMS_HD int dms_step_prep(DmsLane& L, const DmsPointView& P, const DmsSolarView& S, const DmsJobView& J)
 {
 	if(!(L.time_steps < J.max_time_steps))
@@ -935,6 +943,11 @@ DMS_HD int dms_step_prep(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 	}
 	const double cached_rate = L.rate_before;
 	L.rate_before = 0.0;
+	if constexpr(MX)
+	{
+		if(L.r_before >= DMS_rSun)
+			L.rate_before_f = 0.0f;
+	}
 	if(L.r_before >= DMS_rSun)
 	{
 		const double cap = dms_free_propagation_cap(L.r_before, L.v_before, J.max_distance);
@@ -950,17 +963,35 @@ DMS_HD int dms_step_prep(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 	else
 	{
 		L.prop.ts = dms_rk45_sanitized(L.prop.ts);
-		L.rate_before = L.rb_cached ? cached_rate : dms_rate_total(P, S, L.r_before, L.v_before);
-		if(!dm_isfinite(L.rate_before) || L.rate_before < 0.0)
-		{
-			double t;
-			dms_prop_event(L.prop, t, L.ev_r, L.ev_v);
-			L.ev_t = t + L.time_origin;
-			return DMS_NUMERICAL_FAILURE;
-		}
 		const double od_limit = J.terminate_on_capture ? DMS_CAPTURE_MODE_MAX_OPTICAL_DEPTH_STEP : DMS_MAX_OPTICAL_DEPTH_STEP;
-		if(L.rate_before > 0.0)
-			L.prop.ts = dm_min(L.prop.ts, od_limit / L.rate_before);
+		if constexpr(MX)
+		{
+			const float rb = L.rb_cached ? L.rate_before_f
+			                             : dms_rate_total_mx(P, S, (float) (L.r_before * DMS_MX_INV_RSUN), (float) L.v_before);
+			L.rate_before_f = rb;
+			if(!mx_isfinite(rb) || rb < 0.0f)
+			{
+				double t;
+				dms_prop_event(L.prop, t, L.ev_r, L.ev_v);
+				L.ev_t = t + L.time_origin;
+				return DMS_NUMERICAL_FAILURE;
+			}
+			if(rb > 0.0f)
+				L.prop.ts = dm_min(L.prop.ts, (double) ((float) od_limit / rb) * DMS_sec);
+		}
+		else
+		{
+			L.rate_before = L.rb_cached ? cached_rate : dms_rate_total(P, S, L.r_before, L.v_before);
+			if(!dm_isfinite(L.rate_before) || L.rate_before < 0.0)
+			{
+				double t;
+				dms_prop_event(L.prop, t, L.ev_r, L.ev_v);
+				L.ev_t = t + L.time_origin;
+				return DMS_NUMERICAL_FAILURE;
+			}
+			if(L.rate_before > 0.0)
+				L.prop.ts = dm_min(L.prop.ts, od_limit / L.rate_before);
+		}
 	}
 	L.pb_time = L.prop.time;
 	L.pb_radius = L.prop.radius;
@@ -976,13 +1007,20 @@ DMS_HD int dms_step_prep(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 /* Loop body after the RKF45 step (:921-1115).  Returns -2 to run PREP again,
  * -3 when the segment ended in a scattering (Scatter follows), or the
  * Propagate_Freely return value for every other exit. */
+template <bool MX = false>
 DMS_HD int dms_step_post(DmsLane& L, const DmsPointView& P, const DmsSolarView& S, const DmsJobView& J, const DmsRows& R)
 {
 	const double actual_dt = L.prop.time - L.pb_time;
 	const double r_after = dms_prop_radius(L.prop);
-	const double v_after = dms_prop_speed(L.prop);
 	double ea_t, ea_r[3], ea_v[3];
-	dms_prop_event(L.prop, ea_t, ea_r, ea_v);
+	double v_after;
+	if constexpr(MX)
+		v_after = dms_prop_event_mx(L.prop, ea_t, ea_r, ea_v);
+	else
+	{
+		v_after = dms_prop_speed(L.prop);
+		dms_prop_event(L.prop, ea_t, ea_r, ea_v);
+	}
 #define DMS_RETURN_AT(tt, rr, vv, code) \
 	do                                  \
 	{                                   \
@@ -1008,6 +1046,7 @@ DMS_HD int dms_step_post(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 	int scattering = 0, reflection = 0;
 	bool want_next = false;
 	double rate_next = 0.0;
+	float rate_next_f = 0.0f;
 	double acc_t = ea_t, acc_r[3] = {ea_r[0], ea_r[1], ea_r[2]}, acc_v[3] = {ea_v[0], ea_v[1], ea_v[2]};
 	double is = 0.0, ie = 0.0;
 	if(dms_interior_interval(L.eb_r, ea_r, L.r_before, r_after, is, ie))
@@ -1015,115 +1054,223 @@ DMS_HD int dms_step_post(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 		L.n_int++;
 		if(r_after < DMS_rSun && v_after < 0.0)
 			return DMS_MAX_FREE_STEPS; /* unreachable 'break' of the reference, :964-968 */
-		/* Build_Optical_Depth_Pieces, :347-414 */
-		double fr[DMS_MAX_OPTICAL_DEPTH_PIECES + 1], rt[DMS_MAX_OPTICAL_DEPTH_PIECES + 1];
-		const bool use_cached = L.r_before < DMS_rSun && is <= 1.0e-12;
-		double tau2 = 0.0, dtau = 0.0;
-		int invalid = 0, pc = 0;
-		if(ie > is && actual_dt > 0.0)
+		if constexpr(MX)
 		{
-			const double span = ie - is;
-#pragma unroll
-			for(int i = 0; i <= DMS_MAX_OPTICAL_DEPTH_PIECES; i++)
-				fr[i] = (i <= od_count) ? is + (double) i / (double) od_count * span : 0.0;
-			/* Scattering_Rate_At_Fraction for all fractions, grid loads issued together */
-			want_next = r_after < DMS_rSun;
-			dms_rates_at_fractions(P, S, L.eb_r, L.eb_v, ea_r, ea_v, fr, od_count + 1, use_cached, L.rate_before, rt,
-			                       want_next, r_after, v_after, &rate_next);
-#pragma unroll
-			for(int i = 0; i <= DMS_MAX_OPTICAL_DEPTH_PIECES; i++)
-				if(i <= od_count && (!dm_isfinite(rt[i]) || rt[i] < 0.0))
-					invalid = 1;
-			if(!invalid)
+			/* FP32 optical depth: rates in 1/s, times in s, Kahan-compensated sums */
+			float frf[DMS_MAX_OPTICAL_DEPTH_PIECES + 1], rtf[DMS_MAX_OPTICAL_DEPTH_PIECES + 1];
+			const bool use_cached = L.r_before < DMS_rSun && is <= 1.0e-12;
+			float tau2 = 0.0f, dtau = 0.0f;
+			int invalid = 0;
+			const float dt_s = (float) (actual_dt * DMS_MX_INV_SEC);
+			if(ie > is && actual_dt > 0.0)
 			{
-#define DMS_TRAP(s, e) (0.5 * (rt[s] + rt[e]) * (actual_dt * (fr[e] - fr[s])))
-				if(od_count >= 4)
-					tau2 = DMS_TRAP(0, 2) + DMS_TRAP(2, 4);
-				else
+				const float isf = (float) is, span = (float) (ie - is);
+#pragma unroll
+				for(int i = 0; i <= DMS_MAX_OPTICAL_DEPTH_PIECES; i++)
+					frf[i] = (i <= od_count) ? isf + (float) i / (float) od_count * span : 0.0f;
+				const float b[6] = {(float) (L.eb_r[0] * DMS_MX_INV_RSUN), (float) (L.eb_r[1] * DMS_MX_INV_RSUN),
+				                    (float) (L.eb_r[2] * DMS_MX_INV_RSUN), (float) L.eb_v[0], (float) L.eb_v[1], (float) L.eb_v[2]};
+				const float a[6] = {(float) (ea_r[0] * DMS_MX_INV_RSUN), (float) (ea_r[1] * DMS_MX_INV_RSUN),
+				                    (float) (ea_r[2] * DMS_MX_INV_RSUN), (float) ea_v[0], (float) ea_v[1], (float) ea_v[2]};
+				want_next = r_after < DMS_rSun;
+				dms_rates_at_fractions_mx(P, S, b, a, frf, od_count + 1, use_cached, L.rate_before_f, rtf, want_next,
+				                          (float) (r_after * DMS_MX_INV_RSUN), (float) v_after, &rate_next_f);
+#pragma unroll
+				for(int i = 0; i <= DMS_MAX_OPTICAL_DEPTH_PIECES; i++)
+					if(i <= od_count && (!mx_isfinite(rtf[i]) || rtf[i] < 0.0f))
+						invalid = 1;
+				if(!invalid)
 				{
-#pragma unroll
-					for(int i = 0; i < DMS_MAX_OPTICAL_DEPTH_PIECES; i++)
-						if(i < od_count)
-							tau2 += DMS_TRAP(i, i + 1);
-				}
-#pragma unroll
-				for(int i = 0; i < DMS_MAX_OPTICAL_DEPTH_PIECES; i++)
-					if(i < od_count && fr[i + 1] > fr[i])
+					/* equal pieces of width w: composite trapezoid rules on 4 and 2 pieces */
+					const float w = dt_s * span / (float) od_count;
+					if(od_count >= 4)
 					{
-						pc++;
-						dtau += DMS_TRAP(i, i + 1);
+						const float ends = 0.5f * (rtf[0] + rtf[4]);
+						dtau = w * (ends + (rtf[1] + rtf[2] + rtf[3]));
+						tau2 = (2.0f * w) * (ends + rtf[2]);
 					}
-#undef DMS_TRAP
-			}
-		}
-		if(invalid || !dm_isfinite(dtau) || dtau < 0.0)
-			DMS_RETURN_AT(ea_t, ea_r, ea_v, DMS_NUMERICAL_FAILURE);
-		const double scale = dm_max(dtau, DMS_OPTICAL_DEPTH_ABSOLUTE_TOLERANCE);
-		const double relerr = dm_fabs(dtau - tau2) / scale;
-		const bool rej_od = dtau > od_limit;
-		const bool rej_acc = !fast && relerr > DMS_OPTICAL_DEPTH_RELATIVE_TOLERANCE;
-		if(rej_od || rej_acc)
-		{
-			L.n_rej++;
-			L.od_retries++;
-			if(L.od_retries > DMS_MAX_OPTICAL_DEPTH_RETRIES)
-				DMS_RETURN_AT(L.eb_t, L.eb_r, L.eb_v, DMS_NUMERICAL_FAILURE);
-			L.prop.time = L.pb_time;
-			L.prop.radius = L.pb_radius;
-			L.prop.phi = L.pb_phi;
-			L.prop.vr = L.pb_vr;
-			L.prop.ts = L.pb_ts;
-			/* the step is retried from the same state: rate_before stays valid */
-			L.rb_cached = 1;
-			double factor = DMS_RK45_MAX_STEP_FACTOR;
-			if(dtau > od_limit && dtau > 0.0)
-				factor = dm_min(factor, 0.8 * od_limit / dtau);
-			if(rej_acc && relerr > 0.0)
-				factor = dm_min(factor, 0.8 * dm_sqrt(DMS_OPTICAL_DEPTH_RELATIVE_TOLERANCE / relerr));
-			factor = dm_max(DMS_RK45_MIN_STEP_FACTOR, dm_min(factor, 1.0));
-			L.prop.ts = dms_rk45_sanitized(factor * actual_dt);
-			return -2;
-		}
-		if(dtau >= L.mlx)
-		{
-			/* locate the scattering piece; pieces with end > start only */
-			double remaining = L.mlx;
-			int hit = -1;
-#pragma unroll
-			for(int i = 0; i < DMS_MAX_OPTICAL_DEPTH_PIECES; i++)
-			{
-				if(hit < 0 && i < od_count && fr[i + 1] > fr[i])
-				{
-					const double piece_dt = actual_dt * (fr[i + 1] - fr[i]);
-					const double piece_tau = 0.5 * (rt[i] + rt[i + 1]) * piece_dt;
-					if(piece_tau >= remaining)
-						hit = i;
 					else
-						remaining -= piece_tau;
+					{
+						dtau = w * (0.5f * (rtf[0] + rtf[od_count]) + (od_count == 2 ? rtf[1] : 0.0f));
+						tau2 = dtau;
+					}
 				}
 			}
-			double fraction = ie;
-			if(hit >= 0)
+			if(invalid || !mx_isfinite(dtau) || dtau < 0.0f)
+				DMS_RETURN_AT(ea_t, ea_r, ea_v, DMS_NUMERICAL_FAILURE);
+			const float scale = fmaxf(dtau, (float) DMS_OPTICAL_DEPTH_ABSOLUTE_TOLERANCE);
+			const float relerr = fabsf(dtau - tau2) * mx_rcp(scale);
+			const bool rej_od = dtau > (float) od_limit;
+			const bool rej_acc = !fast && relerr > (float) DMS_OPTICAL_DEPTH_RELATIVE_TOLERANCE;
+			if(rej_od || rej_acc)
 			{
-				double ps = 0.0, pe = 0.0, rs0 = 0.0, re0 = 0.0;
+				L.n_rej++;
+				L.od_retries++;
+				if(L.od_retries > DMS_MAX_OPTICAL_DEPTH_RETRIES)
+					DMS_RETURN_AT(L.eb_t, L.eb_r, L.eb_v, DMS_NUMERICAL_FAILURE);
+				L.prop.time = L.pb_time;
+				L.prop.radius = L.pb_radius;
+				L.prop.phi = L.pb_phi;
+				L.prop.vr = L.pb_vr;
+				L.prop.ts = L.pb_ts;
+				L.rb_cached = 1;
+				double factor = DMS_RK45_MAX_STEP_FACTOR;
+				if(rej_od && dtau > 0.0f)
+					factor = dm_min(factor, 0.8 * od_limit / (double) dtau);
+				if(rej_acc && relerr > 0.0f)
+					factor = dm_min(factor, 0.8 * (double) mx_sqrt((float) DMS_OPTICAL_DEPTH_RELATIVE_TOLERANCE * mx_rcp(relerr)));
+				factor = dm_max(DMS_RK45_MIN_STEP_FACTOR, dm_min(factor, 1.0));
+				L.prop.ts = dms_rk45_sanitized(factor * actual_dt);
+				return -2;
+			}
+			const float remaining_total = L.mlx_f - L.mlx_c;
+			if(dtau >= remaining_total)
+			{
+				float remaining = remaining_total;
+				int hit = -1;
+				float ps = 0.0f, pe = 0.0f, rs0 = 0.0f, re0 = 0.0f;
 #pragma unroll
 				for(int i = 0; i < DMS_MAX_OPTICAL_DEPTH_PIECES; i++)
-					if(i == hit)
+				{
+					if(hit < 0 && i < od_count && frf[i + 1] > frf[i])
 					{
-						ps = fr[i];
-						pe = fr[i + 1];
-						rs0 = rt[i];
-						re0 = rt[i + 1];
+						const float piece_tau = 0.5f * (rtf[i] + rtf[i + 1]) * (dt_s * (frf[i + 1] - frf[i]));
+						if(piece_tau >= remaining)
+						{
+							hit = i;
+							ps = frf[i];
+							pe = frf[i + 1];
+							rs0 = rtf[i];
+							re0 = rtf[i + 1];
+						}
+						else
+							remaining -= piece_tau;
 					}
-				const double piece_dt = actual_dt * (pe - ps);
-				const double lf = dms_solve_linear_rate(rs0, re0, piece_dt, remaining);
-				fraction = ps + lf * (pe - ps);
+				}
+				double fraction = ie;
+				if(hit >= 0)
+				{
+					const float lf = dms_solve_linear_rate_mx(rs0, re0, dt_s * (pe - ps), remaining);
+					fraction = (double) ps + (double) lf * (double) (pe - ps);
+				}
+				dms_interpolate_event(L.eb_t, L.eb_r, L.eb_v, ea_t, ea_r, ea_v, fraction, acc_t, acc_r, acc_v);
+				scattering = 1;
 			}
-			dms_interpolate_event(L.eb_t, L.eb_r, L.eb_v, ea_t, ea_r, ea_v, fraction, acc_t, acc_r, acc_v);
-			scattering = 1;
+			else
+				dms_kahan_sub_mx(L.mlx_f, L.mlx_c, dtau);
 		}
 		else
-			L.mlx -= dtau;
+		{
+			/* Build_Optical_Depth_Pieces, :347-414 */
+			double fr[DMS_MAX_OPTICAL_DEPTH_PIECES + 1], rt[DMS_MAX_OPTICAL_DEPTH_PIECES + 1];
+			const bool use_cached = L.r_before < DMS_rSun && is <= 1.0e-12;
+			double tau2 = 0.0, dtau = 0.0;
+			int invalid = 0, pc = 0;
+			if(ie > is && actual_dt > 0.0)
+			{
+				const double span = ie - is;
+	#pragma unroll
+				for(int i = 0; i <= DMS_MAX_OPTICAL_DEPTH_PIECES; i++)
+					fr[i] = (i <= od_count) ? is + (double) i / (double) od_count * span : 0.0;
+				/* Scattering_Rate_At_Fraction for all fractions, grid loads issued together */
+				want_next = r_after < DMS_rSun;
+				dms_rates_at_fractions(P, S, L.eb_r, L.eb_v, ea_r, ea_v, fr, od_count + 1, use_cached, L.rate_before, rt,
+				                       want_next, r_after, v_after, &rate_next);
+	#pragma unroll
+				for(int i = 0; i <= DMS_MAX_OPTICAL_DEPTH_PIECES; i++)
+					if(i <= od_count && (!dm_isfinite(rt[i]) || rt[i] < 0.0))
+						invalid = 1;
+				if(!invalid)
+				{
+	#define DMS_TRAP(s, e) (0.5 * (rt[s] + rt[e]) * (actual_dt * (fr[e] - fr[s])))
+					if(od_count >= 4)
+						tau2 = DMS_TRAP(0, 2) + DMS_TRAP(2, 4);
+					else
+					{
+	#pragma unroll
+						for(int i = 0; i < DMS_MAX_OPTICAL_DEPTH_PIECES; i++)
+							if(i < od_count)
+								tau2 += DMS_TRAP(i, i + 1);
+					}
+	#pragma unroll
+					for(int i = 0; i < DMS_MAX_OPTICAL_DEPTH_PIECES; i++)
+						if(i < od_count && fr[i + 1] > fr[i])
+						{
+							pc++;
+							dtau += DMS_TRAP(i, i + 1);
+						}
+	#undef DMS_TRAP
+				}
+			}
+			if(invalid || !dm_isfinite(dtau) || dtau < 0.0)
+				DMS_RETURN_AT(ea_t, ea_r, ea_v, DMS_NUMERICAL_FAILURE);
+			const double scale = dm_max(dtau, DMS_OPTICAL_DEPTH_ABSOLUTE_TOLERANCE);
+			const double relerr = dm_fabs(dtau - tau2) / scale;
+			const bool rej_od = dtau > od_limit;
+			const bool rej_acc = !fast && relerr > DMS_OPTICAL_DEPTH_RELATIVE_TOLERANCE;
+			if(rej_od || rej_acc)
+			{
+				L.n_rej++;
+				L.od_retries++;
+				if(L.od_retries > DMS_MAX_OPTICAL_DEPTH_RETRIES)
+					DMS_RETURN_AT(L.eb_t, L.eb_r, L.eb_v, DMS_NUMERICAL_FAILURE);
+				L.prop.time = L.pb_time;
+				L.prop.radius = L.pb_radius;
+				L.prop.phi = L.pb_phi;
+				L.prop.vr = L.pb_vr;
+				L.prop.ts = L.pb_ts;
+				/* the step is retried from the same state: rate_before stays valid */
+				L.rb_cached = 1;
+				double factor = DMS_RK45_MAX_STEP_FACTOR;
+				if(dtau > od_limit && dtau > 0.0)
+					factor = dm_min(factor, 0.8 * od_limit / dtau);
+				if(rej_acc && relerr > 0.0)
+					factor = dm_min(factor, 0.8 * dm_sqrt(DMS_OPTICAL_DEPTH_RELATIVE_TOLERANCE / relerr));
+				factor = dm_max(DMS_RK45_MIN_STEP_FACTOR, dm_min(factor, 1.0));
+				L.prop.ts = dms_rk45_sanitized(factor * actual_dt);
+				return -2;
+			}
+			if(dtau >= L.mlx)
+			{
+				/* locate the scattering piece; pieces with end > start only */
+				double remaining = L.mlx;
+				int hit = -1;
+	#pragma unroll
+				for(int i = 0; i < DMS_MAX_OPTICAL_DEPTH_PIECES; i++)
+				{
+					if(hit < 0 && i < od_count && fr[i + 1] > fr[i])
+					{
+						const double piece_dt = actual_dt * (fr[i + 1] - fr[i]);
+						const double piece_tau = 0.5 * (rt[i] + rt[i + 1]) * piece_dt;
+						if(piece_tau >= remaining)
+							hit = i;
+						else
+							remaining -= piece_tau;
+					}
+				}
+				double fraction = ie;
+				if(hit >= 0)
+				{
+					double ps = 0.0, pe = 0.0, rs0 = 0.0, re0 = 0.0;
+	#pragma unroll
+					for(int i = 0; i < DMS_MAX_OPTICAL_DEPTH_PIECES; i++)
+						if(i == hit)
+						{
+							ps = fr[i];
+							pe = fr[i + 1];
+							rs0 = rt[i];
+							re0 = rt[i + 1];
+						}
+					const double piece_dt = actual_dt * (pe - ps);
+					const double lf = dms_solve_linear_rate(rs0, re0, piece_dt, remaining);
+					fraction = ps + lf * (pe - ps);
+				}
+				dms_interpolate_event(L.eb_t, L.eb_r, L.eb_v, ea_t, ea_r, ea_v, fraction, acc_t, acc_r, acc_v);
+				scattering = 1;
+			}
+			else
+				L.mlx -= dtau;
+		}
 	}
 	else
 		L.n_ext++;
@@ -1157,7 +1304,10 @@ DMS_HD int dms_step_post(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 			{
 				L.time_steps++;
 				L.od_retries = 0;
-				dms_commit(L, R, J, S, P.mass, bt, br, bv);
+				if constexpr(MX)
+					dms_commit_mx(L, R, J, S, P.mass, bt, br, bv);
+				else
+					dms_commit(L, R, J, S, P.mass, bt, br, bv);
 				L.ev_t = it + L.time_origin;
 				for(int c = 0; c < 3; c++)
 				{
@@ -1176,7 +1326,11 @@ DMS_HD int dms_step_post(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 	}
 	L.time_steps++;
 	L.od_retries = 0;
-	const int captured_now = dms_commit(L, R, J, S, P.mass, acc_t, acc_r, acc_v);
+	int captured_now = 0;
+	if constexpr(MX)
+		dms_commit_mx(L, R, J, S, P.mass, acc_t, acc_r, acc_v);
+	else
+		captured_now = dms_commit(L, R, J, S, P.mass, acc_t, acc_r, acc_v);
 	L.ev_t = acc_t + L.time_origin;
 	for(int c = 0; c < 3; c++)
 	{
@@ -1186,7 +1340,10 @@ DMS_HD int dms_step_post(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 	{
 		/* abort_if_uncaptured_bound on the committed event: the energy was just
 		 * computed for the same (|r|, |v|) by Update_Capture_State */
-		const double E = dm_isnan(L.last_E) ? dms_capture_energy_eV(S, P.mass, dms_norm(L.ev_r), dms_norm(L.ev_v)) : L.last_E;
+		double E = L.last_E;
+		if(dm_isnan(E))
+			E = MX ? dms_capture_energy_mx(S, P.mass, dms_norm(L.ev_r), dms_dot(L.ev_v, L.ev_v))
+			       : dms_capture_energy_eV(S, P.mass, dms_norm(L.ev_r), dms_norm(L.ev_v));
 		if(!L.captured && !(dm_isfinite(E) && E >= 0.0))
 			return DMS_NUMERICAL_FAILURE;
 	}
@@ -1209,6 +1366,7 @@ DMS_HD int dms_step_post(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 	/* Total_DM_Scattering_Rate at the new r_before, read with this step's gathers */
 	L.rb_cached = want_next ? 1 : 0;
 	L.rate_before = rate_next;
+	L.rate_before_f = rate_next_f;
 	return -2;
 #undef DMS_RETURN_AT
 }
@@ -1636,13 +1794,14 @@ struct DmsLaneCtx
 
 /* RK attempt, step post-processing and the scatter hand-off (r, v and the
  * Sample_Target uniform are drawn here, before the wave-level target choice). */
+template <bool MX = false>
 DMS_HD void dms_lane_advance_a(DmsLane& L, const DmsLaneCtx& c)
 {
 	if(L.phase == PH_RK)
 	{
 		L.inner_iter++;
 		L.rk_all++;
-		const int r = dms_rk45_attempt(L.prop, c.S, L.inner_iter);
+		const int r = MX ? dms_rk45_attempt_mx(L.prop, c.S, L.inner_iter) : dms_rk45_attempt(L.prop, c.S, L.inner_iter);
 		if(r >= 0)
 		{
 			L.rk_ok = r;
@@ -1651,7 +1810,7 @@ DMS_HD void dms_lane_advance_a(DmsLane& L, const DmsLaneCtx& c)
 	}
 	if(L.phase == PH_POST)
 	{
-		const int pr = dms_step_post(L, c.P, c.S, c.J, c.R);
+		const int pr = dms_step_post<MX>(L, c.P, c.S, c.J, c.R);
 		if(pr == -2)
 			L.phase = PH_PREP;
 		else
@@ -1666,11 +1825,12 @@ DMS_HD void dms_lane_advance_a(DmsLane& L, const DmsLaneCtx& c)
 }
 
 /* Rest of Scatter once sc_target holds the Sample_Target result. */
+template <bool MX = false>
 DMS_HD void dms_lane_finish_scatter(DmsLane& L, const DmsLaneCtx& c)
 {
 	if(L.phase != PH_SCAT)
 		return;
-	if(L.sc_target < -1 || !dms_scatter_rest(L, c.P, c.S, L.sc_target))
+	if(L.sc_target < -1 || !(MX ? dms_scatter_rest_mx(L, c.P, c.S, L.sc_target) : dms_scatter_rest(L, c.P, c.S, L.sc_target)))
 	{
 		L.reason = DMS_NUMERICAL_FAILURE;
 		L.phase = PH_FIN;
@@ -1709,6 +1869,7 @@ DMS_HD void dms_lane_start(DmsLane& L, const DmsLaneCtx& c, const dms_event& ic,
 }
 
 /* Segment start and step preparation. */
+template <bool MX = false>
 DMS_HD void dms_lane_advance_b(DmsLane& L, const DmsLaneCtx& c)
 {
 	if(L.phase == PH_SEG)
@@ -1726,7 +1887,7 @@ DMS_HD void dms_lane_advance_b(DmsLane& L, const DmsLaneCtx& c)
 	}
 	if(L.phase == PH_PREP)
 	{
-		const int pr = dms_step_prep(L, c.P, c.S, c.J);
+		const int pr = dms_step_prep<MX>(L, c.P, c.S, c.J);
 		L.phase = (pr < 0) ? (int) PH_RK : dms_after_propagation(L, pr);
 	}
 }

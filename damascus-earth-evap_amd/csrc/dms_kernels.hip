@@ -1,7 +1,8 @@
 /*
  * gfx950 kernels of the solar dark-matter trajectory path (FP64).
  *
- *   dms_traj_kernel   persistent, one lane per trajectory, lane refill from an
+ *   dms_traj_kernel   persistent, one lane per trajectory (FP64; the mixed-precision
+ *                     instantiation is dms_traj_kernel_mixed), lane refill from an
  *                     atomic work counter; Sample_Target's 63 nuclear + 1
  *                     electron rates are evaluated by the 64 lanes of the wave
  *                     together (one target per lane) for each scattering lane.
@@ -31,7 +32,8 @@ __device__ inline double shfl_d(double x, int src) { return __shfl(x, src); }
 
 } // namespace
 
-__global__ void __launch_bounds__(DMS_TRAJ_BLOCK) dms_traj_kernel(DmsTrajArgs a)
+template <bool MX>
+__device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 {
 	const int lane = threadIdx.x & 63;
 	/* Stage the profile nodes and the Mass and escape-speed cubics in LDS: the
@@ -71,7 +73,7 @@ __global__ void __launch_bounds__(DMS_TRAJ_BLOCK) dms_traj_kernel(DmsTrajArgs a)
 	const int n_iso = a.S.n_iso;
 	for(;;)
 	{
-		dms_lane_advance_a(L, c);
+		dms_lane_advance_a<MX>(L, c);
 		/* Sample_Target (src/Simulation_Trajectory.cpp:1122-1158): for each
 		 * scattering lane s the 64 lanes evaluate one target rate each. */
 		unsigned long long m = __ballot(L.phase == PH_SCAT);
@@ -83,11 +85,46 @@ __global__ void __launch_bounds__(DMS_TRAJ_BLOCK) dms_traj_kernel(DmsTrajArgs a)
 			const double v = shfl_d(L.sc_v, s);
 			const double xi = shfl_d(L.sc_xi, s);
 			const int k = (lane < n_iso) ? lane : ((lane == DMS_MAX_TARGETS - 1) ? lane : -1);
+			int target = -2;
+			if constexpr(MX)
+			{
+				/* FP32 rates (1/s), same draw and order as below */
+				float rate = 0.0f;
+				if(k >= 0)
+					rate = dms_rate_target_mx(a.P, a.S, r, v, k);
+				const bool bad = (k >= 0) && (!mx_isfinite(rate) || rate < 0.0f);
+				if(__ballot(bad) == 0)
+				{
+					float total = __shfl(rate, DMS_MAX_TARGETS - 1);
+					for(int i = 0; i < n_iso; i++)
+						total += __shfl(rate, i);
+					if(mx_isfinite(total) && total > 0.0f)
+					{
+						const float q = rate * mx_rcp(total);
+						const float xf = (float) xi;
+						float sum = __shfl(q, DMS_MAX_TARGETS - 1);
+						if(sum > xf)
+							target = -1;
+						else
+							for(int i = 0; i < n_iso; i++)
+							{
+								sum += __shfl(q, i);
+								if(sum > xf || i == n_iso - 1)
+								{
+									target = i;
+									break;
+								}
+							}
+					}
+				}
+				if(lane == s)
+					L.sc_target = target;
+				continue;
+			}
 			double rate = 0.0;
 			if(k >= 0)
 				rate = dms_rate_target(a.P, a.S, r, v, k);
 			const bool bad = (k >= 0) && (!dm_isfinite(rate) || rate < 0.0);
-			int target = -2;
 			if(__ballot(bad) == 0)
 			{
 				/* same summation and comparison order as dms_choose_target */
@@ -116,7 +153,7 @@ __global__ void __launch_bounds__(DMS_TRAJ_BLOCK) dms_traj_kernel(DmsTrajArgs a)
 			if(lane == s)
 				L.sc_target = target;
 		}
-		dms_lane_finish_scatter(L, c);
+		dms_lane_finish_scatter<MX>(L, c);
 		if(L.phase == PH_FIN)
 		{
 			dms_lane_finish(L, c, a.ics[L.slot], a.records ? a.records + L.slot : nullptr, a.summ ? a.summ + L.slot : nullptr);
@@ -158,10 +195,22 @@ __global__ void __launch_bounds__(DMS_TRAJ_BLOCK) dms_traj_kernel(DmsTrajArgs a)
 			if(L.phase == PH_NEED && a.job_done) /* initial shift failed: recorded, done */
 				atomicAdd(a.job_done + L.job, 1u);
 		}
-		dms_lane_advance_b(L, c);
+		dms_lane_advance_b<MX>(L, c);
 		if(__ballot(L.phase != PH_DONE) == 0)
 			break;
 	}
+}
+
+__global__ void __launch_bounds__(DMS_TRAJ_BLOCK) dms_traj_kernel(DmsTrajArgs a) { dms_traj_body<false>(a); }
+
+__global__ void __launch_bounds__(DMS_TRAJ_BLOCK) dms_traj_kernel_mixed(DmsTrajArgs a) { dms_traj_body<true>(a); }
+
+/* FP32 copy of the rate table in 1/s for the mixed-precision kernel */
+__global__ void dms_grid_f32_kernel(const double* grid, size_t n, float* out)
+{
+	const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
+	if(i < n)
+		out[i] = (float) (grid[i] * DMS_sec);
 }
 
 __global__ void dms_ic_kernel(DmsHaloView H, uint32_t seed, uint32_t point_id, uint64_t first_id, uint32_t n, double R_final,

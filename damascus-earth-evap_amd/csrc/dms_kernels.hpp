@@ -36,6 +36,8 @@ struct DmsTrajArgs
 };
 
 __global__ void dms_traj_kernel(DmsTrajArgs a);
+__global__ void dms_traj_kernel_mixed(DmsTrajArgs a);
+__global__ void dms_grid_f32_kernel(const double* grid, size_t n, float* out);
 __global__ void dms_ic_kernel(DmsHaloView H, uint32_t seed, uint32_t point_id, uint64_t first_id, uint32_t n, double R_final,
                               dms_event* out, int32_t* ok);
 __global__ void dms_grid_kernel(DmsPointView P, DmsSolarView S, int Nr, int Nv, double vmax, double* grid);

@@ -53,6 +53,7 @@ struct DmsPointView
 	int use_grid, Nr, Nv;
 	double inv_dr, inv_dv, vmax;
 	const double* grid;
+	const float* grid32; /* the same table in FP32 and 1/s (mixed-precision kernel) */
 };
 
 /* Interval j with x_j <= x < x_{j+1}, clamped to [0, n-2]: the same index a

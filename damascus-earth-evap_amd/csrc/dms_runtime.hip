@@ -54,6 +54,7 @@ struct dms_point
 	dms::PointParams P;
 	double* d_sigmaN = nullptr;
 	double* d_grid = nullptr;
+	float* d_grid32 = nullptr;
 	int N = 0;
 	DmsPointView dev;
 };
@@ -137,6 +138,7 @@ DmsPointView host_point_view(const dms_point* pt)
 	v.inv_dv = pt->N >= 2 ? (double) (pt->N - 1) / DMS_V_MAX : 0.0;
 	v.vmax = pt->N >= 2 ? DMS_V_MAX : 0.0;
 	v.grid = nullptr;
+	v.grid32 = nullptr;
 	return v;
 }
 
@@ -153,6 +155,7 @@ DmsJobView job_view(const dms_job_desc& d)
 	J.seed = d.seed;
 	J.point_id = d.point_id;
 	J.first_id = d.first_trajectory_id;
+	J.mixed = (d.precision == DMS_MIXED) ? 1 : 0;
 	return J;
 }
 
@@ -360,7 +363,10 @@ hipError_t run_trajectories(dms_ctx* ctx, const dms_point* pt, const DmsJobView&
 	hipEventCreate(&e1);
 	(void) hipGetLastError();
 	hipEventRecord(e0, ctx->stream);
-	hipLaunchKernelGGL(dms_traj_kernel, dim3(traj_grid(ctx, n)), dim3(DMS_TRAJ_BLOCK), ctx->traj_lds, ctx->stream, a);
+	if(J.mixed)
+		hipLaunchKernelGGL(dms_traj_kernel_mixed, dim3(traj_grid(ctx, n)), dim3(DMS_TRAJ_BLOCK), ctx->traj_lds, ctx->stream, a);
+	else
+		hipLaunchKernelGGL(dms_traj_kernel, dim3(traj_grid(ctx, n)), dim3(DMS_TRAJ_BLOCK), ctx->traj_lds, ctx->stream, a);
 	e = hipGetLastError();
 	hipEventRecord(e1, ctx->stream);
 	if(e == hipSuccess && progress && LJ.d_done)
@@ -1026,7 +1032,8 @@ int dms_create(dms_ctx** out, int device, const double* raw_rows, size_t n_rows)
 		int max_lds = 0;
 		if(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess || (size_t) max_lds < lds + 16384)
 			return cleanup("solar profile too large for the LDS-resident Mass/escape-speed tables");
-		if(hipFuncSetAttribute((const void*) dms_traj_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds) != hipSuccess)
+		if(hipFuncSetAttribute((const void*) dms_traj_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds) != hipSuccess
+		   || hipFuncSetAttribute((const void*) dms_traj_kernel_mixed, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds) != hipSuccess)
 			return cleanup("hipFuncSetAttribute(MaxDynamicSharedMemorySize) failed");
 		int blocks = 0;
 		if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, dms_traj_kernel, DMS_TRAJ_BLOCK, lds) != hipSuccess)
@@ -1170,6 +1177,7 @@ int dms_point_create(dms_ctx* ctx, const dms_point_desc* desc, dms_point** out)
 	auto bail = [&](const std::string& what) {
 		hipFree(pt->d_sigmaN);
 		hipFree(pt->d_grid);
+		hipFree(pt->d_grid32);
 		delete pt;
 		return fail(ctx, DMS_ERR_DEVICE, "dms_point_create: " + what);
 	};
@@ -1196,6 +1204,15 @@ int dms_point_create(dms_ctx* ctx, const dms_point_desc* desc, dms_point** out)
 				return bail("rate-grid kernel failed");
 		}
 		pt->dev.grid = pt->d_grid;
+		/* FP32 copy (1/s) for the mixed-precision kernel */
+		if(hipMalloc((void**) &pt->d_grid32, sizeof(float) * cells) != hipSuccess)
+			return bail("FP32 grid allocation failed");
+		(void) hipGetLastError();
+		hipLaunchKernelGGL(dms_grid_f32_kernel, dim3((unsigned) ((cells + 255) / 256)), dim3(256), 0, ctx->stream, pt->d_grid, cells,
+		                   pt->d_grid32);
+		if(hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess)
+			return bail("FP32 grid conversion failed");
+		pt->dev.grid32 = pt->d_grid32;
 	}
 	*out = pt;
 	return DMS_OK;
@@ -1207,6 +1224,7 @@ void dms_point_destroy(dms_point* pt)
 		return;
 	hipFree(pt->d_sigmaN);
 	hipFree(pt->d_grid);
+	hipFree(pt->d_grid32);
 	delete pt;
 }
 
@@ -1313,8 +1331,6 @@ int dms_simulate_events(dms_point* pt, const dms_job_desc* job, const dms_event*
 	const std::string jm = check_job(*job);
 	if(!jm.empty())
 		return fail(ctx, DMS_ERR_ARG, "dms_simulate_events: " + jm);
-	if(job->precision == DMS_MIXED)
-		return fail(ctx, DMS_ERR_UNSUPPORTED, "dms_simulate_events: the mixed-precision kernel is not available in this build");
 	if(n == 0)
 		return DMS_OK;
 	if(n > 0x7fffffffu)
@@ -1364,8 +1380,6 @@ std::string check_submit(const dms_job_desc& d)
 		m = check_halo(d.halo);
 	if(m.empty() && d.max_trajectories == 0 && d.capture_target == 0)
 		m = "a job needs max_trajectories or capture_target";
-	if(m.empty() && d.precision == DMS_MIXED)
-		m = "the mixed-precision kernel is not available in this build";
 	return m;
 }
 

@@ -26,6 +26,7 @@ struct rp_point
 	rp_ctx* ctx;
 	dms::PointParams P;
 	std::vector<double> grid;
+	std::vector<float> grid32;
 	int N;
 	DmsPointView view;
 };
@@ -95,6 +96,7 @@ rp_point* rp_point_create(rp_ctx* c, const dms_point_desc* d)
 	v.inv_dv = p->N >= 2 ? (double) (p->N - 1) / DMS_V_MAX : 0.0;
 	v.vmax = p->N >= 2 ? DMS_V_MAX : 0.0;
 	v.grid = nullptr;
+	v.grid32 = nullptr;
 	if(p->N >= 2)
 	{
 		if(d->host_rate_grid)
@@ -114,6 +116,11 @@ rp_point* rp_point_create(rp_ctx* c, const dms_point_desc* d)
 				}
 		}
 		v.grid = p->grid.data();
+		/* FP32 table in 1/s, as dms_grid_f32_kernel makes it */
+		p->grid32.resize(p->grid.size());
+		for(size_t k = 0; k < p->grid.size(); k++)
+			p->grid32[k] = (float) (p->grid[k] * DMS_sec);
+		v.grid32 = p->grid32.data();
 	}
 	return p;
 }
@@ -189,6 +196,66 @@ static void replay_lane(DmsLane& L, const DmsLaneCtx& c, const dms_event& ic, bo
 	}
 }
 
+/* The mixed-precision phase drivers, serially (FP32 Sample_Target in the
+ * kernel's summation order). */
+static void replay_lane_mx(DmsLane& L, const DmsLaneCtx& c, const dms_event& ic, bool shift_ok, dms_traj_record* rec, DmsSummary* sum)
+{
+	float rates[DMS_MAX_TARGETS];
+	dms_lane_start(L, c, ic, shift_ok, rec, sum);
+	if(L.phase == PH_NEED)
+		return;
+	dms_lane_advance_b<true>(L, c);
+	while(L.phase != PH_NEED)
+	{
+		dms_lane_advance_a<true>(L, c);
+		if(L.phase == PH_SCAT)
+		{
+			int target = -2;
+			bool bad = false;
+			for(int k = 0; k < DMS_MAX_TARGETS; k++)
+			{
+				rates[k] = 0.0f;
+				if(k < c.S.n_iso || k == DMS_MAX_TARGETS - 1)
+				{
+					rates[k] = dms_rate_target_mx(c.P, c.S, L.sc_r, L.sc_v, k);
+					bad = bad || !mx_isfinite(rates[k]) || rates[k] < 0.0f;
+				}
+			}
+			if(!bad)
+			{
+				float total = rates[DMS_MAX_TARGETS - 1];
+				for(int i = 0; i < c.S.n_iso; i++)
+					total += rates[i];
+				if(mx_isfinite(total) && total > 0.0f)
+				{
+					const float xf = (float) L.sc_xi;
+					float sum = rates[DMS_MAX_TARGETS - 1] * mx_rcp(total);
+					if(sum > xf)
+						target = -1;
+					else
+						for(int i = 0; i < c.S.n_iso; i++)
+						{
+							sum += rates[i] * mx_rcp(total);
+							if(sum > xf || i == c.S.n_iso - 1)
+							{
+								target = i;
+								break;
+							}
+						}
+				}
+			}
+			L.sc_target = target;
+		}
+		dms_lane_finish_scatter<true>(L, c);
+		if(L.phase == PH_FIN)
+		{
+			dms_lane_finish(L, c, ic, rec, sum);
+			break;
+		}
+		dms_lane_advance_b<true>(L, c);
+	}
+}
+
 static void fill_job_view(DmsJobView& J, const dms_job_desc* job)
 {
 	J.max_time_steps = job->max_free_steps;
@@ -201,6 +268,7 @@ static void fill_job_view(DmsJobView& J, const dms_job_desc* job)
 	J.seed = job->seed;
 	J.point_id = job->point_id;
 	J.first_id = job->first_trajectory_id;
+	J.mixed = (job->precision == DMS_MIXED) ? 1 : 0;
 }
 
 /* Serial replay of the per-lane histogram mode: nlanes scratch rows reused
@@ -273,7 +341,10 @@ int rp_simulate(rp_point* p, const dms_job_desc* job, const dms_event* ics, size
 		L.row = (uint32_t) i;
 		L.job = 0;
 		L.id = J.first_id + i;
-		replay_lane(L, c, ics[i], true, &recs[i], nullptr);
+		if(c.J.mixed)
+			replay_lane_mx(L, c, ics[i], true, &recs[i], nullptr);
+		else
+			replay_lane(L, c, ics[i], true, &recs[i], nullptr);
 	}
 	return 0;
 }

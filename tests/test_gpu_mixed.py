@@ -1,0 +1,93 @@
+"""Mixed-precision kernel (FP32 rates, optical depth, target sampling and
+kinematics; FP64 orbit, energy test and time sums) against the FP64 kernel on
+the same trajectory ids.  The FP64 kernel is bit-identical to the oracle
+(test_gpu_parity.py), so these are the per-output error bounds against the
+all-FP64 restatement:
+
+  capture fraction        |p_mixed - p_fp64| <= 3 sigma (two-proportion)
+  radial histograms       chi^2/dof < 1.2 over bins with content (per-bin
+                          errors from the per-trajectory sums of squares)
+  evaporation lifetimes   two-sample KS p > 1e-3
+  mean scatterings        within 4 standard errors
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy import stats
+
+import dmsun
+
+pytestmark = pytest.mark.gpu
+
+
+def bin_error(s, s2, n):
+    if n <= 1:
+        return np.zeros_like(s)
+    m = s / n
+    return np.sqrt(n * np.maximum(0.0, s2 / n - m * m))
+
+
+def run(pt, mode, n, precision, nbins=500, target=0):
+    job = dmsun.make_job(mode=mode, max_trajectories=n, capture_target=target, max_scatterings=10**14,
+                         n_bins=nbins, record=0, precision=precision, halo=dmsun.default_halo((0.0, 248.0, 0.0)))
+    return pt.run_job(job, want_records=False)
+
+
+@pytest.mark.parametrize("mass,sigma,n", [(0.5, 1e-37, 100_000), (0.3, 1e-36, 100_000)])
+def test_parameter_point_statistics(gpu_ctx, mass, sigma, n):
+    pt = gpu_ctx.point(dmsun.make_point(mass, sigma, interpolation_points=1000))
+    print(f"mixed vs fp64: m={mass} sigma={sigma} n={n}", flush=True)
+    a = run(pt, dmsun.MODE_PARAMETER_POINT, n, dmsun.FP64)
+    b = run(pt, dmsun.MODE_PARAMETER_POINT, n, dmsun.MIXED)
+    ca, cb = a["counters"], b["counters"]
+    assert ca.trajectories == cb.trajectories == n
+    pa, pb = ca.captured / n, cb.captured / n
+    sig = math.sqrt(pa * (1 - pa) / n + pb * (1 - pb) / n)
+    assert abs(pa - pb) <= 3 * max(sig, 1.0 / n), (pa, pb)
+    # histograms: captured (rows 0-3) and not captured (rows 4-7)
+    chi2, dof = 0.0, 0
+    for base, na, nb in [(0, ca.captured, cb.captured), (4, ca.completed_outward_escapes, cb.completed_outward_escapes)]:
+        for q in (0, 1):
+            ha, hb = a["hist"][base + q], b["hist"][base + q]
+            ea = bin_error(ha, a["hist"][base + 2 + q], na)
+            eb = bin_error(hb, b["hist"][base + 2 + q], nb)
+            var = ea ** 2 + eb ** 2
+            use = var > 0
+            chi2 += float(np.sum((ha[use] - hb[use]) ** 2 / var[use]))
+            dof += int(use.sum())
+    print(f"capture {pa:.5f} vs {pb:.5f}; chi2/dof {chi2 / max(dof, 1):.3f} over {dof} bins", flush=True)
+    assert dof > 50 and chi2 / dof < 1.2
+    la = np.array([e.lifetime_unbinding_sec for e in a["events"]])
+    lb = np.array([e.lifetime_unbinding_sec for e in b["events"]])
+    if min(len(la), len(lb)) >= 20:
+        assert stats.ks_2samp(la, lb).pvalue > 1e-3
+    ma, mb = ca.total_scatterings / n, cb.total_scatterings / n
+    assert abs(ma - mb) <= 0.05 * ma + 1e-3
+
+
+def test_capture_mode_fraction(gpu_ctx):
+    pt = gpu_ctx.point(dmsun.make_point(1.0, 1e-35, interpolation_points=1000))
+    a = run(pt, dmsun.MODE_CAPTURE, 40_000, dmsun.FP64)
+    b = run(pt, dmsun.MODE_CAPTURE, 40_000, dmsun.MIXED)
+    na, nb = a["counters"].trajectories, b["counters"].trajectories
+    pa, pb = a["counters"].captured / na, b["counters"].captured / nb
+    sig = math.sqrt(pa * (1 - pa) / na + pb * (1 - pb) / nb)
+    print(f"capture mode: {pa:.5f} vs {pb:.5f} (sigma {sig:.5f})", flush=True)
+    assert abs(pa - pb) <= 3 * sig
+
+
+def test_mixed_batch_equals_alone(gpu_ctx):
+    """Mixed jobs give the same results in a batch as alone (the kernel is
+    deterministic per trajectory; histogram sums to rounding)."""
+    pt = gpu_ctx.point(dmsun.make_point(0.3, 1e-36, interpolation_points=1000))
+    jobs = [dmsun.make_job(mode=dmsun.MODE_PARAMETER_POINT, max_trajectories=n, capture_target=0, max_scatterings=10**14,
+                           n_bins=300, record=0, precision=dmsun.MIXED, first_trajectory_id=f)
+            for n, f in [(3000, 0), (2000, 10**7)]]
+    batch = pt.run_batch(jobs)
+    for job, res in zip(jobs, batch):
+        alone = pt.run_job(job, want_records=False)
+        for f in ("trajectories", "captured", "total_scatterings", "completed_outward_escapes", "n_rk_attempts"):
+            assert getattr(alone["counters"], f) == getattr(res["counters"], f), f
+        assert np.array_equal(alone["counts"], res["counts"])
+        assert np.allclose(alone["hist"], res["hist"], rtol=1e-12, atol=0)
