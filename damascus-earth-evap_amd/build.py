@@ -20,6 +20,7 @@ LIB = HERE / "libdmsun.so"
 HOST = HERE / "host"
 HOST_LIB = HERE / "libdmsun_host.so"
 RUNNER = HERE / "dmsun_run"
+SERVER = HERE / "dmsun_serve"
 CXX = os.environ.get("CXX", "g++")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc" if os.path.exists("/opt/rocm/bin/hipcc") else "hipcc")
@@ -49,11 +50,13 @@ def build_host(verbose=False):
     link = ["-L", str(HERE), "-ldmsun", "-Wl,-rpath,$ORIGIN"]
     steps = [
         [CXX, "-O2", "-std=c++17", "-fPIC", "-shared", "-o", str(HOST_LIB) + ".tmp", str(HOST / "dmsun_output.cpp"),
-         str(HOST / "dmsun_host.cpp"), *link],
+         str(HOST / "dmsun_host.cpp"), str(HOST / "dmsun_server.cpp"), "-pthread", *link],
         [CXX, "-O2", "-std=c++17", "-o", str(RUNNER) + ".tmp", str(HOST / "dmsun_main.cpp"), "-L", str(HERE),
          "-ldmsun_host", "-ldmsun", "-Wl,-rpath,$ORIGIN"],
+        [CXX, "-O2", "-std=c++17", "-pthread", "-o", str(SERVER) + ".tmp", str(HOST / "dmsun_serve.cpp"), "-L", str(HERE),
+         "-ldmsun_host", "-ldmsun", "-Wl,-rpath,$ORIGIN"],
     ]
-    for cmd, out in zip(steps, (HOST_LIB, RUNNER)):
+    for cmd, out in zip(steps, (HOST_LIB, RUNNER, SERVER)):
         if verbose:
             print(" ".join(cmd), flush=True)
         res = subprocess.run(cmd, capture_output=True, text=True)

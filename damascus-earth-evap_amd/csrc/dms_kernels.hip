@@ -32,7 +32,7 @@ __device__ inline double shfl_d(double x, int src) { return __shfl(x, src); }
 
 } // namespace
 
-template <bool MX>
+template <bool MX, bool MP>
 __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 {
 	const int lane = threadIdx.x & 63;
@@ -86,12 +86,14 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 			const double xi = shfl_d(L.sc_xi, s);
 			const int k = (lane < n_iso) ? lane : ((lane == DMS_MAX_TARGETS - 1) ? lane : -1);
 			int target = -2;
+			/* the scattering lane's parameter point (uniform over the wave) */
+			const DmsPointView& Ps = MP ? a.job_points[__shfl((int) L.job, s)] : a.P;
 			if constexpr(MX)
 			{
 				/* FP32 rates (1/s), same draw and order as below */
 				float rate = 0.0f;
 				if(k >= 0)
-					rate = dms_rate_target_mx(a.P, a.S, r, v, k);
+					rate = dms_rate_target_mx(Ps, a.S, r, v, k);
 				const bool bad = (k >= 0) && (!mx_isfinite(rate) || rate < 0.0f);
 				if(__ballot(bad) == 0)
 				{
@@ -123,7 +125,7 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 			}
 			double rate = 0.0;
 			if(k >= 0)
-				rate = dms_rate_target(a.P, a.S, r, v, k);
+				rate = dms_rate_target(Ps, a.S, r, v, k);
 			const bool bad = (k >= 0) && (!dm_isfinite(rate) || rate < 0.0);
 			if(__ballot(bad) == 0)
 			{
@@ -176,10 +178,24 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 				if(idx < a.n)
 				{
 					L.slot = idx;
-					uint32_t j = 0;
-					while(j + 1 < a.n_jobs && a.job_start[j + 1] <= idx)
-						j++;
+					/* job of the slot: last j with job_start[j] <= idx */
+					uint32_t lo = 0, hi = a.n_jobs;
+					while(hi - lo > 1)
+					{
+						const uint32_t mid = (lo + hi) >> 1;
+						if(a.job_start[mid] <= idx)
+							lo = mid;
+						else
+							hi = mid;
+					}
+					const uint32_t j = lo;
 					L.job = j;
+					if constexpr(MP)
+					{
+						c.P = a.job_points[j];
+						c.J.seed = a.job_keys[2 * j];
+						c.J.point_id = a.job_keys[2 * j + 1];
+					}
 					L.id = a.job_first_id[j] + (uint64_t) (idx - a.job_start[j]);
 					L.row = a.R.per_lane ? lane_gid : idx;
 					L.phase = PH_START;
@@ -201,9 +217,13 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 	}
 }
 
-__global__ void __launch_bounds__(DMS_TRAJ_BLOCK) dms_traj_kernel(DmsTrajArgs a) { dms_traj_body<false>(a); }
+__global__ void __launch_bounds__(DMS_TRAJ_BLOCK) dms_traj_kernel(DmsTrajArgs a) { dms_traj_body<false, false>(a); }
 
-__global__ void __launch_bounds__(DMS_TRAJ_BLOCK) dms_traj_kernel_mixed(DmsTrajArgs a) { dms_traj_body<true>(a); }
+__global__ void __launch_bounds__(DMS_TRAJ_BLOCK) dms_traj_kernel_mixed(DmsTrajArgs a) { dms_traj_body<true, false>(a); }
+
+__global__ void __launch_bounds__(DMS_TRAJ_BLOCK) dms_traj_kernel_multi(DmsTrajArgs a) { dms_traj_body<false, true>(a); }
+
+__global__ void __launch_bounds__(DMS_TRAJ_BLOCK) dms_traj_kernel_mixed_multi(DmsTrajArgs a) { dms_traj_body<true, true>(a); }
 
 /* FP32 copy of the rate table in 1/s for the mixed-precision kernel */
 __global__ void dms_grid_f32_kernel(const double* grid, size_t n, float* out)

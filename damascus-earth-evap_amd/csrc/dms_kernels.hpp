@@ -33,10 +33,16 @@ struct DmsTrajArgs
 	/* finished trajectories per job (progress, read by the host while the
 	 * launch runs); may be null */
 	unsigned int* job_done;
+	/* jobs of different parameter points (multi-point launch): per-job point
+	 * views and Philox keys {seed, point_id}; null for one point */
+	const DmsPointView* job_points;
+	const uint32_t* job_keys;
 };
 
 __global__ void dms_traj_kernel(DmsTrajArgs a);
 __global__ void dms_traj_kernel_mixed(DmsTrajArgs a);
+__global__ void dms_traj_kernel_multi(DmsTrajArgs a);
+__global__ void dms_traj_kernel_mixed_multi(DmsTrajArgs a);
 __global__ void dms_grid_f32_kernel(const double* grid, size_t n, float* out);
 __global__ void dms_ic_kernel(DmsHaloView H, uint32_t seed, uint32_t point_id, uint64_t first_id, uint32_t n, double R_final,
                               dms_event* out, int32_t* ok);

@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <condition_variable>
 #include <functional>
 #include <memory>
 #include <mutex>
@@ -28,11 +29,21 @@
 #include "dms_host.hpp"
 #include "dms_kernels.hip"
 
+/* jobs (or batches) that may run on the device at the same time */
+#define DMS_CONCURRENT_JOBS 8
+
 struct dms_ctx
 {
 	int device = -1;
 	hipStream_t stream = nullptr;
 	hipStream_t progress_stream = nullptr; /* progress reads while a launch runs */
+	/* streams of concurrently running jobs (pairs: work, progress reads) */
+	std::vector<hipStream_t> pool;
+	std::vector<int> pool_free;
+	std::mutex pool_mu;
+	std::condition_variable pool_cv;
+	/* device memory a chunked job may use for its trajectory buffers */
+	size_t job_memory_budget = (size_t) 32 << 30;
 	std::mutex mu;
 	std::string err;
 	dms::SolarTables T;
@@ -196,6 +207,23 @@ std::string check_halo(const dms_halo& h)
 	return "";
 }
 
+/* Stream-ordered device allocations of one job (hipMallocAsync/hipFreeAsync on
+ * the job's stream), so that finishing a job never waits for other jobs'
+ * kernels. */
+template <typename T>
+hipError_t dalloc(T** p, size_t count, hipStream_t st)
+{
+	*p = nullptr;
+	return hipMallocAsync((void**) p, sizeof(T) * (count ? count : 1), st);
+}
+template <typename T>
+void dfree(T*& p, hipStream_t st)
+{
+	if(p)
+		hipFreeAsync(p, st);
+	p = nullptr;
+}
+
 /* Device buffers of one chunk, grown on demand. */
 struct ChunkBuffers
 {
@@ -209,62 +237,49 @@ struct ChunkBuffers
 	unsigned int* next = nullptr;
 	size_t cap = 0, rows_cap = 0, recs_cap = 0;
 
-	void release()
+	void release(hipStream_t st)
 	{
-		hipFree(ics);
-		hipFree(ok);
-		hipFree(summ);
-		hipFree(recs);
-		hipFree(rows);
-		hipFree(counts);
-		hipFree(cls);
-		hipFree(next);
+		dfree(ics, st);
+		dfree(ok, st);
+		dfree(summ, st);
+		dfree(recs, st);
+		dfree(rows, st);
+		dfree(counts, st);
+		dfree(cls, st);
+		dfree(next, st);
 		*this = ChunkBuffers();
 	}
 };
 
-hipError_t ensure(ChunkBuffers& B, size_t n, int nbins, bool rows, bool recs)
+hipError_t ensure(ChunkBuffers& B, size_t n, int nbins, bool rows, bool recs, hipStream_t st)
 {
 	hipError_t e = hipSuccess;
-	if(B.next == nullptr && (e = hipMalloc((void**) &B.next, sizeof(unsigned int))) != hipSuccess)
+	if(B.next == nullptr && (e = dalloc(&B.next, 1, st)) != hipSuccess)
 		return e;
 	if(n > B.cap)
 	{
-		hipFree(B.ics);
-		hipFree(B.ok);
-		hipFree(B.summ);
-		hipFree(B.cls);
-		B.ics = nullptr;
-		B.ok = nullptr;
-		B.summ = nullptr;
-		B.cls = nullptr;
-		if((e = hipMalloc((void**) &B.ics, sizeof(dms_event) * n)) != hipSuccess)
-			return e;
-		if((e = hipMalloc((void**) &B.ok, sizeof(int32_t) * n)) != hipSuccess)
-			return e;
-		if((e = hipMalloc((void**) &B.summ, sizeof(DmsSummary) * n)) != hipSuccess)
-			return e;
-		if((e = hipMalloc((void**) &B.cls, n)) != hipSuccess)
+		dfree(B.ics, st);
+		dfree(B.ok, st);
+		dfree(B.summ, st);
+		dfree(B.cls, st);
+		if((e = dalloc(&B.ics, n, st)) != hipSuccess || (e = dalloc(&B.ok, n, st)) != hipSuccess
+		   || (e = dalloc(&B.summ, n, st)) != hipSuccess || (e = dalloc(&B.cls, n, st)) != hipSuccess)
 			return e;
 		B.cap = n;
 	}
 	if(rows && n > B.rows_cap)
 	{
-		hipFree(B.rows);
-		hipFree(B.counts);
-		B.rows = nullptr;
-		B.counts = nullptr;
-		if((e = hipMalloc((void**) &B.rows, sizeof(double) * 2 * (size_t) nbins * n)) != hipSuccess)
-			return e;
-		if((e = hipMalloc((void**) &B.counts, sizeof(uint32_t) * (size_t) nbins * n)) != hipSuccess)
+		dfree(B.rows, st);
+		dfree(B.counts, st);
+		if((e = dalloc(&B.rows, 2 * (size_t) nbins * n, st)) != hipSuccess
+		   || (e = dalloc(&B.counts, (size_t) nbins * n, st)) != hipSuccess)
 			return e;
 		B.rows_cap = n;
 	}
 	if(recs && n > B.recs_cap)
 	{
-		hipFree(B.recs);
-		B.recs = nullptr;
-		if((e = hipMalloc((void**) &B.recs, sizeof(dms_traj_record) * n)) != hipSuccess)
+		dfree(B.recs, st);
+		if((e = dalloc(&B.recs, n, st)) != hipSuccess)
 			return e;
 		B.recs_cap = n;
 	}
@@ -287,30 +302,41 @@ struct LaunchJobs
 	uint32_t* d_start = nullptr;
 	uint64_t* d_first = nullptr;
 	unsigned int* d_done = nullptr;
+	/* multi-point launch */
+	std::vector<DmsPointView> points;
+	std::vector<uint32_t> keys;
+	DmsPointView* d_points = nullptr;
+	uint32_t* d_keys = nullptr;
 
-	hipError_t upload()
+	/* the host vectors must stay unchanged until the stream has run the copies */
+	hipError_t upload(hipStream_t st)
 	{
 		hipError_t e;
-		if((e = hipMalloc((void**) &d_start, sizeof(uint32_t) * start.size())) != hipSuccess)
+		if((e = dalloc(&d_start, start.size(), st)) != hipSuccess || (e = dalloc(&d_first, first_id.size(), st)) != hipSuccess
+		   || (e = dalloc(&d_done, start.size(), st)) != hipSuccess)
 			return e;
-		if((e = hipMalloc((void**) &d_first, sizeof(uint64_t) * first_id.size())) != hipSuccess)
+		if((e = hipMemsetAsync(d_done, 0, sizeof(unsigned int) * start.size(), st)) != hipSuccess)
 			return e;
-		if((e = hipMalloc((void**) &d_done, sizeof(unsigned int) * start.size())) != hipSuccess)
+		if(!points.empty())
+		{
+			if((e = dalloc(&d_points, points.size(), st)) != hipSuccess || (e = dalloc(&d_keys, keys.size(), st)) != hipSuccess)
+				return e;
+			if((e = hipMemcpyAsync(d_points, points.data(), sizeof(DmsPointView) * points.size(), hipMemcpyHostToDevice, st))
+			       != hipSuccess
+			   || (e = hipMemcpyAsync(d_keys, keys.data(), sizeof(uint32_t) * keys.size(), hipMemcpyHostToDevice, st)) != hipSuccess)
+				return e;
+		}
+		if((e = hipMemcpyAsync(d_start, start.data(), sizeof(uint32_t) * start.size(), hipMemcpyHostToDevice, st)) != hipSuccess)
 			return e;
-		if((e = hipMemset(d_done, 0, sizeof(unsigned int) * start.size())) != hipSuccess)
-			return e;
-		if((e = hipMemcpy(d_start, start.data(), sizeof(uint32_t) * start.size(), hipMemcpyHostToDevice)) != hipSuccess)
-			return e;
-		return hipMemcpy(d_first, first_id.data(), sizeof(uint64_t) * first_id.size(), hipMemcpyHostToDevice);
+		return hipMemcpyAsync(d_first, first_id.data(), sizeof(uint64_t) * first_id.size(), hipMemcpyHostToDevice, st);
 	}
-	void release()
+	void release(hipStream_t st)
 	{
-		hipFree(d_start);
-		hipFree(d_first);
-		hipFree(d_done);
-		d_start = nullptr;
-		d_first = nullptr;
-		d_done = nullptr;
+		dfree(d_start, st);
+		dfree(d_first, st);
+		dfree(d_done, st);
+		dfree(d_points, st);
+		dfree(d_keys, st);
 	}
 };
 
@@ -320,8 +346,9 @@ struct LaunchJobs
  * count of each job of the launch. */
 using ProgressFn = std::function<void(const std::vector<unsigned int>&)>;
 
-hipError_t run_trajectories(dms_ctx* ctx, const dms_point* pt, const DmsJobView& J, ChunkBuffers& B, size_t n, bool rows,
-                            bool recs, const LaunchJobs& LJ, const DmsRows* lane_rows, float* ms, const ProgressFn& progress = nullptr)
+hipError_t run_trajectories(dms_ctx* ctx, hipStream_t st, hipStream_t pst, const dms_point* pt, const DmsJobView& J, ChunkBuffers& B,
+                            size_t n, bool rows, bool recs, const LaunchJobs& LJ, const DmsRows* lane_rows, float* ms,
+                            const ProgressFn& progress = nullptr)
 {
 	DmsTrajArgs a;
 	a.S = ctx->dev;
@@ -348,36 +375,36 @@ hipError_t run_trajectories(dms_ctx* ctx, const dms_point* pt, const DmsJobView&
 	a.job_start = LJ.d_start;
 	a.job_first_id = LJ.d_first;
 	a.job_done = LJ.d_done;
+	a.job_points = LJ.d_points;
+	a.job_keys = LJ.d_keys;
 	hipError_t e;
-	if((e = hipMemsetAsync(B.next, 0, sizeof(unsigned int), ctx->stream)) != hipSuccess)
+	if((e = hipMemsetAsync(B.next, 0, sizeof(unsigned int), st)) != hipSuccess)
 		return e;
 	if(rows && !lane_rows)
 	{
-		if((e = hipMemsetAsync(B.rows, 0, sizeof(double) * 2 * (size_t) J.nbins * n, ctx->stream)) != hipSuccess)
+		if((e = hipMemsetAsync(B.rows, 0, sizeof(double) * 2 * (size_t) J.nbins * n, st)) != hipSuccess)
 			return e;
-		if((e = hipMemsetAsync(B.counts, 0, sizeof(uint32_t) * (size_t) J.nbins * n, ctx->stream)) != hipSuccess)
+		if((e = hipMemsetAsync(B.counts, 0, sizeof(uint32_t) * (size_t) J.nbins * n, st)) != hipSuccess)
 			return e;
 	}
 	hipEvent_t e0, e1;
 	hipEventCreate(&e0);
 	hipEventCreate(&e1);
 	(void) hipGetLastError();
-	hipEventRecord(e0, ctx->stream);
-	if(J.mixed)
-		hipLaunchKernelGGL(dms_traj_kernel_mixed, dim3(traj_grid(ctx, n)), dim3(DMS_TRAJ_BLOCK), ctx->traj_lds, ctx->stream, a);
-	else
-		hipLaunchKernelGGL(dms_traj_kernel, dim3(traj_grid(ctx, n)), dim3(DMS_TRAJ_BLOCK), ctx->traj_lds, ctx->stream, a);
+	hipEventRecord(e0, st);
+	void (*kernel)(DmsTrajArgs) = LJ.d_points ? (J.mixed ? dms_traj_kernel_mixed_multi : dms_traj_kernel_multi)
+	                                          : (J.mixed ? dms_traj_kernel_mixed : dms_traj_kernel);
+	hipLaunchKernelGGL(kernel, dim3(traj_grid(ctx, n)), dim3(DMS_TRAJ_BLOCK), ctx->traj_lds, st, a);
 	e = hipGetLastError();
-	hipEventRecord(e1, ctx->stream);
+	hipEventRecord(e1, st);
 	if(e == hipSuccess && progress && LJ.d_done)
 	{
 		/* heartbeat: read the per-job finished counters on a second stream */
 		std::vector<unsigned int> done(LJ.start.size(), 0u);
 		while(hipEventQuery(e1) == hipErrorNotReady)
 		{
-			if(hipMemcpyAsync(done.data(), LJ.d_done, sizeof(unsigned int) * done.size(), hipMemcpyDeviceToHost,
-			                  ctx->progress_stream) == hipSuccess
-			   && hipStreamSynchronize(ctx->progress_stream) == hipSuccess)
+			if(hipMemcpyAsync(done.data(), LJ.d_done, sizeof(unsigned int) * done.size(), hipMemcpyDeviceToHost, pst) == hipSuccess
+			   && hipStreamSynchronize(pst) == hipSuccess)
 				progress(done);
 			std::this_thread::sleep_for(std::chrono::milliseconds(20));
 		}
@@ -394,16 +421,17 @@ hipError_t run_trajectories(dms_ctx* ctx, const dms_point* pt, const DmsJobView&
 }
 
 /* single-job launch over slots [0, n) with ids first_id .. */
-hipError_t run_trajectories_single(dms_ctx* ctx, const dms_point* pt, const DmsJobView& J, ChunkBuffers& B, size_t n, bool rows,
-                                   bool recs, uint64_t first_id, float* ms, const ProgressFn& progress = nullptr)
+hipError_t run_trajectories_single(dms_ctx* ctx, hipStream_t st, hipStream_t pst, const dms_point* pt, const DmsJobView& J,
+                                   ChunkBuffers& B, size_t n, bool rows, bool recs, uint64_t first_id, float* ms,
+                                   const ProgressFn& progress = nullptr)
 {
 	LaunchJobs LJ;
 	LJ.start.push_back(0);
 	LJ.first_id.push_back(first_id);
-	hipError_t e = LJ.upload();
+	hipError_t e = LJ.upload(st);
 	if(e == hipSuccess)
-		e = run_trajectories(ctx, pt, J, B, n, rows, recs, LJ, nullptr, ms, progress);
-	LJ.release();
+		e = run_trajectories(ctx, st, pst, pt, J, B, n, rows, recs, LJ, nullptr, ms, progress);
+	LJ.release(st);
 	return e;
 }
 
@@ -580,7 +608,7 @@ int account_trajectory(dms_job* job, RoundReplay& R, const DmsSummary& s, double
 
 /* Chunked job: speculative id-ordered chunks with per-trajectory rows, so that
  * capture-target runs count exactly the trajectories the reference would. */
-void run_job_chunked(dms_job* job)
+void run_job_chunked(dms_job* job, hipStream_t st, hipStream_t pst)
 {
 	dms_point* pt = job->pt;
 	dms_ctx* ctx = pt->ctx;
@@ -613,20 +641,21 @@ void run_job_chunked(dms_job* job)
 	};
 	if(want_rows)
 	{
-		if(hipMalloc((void**) &d_hist, sizeof(double) * 8 * nb) != hipSuccess
-		   || hipMalloc((void**) &d_counts, sizeof(unsigned long long) * 2 * nb) != hipSuccess)
+		if(dalloc(&d_hist, 8 * (size_t) nb, st) != hipSuccess || dalloc(&d_counts, 2 * (size_t) nb, st) != hipSuccess)
 		{
-			set_err("hipMalloc of histogram accumulators failed");
+			set_err("device allocation of histogram accumulators failed");
+			dfree(d_hist, st);
+			dfree(d_counts, st);
 			return;
 		}
-		hipMemset(d_hist, 0, sizeof(double) * 8 * nb);
-		hipMemset(d_counts, 0, sizeof(unsigned long long) * 2 * nb);
+		hipMemsetAsync(d_hist, 0, sizeof(double) * 8 * nb, st);
+		hipMemsetAsync(d_counts, 0, sizeof(unsigned long long) * 2 * nb, st);
 	}
 	/* Chunk size: bounded by a device-memory budget for the per-trajectory rows. */
 	const size_t row_bytes = want_rows ? (size_t) nb * (2 * sizeof(double) + sizeof(uint32_t)) : 0;
 	const size_t rec_bytes = want_recs ? sizeof(dms_traj_record) : 0;
 	const size_t per_traj = row_bytes + rec_bytes + sizeof(dms_event) + sizeof(DmsSummary) + 8;
-	const size_t budget = (size_t) 48 << 30;
+	const size_t budget = ctx->job_memory_budget;
 	size_t max_chunk = budget / per_traj;
 	if(max_chunk > (size_t) 1 << 26)
 		max_chunk = (size_t) 1 << 26;
@@ -654,18 +683,18 @@ void run_job_chunked(dms_job* job)
 		size_t n = (size_t) (want < max_chunk ? want : max_chunk);
 		if(n == 0)
 			n = 1;
-		if(ensure(B, n, nb, want_rows, want_recs) != hipSuccess)
+		if(ensure(B, n, nb, want_rows, want_recs, st) != hipSuccess)
 		{
 			set_err("device allocation for a trajectory chunk failed");
 			break;
 		}
 		const double t0 = now_ms();
 		(void) hipGetLastError();
-		hipLaunchKernelGGL(dms_ic_kernel, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, ctx->stream, H, d.seed, d.point_id,
+		hipLaunchKernelGGL(dms_ic_kernel, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, st, H, d.seed, d.point_id,
 		                   next_id, (uint32_t) n, d.r_max_rsun * DMS_rSun, B.ics, B.ok);
 		float kms = 0.0f;
 		const uint64_t base_done = C.trajectories;
-		hipError_t e = run_trajectories_single(ctx, pt, J, B, n, want_rows, want_recs, next_id, &kms,
+		hipError_t e = run_trajectories_single(ctx, st, pst, pt, J, B, n, want_rows, want_recs, next_id, &kms,
 		                                       [job, base_done](const std::vector<unsigned int>& d) { job->done_traj = base_done + d[0]; });
 		if(e != hipSuccess)
 		{
@@ -675,11 +704,16 @@ void run_job_chunked(dms_job* job)
 		job->kernel_ms += kms;
 		job->launches += 1.0;
 		hsum.resize(n);
-		hipMemcpy(hsum.data(), B.summ, sizeof(DmsSummary) * n, hipMemcpyDeviceToHost);
+		hipMemcpyAsync(hsum.data(), B.summ, sizeof(DmsSummary) * n, hipMemcpyDeviceToHost, st);
 		if(want_recs)
 		{
 			hrec.resize(n);
-			hipMemcpy(hrec.data(), B.recs, sizeof(dms_traj_record) * n, hipMemcpyDeviceToHost);
+			hipMemcpyAsync(hrec.data(), B.recs, sizeof(dms_traj_record) * n, hipMemcpyDeviceToHost, st);
+		}
+		if(hipStreamSynchronize(st) != hipSuccess)
+		{
+			set_err("reading trajectory summaries failed");
+			break;
 		}
 		simulated += n;
 		next_id += n;
@@ -698,26 +732,25 @@ void run_job_chunked(dms_job* job)
 		}
 		if(want_rows && counted > 0)
 		{
-			hipMemcpyAsync(B.cls, hcls.data(), counted, hipMemcpyHostToDevice, ctx->stream);
+			hipMemcpyAsync(B.cls, hcls.data(), counted, hipMemcpyHostToDevice, st);
 			const uint32_t nblk = (uint32_t) ((counted + TB - 1) / TB);
 			const size_t need = (size_t) nblk * nb;
 			if(need > partial_cap)
 			{
-				hipFree(d_partial);
-				hipFree(d_pcount);
-				if(hipMalloc((void**) &d_partial, sizeof(double) * 8 * need) != hipSuccess
-				   || hipMalloc((void**) &d_pcount, sizeof(unsigned long long) * 2 * need) != hipSuccess)
+				dfree(d_partial, st);
+				dfree(d_pcount, st);
+				if(dalloc(&d_partial, 8 * need, st) != hipSuccess || dalloc(&d_pcount, 2 * need, st) != hipSuccess)
 				{
-					set_err("hipMalloc of histogram partials failed");
+					set_err("device allocation of histogram partials failed");
 					break;
 				}
 				partial_cap = need;
 			}
-			hipLaunchKernelGGL(dms_hist_partial_kernel, dim3((unsigned) ((nb + 127) / 128), nblk), dim3(128), 0, ctx->stream,
+			hipLaunchKernelGGL(dms_hist_partial_kernel, dim3((unsigned) ((nb + 127) / 128), nblk), dim3(128), 0, st,
 			                   B.rows, B.counts, B.cls, (uint32_t) counted, nb, TB, d_partial, d_pcount);
-			hipLaunchKernelGGL(dms_hist_final_kernel, dim3((unsigned) ((nb + 127) / 128)), dim3(128), 0, ctx->stream, d_partial,
+			hipLaunchKernelGGL(dms_hist_final_kernel, dim3((unsigned) ((nb + 127) / 128)), dim3(128), 0, st, d_partial,
 			                   d_pcount, nblk, nb, d_hist, d_counts);
-			if(hipStreamSynchronize(ctx->stream) != hipSuccess)
+			if(hipStreamSynchronize(st) != hipSuccess)
 			{
 				set_err("histogram reduction failed");
 				break;
@@ -733,21 +766,22 @@ void run_job_chunked(dms_job* job)
 	C.computing_time_sec = (now_ms() - t_start) * 1e-3;
 	if(want_rows && job->status == DMS_OK)
 	{
-		hipMemcpy(job->hist.data(), d_hist, sizeof(double) * 8 * nb, hipMemcpyDeviceToHost);
-		hipMemcpy(job->counts.data(), d_counts, sizeof(unsigned long long) * 2 * nb, hipMemcpyDeviceToHost);
+		hipMemcpyAsync(job->hist.data(), d_hist, sizeof(double) * 8 * nb, hipMemcpyDeviceToHost, st);
+		hipMemcpyAsync(job->counts.data(), d_counts, sizeof(unsigned long long) * 2 * nb, hipMemcpyDeviceToHost, st);
 	}
-	hipFree(d_hist);
-	hipFree(d_counts);
-	hipFree(d_partial);
-	hipFree(d_pcount);
-	B.release();
+	dfree(d_hist, st);
+	dfree(d_counts, st);
+	dfree(d_partial, st);
+	dfree(d_pcount, st);
+	B.release(st);
+	hipStreamSynchronize(st);
 }
 
 /* Fixed-count jobs (max_trajectories > 0, no capture target) of one point
  * streamed through a single persistent launch: lanes refill across job
  * boundaries, so the long tail of one job overlaps the bulk of the others.
  * Histograms use per-lane scratch rows flushed into per-job sums. */
-void run_batch_fixed(std::vector<dms_job*>& jobs)
+void run_batch_fixed(std::vector<dms_job*>& jobs, hipStream_t st, hipStream_t pst)
 {
 	dms_point* pt = jobs[0]->pt;
 	dms_ctx* ctx = pt->ctx;
@@ -762,10 +796,19 @@ void run_batch_fixed(std::vector<dms_job*>& jobs)
 	const size_t nj = jobs.size();
 	LaunchJobs LJ;
 	size_t n = 0;
+	bool multi = false;
+	for(dms_job* j : jobs)
+		multi = multi || j->pt != pt || j->desc.seed != d0.seed || j->desc.point_id != d0.point_id;
 	for(dms_job* j : jobs)
 	{
 		LJ.start.push_back((uint32_t) n);
 		LJ.first_id.push_back(j->desc.first_trajectory_id);
+		if(multi)
+		{
+			LJ.points.push_back(j->pt->dev);
+			LJ.keys.push_back(j->desc.seed);
+			LJ.keys.push_back(j->desc.point_id);
+		}
 		n += (size_t) j->desc.max_trajectories;
 		memset(&j->C, 0, sizeof(j->C));
 		j->events.clear();
@@ -785,36 +828,35 @@ void run_batch_fixed(std::vector<dms_job*>& jobs)
 	DmsRows LR;
 	memset(&LR, 0, sizeof(LR));
 	const size_t lanes = (size_t) traj_grid(ctx, n) * DMS_TRAJ_BLOCK;
-	bool ok = LJ.upload() == hipSuccess && ensure(B, n, nb, false, want_recs) == hipSuccess;
+	bool ok = LJ.upload(st) == hipSuccess && ensure(B, n, nb, false, want_recs, st) == hipSuccess;
 	if(ok && want_rows)
 	{
 		LR.per_lane = 1;
-		ok = hipMalloc((void**) &LR.rows, sizeof(double) * 2 * (size_t) nb * lanes) == hipSuccess
-		     && hipMalloc((void**) &LR.counts, sizeof(uint32_t) * (size_t) nb * lanes) == hipSuccess
-		     && hipMalloc((void**) &LR.touched, sizeof(uint16_t) * (size_t) nb * lanes) == hipSuccess
-		     && hipMalloc((void**) &LR.hist, sizeof(double) * 8 * (size_t) nb * nj) == hipSuccess
-		     && hipMalloc((void**) &LR.hcounts, sizeof(unsigned long long) * 2 * (size_t) nb * nj) == hipSuccess;
+		ok = dalloc(&LR.rows, 2 * (size_t) nb * lanes, st) == hipSuccess && dalloc(&LR.counts, (size_t) nb * lanes, st) == hipSuccess
+		     && dalloc(&LR.touched, (size_t) nb * lanes, st) == hipSuccess && dalloc(&LR.hist, 8 * (size_t) nb * nj, st) == hipSuccess
+		     && dalloc(&LR.hcounts, 2 * (size_t) nb * nj, st) == hipSuccess;
 		if(ok)
-			ok = hipMemsetAsync(LR.rows, 0, sizeof(double) * 2 * (size_t) nb * lanes, ctx->stream) == hipSuccess
-			     && hipMemsetAsync(LR.counts, 0, sizeof(uint32_t) * (size_t) nb * lanes, ctx->stream) == hipSuccess
-			     && hipMemsetAsync(LR.hist, 0, sizeof(double) * 8 * (size_t) nb * nj, ctx->stream) == hipSuccess
-			     && hipMemsetAsync(LR.hcounts, 0, sizeof(unsigned long long) * 2 * (size_t) nb * nj, ctx->stream) == hipSuccess;
+			ok = hipMemsetAsync(LR.rows, 0, sizeof(double) * 2 * (size_t) nb * lanes, st) == hipSuccess
+			     && hipMemsetAsync(LR.counts, 0, sizeof(uint32_t) * (size_t) nb * lanes, st) == hipSuccess
+			     && hipMemsetAsync(LR.hist, 0, sizeof(double) * 8 * (size_t) nb * nj, st) == hipSuccess
+			     && hipMemsetAsync(LR.hcounts, 0, sizeof(unsigned long long) * 2 * (size_t) nb * nj, st) == hipSuccess;
 	}
 	if(!ok)
 		set_err("device allocation for a batched launch failed");
 	float kms = 0.0f;
 	if(ok)
 	{
-		const DmsHaloView H = halo_view(ctx, d0.halo);
 		(void) hipGetLastError();
 		for(size_t k = 0; k < nj; k++)
 		{
-			const size_t nk = (size_t) jobs[k]->desc.max_trajectories;
-			hipLaunchKernelGGL(dms_ic_kernel, dim3((unsigned) ((nk + 255) / 256)), dim3(256), 0, ctx->stream, H, d0.seed, d0.point_id,
-			                   jobs[k]->desc.first_trajectory_id, (uint32_t) nk, d0.r_max_rsun * DMS_rSun, B.ics + LJ.start[k],
+			const dms_job_desc& dk = jobs[k]->desc;
+			const DmsHaloView H = halo_view(ctx, dk.halo);
+			const size_t nk = (size_t) dk.max_trajectories;
+			hipLaunchKernelGGL(dms_ic_kernel, dim3((unsigned) ((nk + 255) / 256)), dim3(256), 0, st, H, dk.seed, dk.point_id,
+			                   dk.first_trajectory_id, (uint32_t) nk, dk.r_max_rsun * DMS_rSun, B.ics + LJ.start[k],
 			                   B.ok + LJ.start[k]);
 		}
-		hipError_t e = run_trajectories(ctx, pt, J, B, n, want_rows, want_recs, LJ, want_rows ? &LR : nullptr, &kms,
+		hipError_t e = run_trajectories(ctx, st, pst, pt, J, B, n, want_rows, want_recs, LJ, want_rows ? &LR : nullptr, &kms,
 		                                [&jobs](const std::vector<unsigned int>& d) {
 			                                for(size_t k = 0; k < jobs.size() && k < d.size(); k++)
 				                                jobs[k]->done_traj = d[k];
@@ -828,19 +870,27 @@ void run_batch_fixed(std::vector<dms_job*>& jobs)
 	if(ok)
 	{
 		std::vector<DmsSummary> hsum(n);
-		hipMemcpy(hsum.data(), B.summ, sizeof(DmsSummary) * n, hipMemcpyDeviceToHost);
+		hipMemcpyAsync(hsum.data(), B.summ, sizeof(DmsSummary) * n, hipMemcpyDeviceToHost, st);
 		std::vector<dms_traj_record> hrec;
 		if(want_recs)
 		{
 			hrec.resize(n);
-			hipMemcpy(hrec.data(), B.recs, sizeof(dms_traj_record) * n, hipMemcpyDeviceToHost);
+			hipMemcpyAsync(hrec.data(), B.recs, sizeof(dms_traj_record) * n, hipMemcpyDeviceToHost, st);
 		}
+		if(want_rows)
+			for(size_t k = 0; k < nj; k++)
+			{
+				hipMemcpyAsync(jobs[k]->hist.data(), LR.hist + (size_t) k * 8 * nb, sizeof(double) * 8 * nb, hipMemcpyDeviceToHost, st);
+				hipMemcpyAsync(jobs[k]->counts.data(), LR.hcounts + (size_t) k * 2 * nb, sizeof(unsigned long long) * 2 * nb,
+				               hipMemcpyDeviceToHost, st);
+			}
+		hipStreamSynchronize(st);
 		const double elapsed_sec = (now_ms() - t_start) * 1e-3;
 		for(size_t k = 0; k < nj; k++)
 		{
 			dms_job* job = jobs[k];
 			RoundReplay R;
-			R.init(job->desc, pt, &job->C);
+			R.init(job->desc, job->pt, &job->C);
 			const size_t nk = (size_t) job->desc.max_trajectories;
 			size_t counted = 0;
 			for(size_t i = 0; i < nk; i++)
@@ -859,12 +909,6 @@ void run_batch_fixed(std::vector<dms_job*>& jobs)
 			job->kernel_ms = kms;
 			job->launches = 1.0;
 			job->batch_size = (double) nj;
-			if(want_rows)
-			{
-				hipMemcpy(job->hist.data(), LR.hist + (size_t) k * 8 * nb, sizeof(double) * 8 * nb, hipMemcpyDeviceToHost);
-				hipMemcpy(job->counts.data(), LR.hcounts + (size_t) k * 2 * nb, sizeof(unsigned long long) * 2 * nb,
-				          hipMemcpyDeviceToHost);
-			}
 			job->done_traj = job->C.trajectories;
 			job->done_captured = job->C.captured;
 			job->done_scat = job->C.total_scatterings;
@@ -877,18 +921,19 @@ void run_batch_fixed(std::vector<dms_job*>& jobs)
 			job->other_ms = total * 1e3 - kms;
 		}
 	}
-	hipFree(LR.rows);
-	hipFree(LR.counts);
-	hipFree(LR.touched);
-	hipFree(LR.hist);
-	hipFree(LR.hcounts);
-	LJ.release();
-	B.release();
+	dfree(LR.rows, st);
+	dfree(LR.counts, st);
+	dfree(LR.touched, st);
+	dfree(LR.hist, st);
+	dfree(LR.hcounts, st);
+	LJ.release(st);
+	B.release(st);
+	hipStreamSynchronize(st);
 	/* A run-level fuse stopped a job before its last trajectory: its histogram
 	 * sums hold trajectories past the stop, so recount it exactly (chunked). */
 	for(dms_job* job : jobs)
 		if(job->status == DMS_OK && job->needs_rerun)
-			run_job_chunked(job);
+			run_job_chunked(job, st, pst);
 }
 
 /* Largest slot count of one batched launch (initial conditions and summaries
@@ -909,16 +954,39 @@ bool same_batch_config(const dms_job_desc& a, const dms_job_desc& b)
 	       && memcmp(&a.halo, &b.halo, sizeof(dms_halo)) == 0;
 }
 
+/* Jobs of different points (and keys/halos) can share a launch when the
+ * kernel-wide job view agrees. */
+bool same_launch_config(const dms_job_desc& a, const dms_job_desc& b)
+{
+	return a.mode == b.mode && a.precision == b.precision && a.max_scatterings == b.max_scatterings
+	       && a.max_free_steps == b.max_free_steps && a.rk_attempt_budget == b.rk_attempt_budget && a.r_max_rsun == b.r_max_rsun
+	       && a.n_bins == b.n_bins;
+}
+
+/* A job (or batch) runs on a stream pair of its own, so jobs submitted while
+ * others run overlap on the device (continuous batching: the blocks of a new
+ * launch take the compute units that finished launches leave). */
 void run_batch_worker(std::shared_ptr<dms_batch> batch)
 {
 	dms_ctx* ctx = batch->jobs[0]->pt->ctx;
+	int slot;
 	{
-		std::lock_guard<std::mutex> lock(ctx->mu);
-		if(batch->jobs.size() == 1 && !batchable(batch->jobs[0]->desc))
-			run_job_chunked(batch->jobs[0]);
-		else
-			run_batch_fixed(batch->jobs);
+		std::unique_lock<std::mutex> lock(ctx->pool_mu);
+		ctx->pool_cv.wait(lock, [ctx] { return !ctx->pool_free.empty(); });
+		slot = ctx->pool_free.back();
+		ctx->pool_free.pop_back();
 	}
+	const hipStream_t st = ctx->pool[2 * slot], pst = ctx->pool[2 * slot + 1];
+	hipSetDevice(ctx->device);
+	if(batch->jobs.size() == 1 && !batchable(batch->jobs[0]->desc))
+		run_job_chunked(batch->jobs[0], st, pst);
+	else
+		run_batch_fixed(batch->jobs, st, pst);
+	{
+		std::lock_guard<std::mutex> lock(ctx->pool_mu);
+		ctx->pool_free.push_back(slot);
+	}
+	ctx->pool_cv.notify_one();
 	for(dms_job* j : batch->jobs)
 		j->finished = 1;
 }
@@ -1024,6 +1092,15 @@ int dms_create(dms_ctx** out, int device, const double* raw_rows, size_t n_rows)
 		if(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess
 		   || hipStreamCreateWithFlags(&ctx->progress_stream, hipStreamNonBlocking) != hipSuccess)
 			return cleanup("hipStreamCreate failed");
+		for(int k = 0; k < 2 * DMS_CONCURRENT_JOBS; k++)
+		{
+			hipStream_t sk = nullptr;
+			if(hipStreamCreateWithFlags(&sk, hipStreamNonBlocking) != hipSuccess)
+				return cleanup("hipStreamCreate failed");
+			ctx->pool.push_back(sk);
+		}
+		for(int k = 0; k < DMS_CONCURRENT_JOBS; k++)
+			ctx->pool_free.push_back(k);
 		hipDeviceProp_t prop;
 		if(hipGetDeviceProperties(&prop, device) != hipSuccess)
 			return cleanup("hipGetDeviceProperties failed");
@@ -1033,7 +1110,10 @@ int dms_create(dms_ctx** out, int device, const double* raw_rows, size_t n_rows)
 		if(hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess || (size_t) max_lds < lds + 16384)
 			return cleanup("solar profile too large for the LDS-resident Mass/escape-speed tables");
 		if(hipFuncSetAttribute((const void*) dms_traj_kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds) != hipSuccess
-		   || hipFuncSetAttribute((const void*) dms_traj_kernel_mixed, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds) != hipSuccess)
+		   || hipFuncSetAttribute((const void*) dms_traj_kernel_mixed, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds) != hipSuccess
+		   || hipFuncSetAttribute((const void*) dms_traj_kernel_multi, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds) != hipSuccess
+		   || hipFuncSetAttribute((const void*) dms_traj_kernel_mixed_multi, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds)
+		          != hipSuccess)
 			return cleanup("hipFuncSetAttribute(MaxDynamicSharedMemorySize) failed");
 		int blocks = 0;
 		if(hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, dms_traj_kernel, DMS_TRAJ_BLOCK, lds) != hipSuccess)
@@ -1082,8 +1162,18 @@ void dms_destroy(dms_ctx* ctx)
 			hipStreamDestroy(ctx->stream);
 		if(ctx->progress_stream)
 			hipStreamDestroy(ctx->progress_stream);
+		for(hipStream_t sk : ctx->pool)
+			hipStreamDestroy(sk);
 	}
 	delete ctx;
+}
+
+int dms_set_job_memory_budget(dms_ctx* ctx, size_t bytes)
+{
+	if(ctx == nullptr || bytes < ((size_t) 64 << 20))
+		return fail(ctx, DMS_ERR_ARG, "dms_set_job_memory_budget: null context or budget below 64 MiB");
+	ctx->job_memory_budget = bytes;
+	return DMS_OK;
 }
 
 int dms_solar_eval(const dms_ctx* ctx, int which, const double* r, double* out, size_t n)
@@ -1174,6 +1264,7 @@ int dms_point_create(dms_ctx* ctx, const dms_point_desc* desc, dms_point** out)
 	}
 	pt->N = (N >= 2) ? N : 0;
 	std::lock_guard<std::mutex> lock(ctx->mu);
+	hipSetDevice(ctx->device); /* callers may be on any host thread */
 	auto bail = [&](const std::string& what) {
 		hipFree(pt->d_sigmaN);
 		hipFree(pt->d_grid);
@@ -1222,6 +1313,7 @@ void dms_point_destroy(dms_point* pt)
 {
 	if(pt == nullptr)
 		return;
+	hipSetDevice(pt->ctx->device);
 	hipFree(pt->d_sigmaN);
 	hipFree(pt->d_grid);
 	hipFree(pt->d_grid32);
@@ -1237,6 +1329,7 @@ int dms_point_rate_grid(const dms_point* pt, double* out, size_t n)
 	const size_t cells = (size_t) pt->N * (size_t) pt->N;
 	if(n < cells)
 		return fail(pt->ctx, DMS_ERR_ARG, "dms_point_rate_grid: output too small");
+	hipSetDevice(pt->ctx->device);
 	DMS_HIP(pt->ctx, hipMemcpy(out, pt->d_grid, sizeof(double) * cells, hipMemcpyDeviceToHost));
 	return DMS_OK;
 }
@@ -1262,6 +1355,7 @@ int dms_point_rates(const dms_point* pt, int which, const double* r, const doubl
 		return DMS_OK;
 	dms_ctx* ctx = pt->ctx;
 	std::lock_guard<std::mutex> lock(ctx->mu);
+	hipSetDevice(ctx->device); /* callers may be on any host thread */
 	double *dr = nullptr, *dv = nullptr, *dout = nullptr;
 	DMS_HIP(ctx, hipMalloc((void**) &dr, sizeof(double) * n));
 	DMS_HIP(ctx, hipMalloc((void**) &dv, sizeof(double) * n));
@@ -1299,6 +1393,7 @@ int dms_initial_conditions(dms_ctx* ctx, const dms_halo* halo, uint32_t seed, ui
 	if(n > 0xffffffffu)
 		return fail(ctx, DMS_ERR_ARG, "dms_initial_conditions: too many events");
 	std::lock_guard<std::mutex> lock(ctx->mu);
+	hipSetDevice(ctx->device); /* callers may be on any host thread */
 	const DmsHaloView H = halo_view(ctx, *halo);
 	dms_event* d_ev = nullptr;
 	int32_t* d_ok = nullptr;
@@ -1344,28 +1439,30 @@ int dms_simulate_events(dms_point* pt, const dms_job_desc* job, const dms_event*
 			return fail(ctx, DMS_ERR_ARG, "dms_simulate_events: initial state is invalid (Trajectory_Simulator::Simulate)");
 	}
 	std::lock_guard<std::mutex> lock(ctx->mu);
+	hipSetDevice(ctx->device); /* callers may be on any host thread */
 	const DmsJobView J = job_view(*job);
 	const bool rows = hist_rows != nullptr && !J.terminate_on_capture;
 	ChunkBuffers B;
-	if(ensure(B, n, J.nbins, rows, true) != hipSuccess)
+	if(ensure(B, n, J.nbins, rows, true, ctx->stream) != hipSuccess)
 	{
-		B.release();
+		B.release(ctx->stream);
 		return fail(ctx, DMS_ERR_NOMEM, "dms_simulate_events: device allocation failed");
 	}
-	hipMemcpy(B.ics, initial, sizeof(dms_event) * n, hipMemcpyHostToDevice);
-	hipMemset(B.ok, 0xff, sizeof(int32_t) * n);
+	hipMemcpyAsync(B.ics, initial, sizeof(dms_event) * n, hipMemcpyHostToDevice, ctx->stream);
+	hipMemsetAsync(B.ok, 0xff, sizeof(int32_t) * n, ctx->stream);
 	float ms = 0.0f;
-	hipError_t e = run_trajectories_single(ctx, pt, J, B, n, rows, true, J.first_id, &ms);
+	hipError_t e = run_trajectories_single(ctx, ctx->stream, ctx->progress_stream, pt, J, B, n, rows, true, J.first_id, &ms);
 	if(e == hipSuccess)
-		e = hipMemcpy(records, B.recs, sizeof(dms_traj_record) * n, hipMemcpyDeviceToHost);
+		e = hipMemcpyAsync(records, B.recs, sizeof(dms_traj_record) * n, hipMemcpyDeviceToHost, ctx->stream);
 	if(e == hipSuccess && hist_rows)
 	{
 		if(rows)
-			e = hipMemcpy(hist_rows, B.rows, sizeof(double) * 2 * (size_t) J.nbins * n, hipMemcpyDeviceToHost);
+			e = hipMemcpyAsync(hist_rows, B.rows, sizeof(double) * 2 * (size_t) J.nbins * n, hipMemcpyDeviceToHost, ctx->stream);
 		else
 			memset(hist_rows, 0, sizeof(double) * 2 * (size_t) J.nbins * n);
 	}
-	B.release();
+	B.release(ctx->stream);
+	hipStreamSynchronize(ctx->stream);
 	if(e != hipSuccess)
 		return fail(ctx, DMS_ERR_DEVICE, std::string("dms_simulate_events: ") + hipGetErrorString(e));
 	return DMS_OK;
@@ -1383,18 +1480,13 @@ std::string check_submit(const dms_job_desc& d)
 	return m;
 }
 
-int submit_status(const dms_job_desc& d)
-{
-	return d.precision == DMS_MIXED ? DMS_ERR_UNSUPPORTED : DMS_ERR_ARG;
-}
-
-void start_batch(dms_point* pt, const dms_job_desc* descs, size_t n, dms_job** out)
+void start_batch(dms_point* const* pts, const dms_job_desc* descs, size_t n, dms_job** out)
 {
 	auto batch = std::make_shared<dms_batch>();
 	for(size_t k = 0; k < n; k++)
 	{
 		dms_job* job = new dms_job();
-		job->pt = pt;
+		job->pt = pts[k];
 		job->desc = descs[k];
 		job->batch = batch;
 		memset(&job->C, 0, sizeof(job->C));
@@ -1423,8 +1515,8 @@ int dms_job_submit(dms_point* pt, const dms_job_desc* desc, dms_job** out)
 	*out = nullptr;
 	const std::string m = check_submit(*desc);
 	if(!m.empty())
-		return fail(pt->ctx, submit_status(*desc), "dms_job_submit: " + m);
-	start_batch(pt, desc, 1, out);
+		return fail(pt->ctx, DMS_ERR_ARG, "dms_job_submit: " + m);
+	start_batch(&pt, desc, 1, out);
 	return DMS_OK;
 }
 
@@ -1439,7 +1531,7 @@ int dms_job_submit_batch(dms_point* pt, const dms_job_desc* descs, size_t n, dms
 	{
 		const std::string m = check_submit(descs[k]);
 		if(!m.empty())
-			return fail(pt->ctx, submit_status(descs[k]), "dms_job_submit_batch: job " + std::to_string(k) + ": " + m);
+			return fail(pt->ctx, DMS_ERR_ARG, "dms_job_submit_batch: job " + std::to_string(k) + ": " + m);
 		if(n > 1 && !batchable(descs[k]))
 			return fail(pt->ctx, DMS_ERR_ARG,
 			            "dms_job_submit_batch: job " + std::to_string(k)
@@ -1454,7 +1546,40 @@ int dms_job_submit_batch(dms_point* pt, const dms_job_desc* descs, size_t n, dms
 	if(n > 1 && total > DMS_BATCH_MAX_SLOTS)
 		return fail(pt->ctx, DMS_ERR_ARG, "dms_job_submit_batch: the batch holds more than "
 		                                      + std::to_string(DMS_BATCH_MAX_SLOTS) + " trajectories");
-	start_batch(pt, descs, n, out);
+	std::vector<dms_point*> pts(n, pt);
+	start_batch(pts.data(), descs, n, out);
+	return DMS_OK;
+}
+
+int dms_job_submit_multi(dms_point* const* points, const dms_job_desc* descs, size_t n, dms_job** out)
+{
+	if(points == nullptr || descs == nullptr || out == nullptr || n == 0 || points[0] == nullptr)
+		return fail(nullptr, DMS_ERR_ARG, "dms_job_submit_multi: null argument or empty batch");
+	dms_ctx* ctx = points[0]->ctx;
+	for(size_t k = 0; k < n; k++)
+		out[k] = nullptr;
+	uint64_t total = 0;
+	for(size_t k = 0; k < n; k++)
+	{
+		if(points[k] == nullptr || points[k]->ctx != ctx)
+			return fail(ctx, DMS_ERR_ARG, "dms_job_submit_multi: job " + std::to_string(k) + ": point is null or of another context");
+		const std::string m = check_submit(descs[k]);
+		if(!m.empty())
+			return fail(ctx, DMS_ERR_ARG, "dms_job_submit_multi: job " + std::to_string(k) + ": " + m);
+		if(!batchable(descs[k]))
+			return fail(ctx, DMS_ERR_ARG,
+			            "dms_job_submit_multi: job " + std::to_string(k)
+			                + ": batched jobs need a fixed trajectory count (max_trajectories > 0, capture_target 0) and at most "
+			                + std::to_string(DMS_BATCH_MAX_SLOTS) + " trajectories and 65536 bins");
+		if(!same_launch_config(descs[0], descs[k]))
+			return fail(ctx, DMS_ERR_ARG,
+			            "dms_job_submit_multi: job " + std::to_string(k)
+			                + " differs from job 0 in mode, precision, limits, r_max or n_bins");
+		total += descs[k].max_trajectories;
+	}
+	if(total > DMS_BATCH_MAX_SLOTS)
+		return fail(ctx, DMS_ERR_ARG, "dms_job_submit_multi: the batch holds more than " + std::to_string(DMS_BATCH_MAX_SLOTS) + " trajectories");
+	start_batch(points, descs, n, out);
 	return DMS_OK;
 }
 
@@ -1547,6 +1672,7 @@ int dms_probe_math(dms_ctx* ctx, int fn, const double* x, const double* y, doubl
 	if(n == 0)
 		return DMS_OK;
 	std::lock_guard<std::mutex> lock(ctx->mu);
+	hipSetDevice(ctx->device); /* callers may be on any host thread */
 	double *dx = nullptr, *dy = nullptr, *dout = nullptr;
 	DMS_HIP(ctx, hipMalloc((void**) &dx, sizeof(double) * n));
 	DMS_HIP(ctx, hipMalloc((void**) &dy, sizeof(double) * n));

@@ -186,6 +186,7 @@ def _declare(lib):
         "dms_destroy": (None, [vp]),
         "dms_last_error": (C.c_char_p, [vp]),
         "dms_load_solar_file": (C.c_int, [C.c_char_p, _P(C.c_double), C.c_size_t, _P(C.c_size_t)]),
+        "dms_set_job_memory_budget": (C.c_int, [vp, C.c_size_t]),
         "dms_solar_eval": (C.c_int, [vp, C.c_int, _P(C.c_double), _P(C.c_double), C.c_size_t]),
         "dms_isotopes": (C.c_int, [vp, _P(C.c_int32), _P(C.c_int32), _P(C.c_double), _P(C.c_double),
                                    C.c_size_t, _P(C.c_size_t)]),
@@ -202,6 +203,7 @@ def _declare(lib):
                                           _P(C.c_double)]),
         "dms_job_submit": (C.c_int, [vp, _P(JobDesc), _P(vp)]),
         "dms_job_submit_batch": (C.c_int, [vp, _P(JobDesc), C.c_size_t, _P(vp)]),
+        "dms_job_submit_multi": (C.c_int, [_P(vp), _P(JobDesc), C.c_size_t, _P(vp)]),
         "dms_job_poll": (C.c_int, [vp, _P(JobStatus)]),
         "dms_job_wait": (C.c_int, [vp, _P(JobStatus)]),
         "dms_job_read": (C.c_int, [vp, _P(Counters), _P(C.c_double), _P(C.c_uint64), C.c_size_t,
@@ -219,10 +221,10 @@ def _declare(lib):
     return lib
 
 
-EXPORTED = ["dms_abi_version", "dms_create", "dms_destroy", "dms_last_error", "dms_load_solar_file",
+EXPORTED = ["dms_abi_version", "dms_create", "dms_destroy", "dms_last_error", "dms_set_job_memory_budget", "dms_load_solar_file",
             "dms_solar_eval", "dms_isotopes", "dms_constant", "dms_point_create", "dms_point_destroy",
             "dms_point_rate_grid", "dms_point_cross_sections", "dms_point_rates", "dms_initial_conditions",
-            "dms_simulate_events", "dms_job_submit", "dms_job_submit_batch", "dms_job_poll", "dms_job_wait", "dms_job_read",
+            "dms_simulate_events", "dms_job_submit", "dms_job_submit_batch", "dms_job_submit_multi", "dms_job_poll", "dms_job_wait", "dms_job_read",
             "dms_job_timing", "dms_job_destroy", "dms_probe_math"]
 
 _lib = None
@@ -284,6 +286,11 @@ class Context:
     def point(self, desc):
         return Point(self, desc)
 
+    def set_job_memory_budget(self, nbytes):
+        """Device bytes one job's trajectory buffers may hold (default 32 GiB);
+        a job larger than that runs as several launches."""
+        self.check(lib().dms_set_job_memory_budget(self.h, nbytes), "dms_set_job_memory_budget")
+
     def probe_math(self, fn, x, y=None):
         x = np.ascontiguousarray(x, dtype=np.float64)
         y = np.ascontiguousarray(y if y is not None else x, dtype=np.float64)
@@ -297,6 +304,24 @@ class Context:
         self.check(lib().dms_initial_conditions(self.h, C.byref(halo), seed, point_id, first_id, n, r_final_rsun,
                                                 ev, ok.ctypes.data_as(_P(C.c_int32))), "dms_initial_conditions")
         return ev, ok
+
+
+def run_multi(points, jobs, want_records=False):
+    """Fixed-count jobs of several points in one trajectory launch
+    (dms_job_submit_multi); points[k] runs jobs[k]."""
+    L = lib()
+    n = len(jobs)
+    assert len(points) == n and n > 0
+    ctx = points[0].ctx
+    descs = (JobDesc * n)(*jobs)
+    hp = (C.c_void_p * n)(*[p.h.value for p in points])
+    hs = (C.c_void_p * n)()
+    ctx.check(L.dms_job_submit_multi(hp, descs, n, hs), "dms_job_submit_multi")
+    try:
+        return [points[k]._collect(C.c_void_p(hs[k]), jobs[k], want_records) for k in range(n)]
+    finally:
+        for k in range(n):
+            L.dms_job_destroy(C.c_void_p(hs[k]))
 
 
 class Point:
@@ -375,6 +400,22 @@ class Point:
             return self._collect(h, job, want_records)
         finally:
             L.dms_job_destroy(h)
+
+    def submit(self, job):
+        """Start one job (dms_job_submit) without waiting: jobs submitted
+        back to back run concurrently on their own streams.  Returns a
+        handle for collect()."""
+        h = C.c_void_p()
+        self.ctx.check(lib().dms_job_submit(self.h, C.byref(job), C.byref(h)), "dms_job_submit")
+        return (h, job)
+
+    def collect(self, handle, want_records=False):
+        """Wait for a submitted job, read its results and release it."""
+        h, job = handle
+        try:
+            return self._collect(h, job, want_records)
+        finally:
+            lib().dms_job_destroy(h)
 
     def run_batch(self, jobs, want_records=False):
         """Fixed-count jobs of this point in one trajectory launch

@@ -318,6 +318,64 @@ class Configuration
 	std::map<std::string, std::vector<std::string>> settings;
 };
 
+/* Serving loop with continuous batching: requests (parameter points with a
+ * captured-particle target) are admitted while the device-memory budget and
+ * the concurrent-job limit allow; each admitted request runs as its own job
+ * on its own streams, so requests that arrive while others are on the GPU
+ * overlap with them.  Parameter points stay resident (rate tables are built
+ * once per distinct point).  Latency = completion - arrival. */
+struct Request
+{
+	DM_Particle DM;
+	Standard_Halo_Model halo;
+	uint64_t sample_size = 100;      /* captured-particle target */
+	uint64_t max_trajectories = 0;   /* cap (0 = none) */
+	uint64_t max_scatterings = 100000000000000ull; /* per trajectory (maximum_number_of_scatterings) */
+	uint64_t rk_attempt_budget = 0;  /* per-trajectory wall-time guard analogue (0 = none) */
+	unsigned int seed = 20260710, point_id = 0;
+	int mode = DMS_MODE_PARAMETER_POINT;
+	int precision = DMS_FP64;
+	int n_bins = 2000;
+	unsigned int interpolation_points = 1000;
+};
+
+struct Request_Result
+{
+	uint64_t id = 0;
+	double arrival_sec = 0.0, start_sec = 0.0, finish_sec = 0.0; /* since the server started */
+	int status = DMS_OK;
+	std::string error;
+	dms_counters counters{};
+	std::vector<double> hist;
+	std::vector<uint64_t> counts;
+	std::vector<dms_evap_event> events;
+	double Latency() const { return finish_sec - arrival_sec; }
+};
+
+class Server
+{
+  public:
+	/* memory_budget: device bytes the running requests may hold together;
+	 * per_job_budget: bytes one request's trajectory buffers may use */
+	Server(Solar_Model& model, size_t memory_budget = (size_t) 128 << 30, size_t per_job_budget = (size_t) 8 << 30,
+	       int max_concurrent = 8);
+	~Server();
+	Server(const Server&) = delete;
+	Server& operator=(const Server&) = delete;
+
+	uint64_t Submit(const Request& r); /* thread-safe; returns the request id */
+	void Wait_All();
+	std::vector<Request_Result> Results() const; /* finished requests, by id */
+	double Elapsed() const;
+
+  private:
+	struct Impl;
+	std::unique_ptr<Impl> impl;
+};
+
+/* p-th percentile (0..100, linear interpolation) */
+double Percentile(std::vector<double> v, double p);
+
 /* libconfig subset: `name = value;` with strings, numbers, booleans and
  * (possibly nested) lists `( ... )` / arrays `[ ... ]`; comments //, #, / * * /. */
 std::map<std::string, std::vector<std::string>> Parse_Config_Text(const std::string& text);

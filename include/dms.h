@@ -237,6 +237,12 @@ int dms_create(dms_ctx** ctx, int device, const double* raw_rows, size_t n_rows)
 void dms_destroy(dms_ctx* ctx);
 const char* dms_last_error(const dms_ctx* ctx);
 
+/* Device memory a capture-target job may use for its trajectory buffers
+ * (chunk size = budget / per-trajectory bytes; default 32 GiB).  Up to 8 jobs
+ * run on the device at once, each on its own streams; a serving loop sizes
+ * this to its admission budget.  No reference counterpart. */
+int dms_set_job_memory_budget(dms_ctx* ctx, size_t bytes);
+
 /* Parse the data rows of a model_agss09.dat-format file (20 header lines). */
 int dms_load_solar_file(const char* path, double* rows_out, size_t capacity_rows, size_t* n_rows);
 
@@ -293,6 +299,16 @@ int dms_job_submit(dms_point* point, const dms_job_desc* desc, dms_job** job);
  * rounding of their summation order).  jobs[k] receives the handle of descs[k];
  * every handle is waited on, read and destroyed separately. */
 int dms_job_submit_batch(dms_point* point, const dms_job_desc* descs, size_t n, dms_job** jobs);
+/* Fixed-count jobs of several parameter points (and seeds, point ids, halos)
+ * in ONE trajectory launch: the batched-uneven workload of many (mass, cross
+ * section) points, whose trajectory lengths differ by orders of magnitude,
+ * sharing one persistent launch with lane refill.  points[k] is the point of
+ * descs[k] (all of one context).  The jobs must agree in mode, precision,
+ * max_scatterings, max_free_steps, rk_attempt_budget, r_max_rsun and n_bins;
+ * limits as dms_job_submit_batch.  Each job's results equal those of
+ * dms_job_submit on its own point and descriptor.  No reference counterpart
+ * (one Generate_Data call per point). */
+int dms_job_submit_multi(dms_point* const* points, const dms_job_desc* descs, size_t n, dms_job** jobs);
 int dms_job_poll(dms_job* job, dms_job_status* status);
 int dms_job_wait(dms_job* job, dms_job_status* status);
 /* hist: 8 x n_bins doubles in the order of bincount.txt columns

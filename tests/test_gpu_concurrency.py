@@ -1,0 +1,130 @@
+"""Many parameter points on one GPU: multi-point launches (dms_job_submit_multi),
+concurrent jobs on their own streams, and the serving loop (dmsun_serve).
+
+Bar: every job's results equal the oracle's Generate_Data run on the same
+descriptor (FP64) and the job's results when it runs alone (both precisions):
+counters, visit counts and evaporation events identical, histogram sums within
+1e-12 relative (device atomics sum in a varying order), per-trajectory records
+bit-identical.
+"""
+import json
+import pathlib
+import subprocess
+
+import numpy as np
+import pytest
+
+import dmsun
+import oracle_lib
+from test_gpu_parity import RK_BUDGET, assert_job_equal, rec_bytes
+
+pytestmark = pytest.mark.gpu
+
+REPO = pathlib.Path(__file__).resolve().parents[1]
+SERVE = REPO / "damascus-earth-evap_amd" / "dmsun_serve"
+
+# (point, seed, point_id, trajectories): SI points whose trajectories differ
+# in length (from a few to thousands of scatterings), a dark-photon point and
+# one without a rate table; sized so the oracle finishes each in about a second
+MULTI = [
+    (dict(mass_GeV=0.3, sigma_p_cm2=1e-36, interpolation_points=1000), 11, 0, 900),
+    (dict(mass_GeV=0.2, sigma_p_cm2=1e-35, interpolation_points=1000), 11, 1, 300),
+    (dict(mass_GeV=0.8, sigma_p_cm2=3e-37, interpolation_points=600), 12, 2, 200),
+    (dict(mass_GeV=0.1, sigma_p_cm2=0.0, sigma_e_cm2=1e-37, interaction=dmsun.DARK_PHOTON, form_factor=dmsun.FF_GENERAL,
+          mediator_mass_MeV=0.01, interpolation_points=300), 13, 3, 20),
+    (dict(mass_GeV=0.5, sigma_p_cm2=1e-37, interpolation_points=0), 14, 4, 60),
+]
+
+
+def multi_jobs(mode, precision=dmsun.FP64, record=1):
+    return [dmsun.make_job(mode=mode, max_trajectories=n, capture_target=0, seed=seed, point_id=pid,
+                           max_scatterings=10**14, n_bins=400, precision=precision, record=record,
+                           rk_attempt_budget=RK_BUDGET) for _, seed, pid, n in MULTI]
+
+
+@pytest.fixture(scope="module")
+def multi_points(gpu_ctx):
+    return [gpu_ctx.point(dmsun.make_point(**d)) for d, *_ in MULTI]
+
+
+@pytest.mark.parametrize("mode", [dmsun.MODE_PARAMETER_POINT, dmsun.MODE_CAPTURE])
+def test_multi_point_launch_matches_oracle_and_alone(gpu_ctx, oracle_solar, multi_points, mode):
+    jobs = multi_jobs(mode)
+    print(f"multi-point launch mode={mode} jobs={[(j.point_id, j.max_trajectories) for j in jobs]}", flush=True)
+    res = dmsun.run_multi(multi_points, jobs, want_records=True)
+    for (d, *_), pt, job, r in zip(MULTI, multi_points, jobs, res):
+        ores = oracle_lib.Point(oracle_solar, dmsun.make_point(**d)).run_job(job, record_capacity=job.max_trajectories)
+        assert_job_equal(r, ores)
+        assert np.array_equal(rec_bytes(r["records"]), rec_bytes(ores["records"])), d
+        assert_job_equal(pt.run_job(job, want_records=False), r)
+
+
+def test_multi_point_mixed_equals_alone(multi_points):
+    jobs = multi_jobs(dmsun.MODE_PARAMETER_POINT, precision=dmsun.MIXED)
+    print("multi-point launch, mixed precision", flush=True)
+    res = dmsun.run_multi(multi_points, jobs, want_records=True)
+    for pt, job, r in zip(multi_points, jobs, res):
+        alone = pt.run_job(job, want_records=True)
+        assert_job_equal(alone, r)
+        assert np.array_equal(rec_bytes(alone["records"]), rec_bytes(r["records"]))
+
+
+def test_multi_rejects_disagreeing_jobs(multi_points):
+    a, b = multi_jobs(dmsun.MODE_PARAMETER_POINT)[:2]
+    b.n_bins = 300
+    with pytest.raises(dmsun.DmsError):
+        dmsun.run_multi(multi_points[:2], [a, b])
+    c = multi_jobs(dmsun.MODE_PARAMETER_POINT, precision=dmsun.MIXED)[1]
+    with pytest.raises(dmsun.DmsError):
+        dmsun.run_multi(multi_points[:2], [a, c])
+
+
+def test_concurrent_jobs_equal_alone(multi_points):
+    """Jobs submitted back to back (each on its own streams, capture-target
+    jobs with their synchronisation rounds among them) give the results each
+    gives alone."""
+    jobs = [dmsun.make_job(mode=dmsun.MODE_PARAMETER_POINT, max_trajectories=20000, capture_target=25, seed=21,
+                           point_id=k, max_scatterings=10**14, n_bins=400, record=0, rk_attempt_budget=RK_BUDGET)
+            for k in range(3)]
+    jobs += multi_jobs(dmsun.MODE_CAPTURE, record=0)[:3]
+    pts = multi_points[:3] + multi_points[:3]
+    print(f"concurrent jobs: {len(jobs)}", flush=True)
+    handles = [pt.submit(j) for pt, j in zip(pts, jobs)]
+    together = [pt.collect(h) for pt, h in zip(pts, handles)]
+    for pt, job, r in zip(pts, jobs, together):
+        assert_job_equal(pt.run_job(job, want_records=False), r)
+
+
+def test_job_memory_budget_chunks_give_same_results(gpu_ctx, multi_points):
+    """A small per-job memory budget splits a capture-target job (whose
+    trajectories keep per-trajectory rows until their round is decided) into
+    more launches; the results stay those of the unsplit job."""
+    pt = multi_points[0]
+    # 2000 bins: about 40 KB of rows per trajectory, so 64 MiB holds ~1600
+    job = dmsun.make_job(mode=dmsun.MODE_PARAMETER_POINT, max_trajectories=0, capture_target=400, seed=5,
+                         max_scatterings=10**14, n_bins=2000, record=0, rk_attempt_budget=RK_BUDGET)
+    full = pt.run_job(job, want_records=False)
+    gpu_ctx.set_job_memory_budget(64 << 20)
+    try:
+        small = pt.run_job(job, want_records=False)
+    finally:
+        gpu_ctx.set_job_memory_budget(32 << 30)
+    assert_job_equal(small, full)
+    assert small["launches"] > full["launches"]
+    with pytest.raises(dmsun.DmsError):
+        gpu_ctx.set_job_memory_budget(1 << 20)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_serving_results_same_alone_and_in_the_mix(tmp_path, precision):
+    out = tmp_path / "serve.json"
+    cmd = [str(SERVE), "--requests", "10", "--rate", "40", "--sample-size", "3", "--sample-size-max", "8",
+           "--max-trajectories", "4000", "--max-scatterings", "3000", "--interp", "300", "--bins", "200",
+           "--precision", precision, "--check-alone", "10", "--json", str(out)]
+    print(" ".join(cmd), flush=True)
+    p = subprocess.run(cmd, cwd=str(REPO), capture_output=True, text=True, timeout=110)
+    assert p.returncode == 0, p.stdout + p.stderr
+    s = json.loads(out.read_text())
+    assert s["failed"] == 0 and s["alone_mismatches"] == 0 and s["checked_alone"] == 10
+    assert 0 < s["latency_p50_sec"] <= s["latency_p99_sec"] <= s["latency_max_sec"]
+    assert s["trajectories"] > 0
