@@ -1745,16 +1745,19 @@ DMS_HD void dms_shift_failure_record(uint64_t id, const dms_event& ic, dms_traj_
 }
 
 
-/* Per-lane mode: add the finished trajectory's bins to the sums of its class
- * (1 captured, 2 completed outward escape, 0 none) and clear the scratch row. */
-DMS_HD void dms_flush_lane_rows(DmsLane& L, const DmsRows& R, int nbins, int cls)
+/* Per-lane mode: add the touched bins k = first, first + stride, ... < n of
+ * scratch row `row` (a finished trajectory of job `job`) to the sums of its
+ * class (1 captured, 2 completed outward escape, 0 none) and clear them.  One
+ * lane walks the whole list (stride 1), or the 64 lanes of a wave share it. */
+DMS_HD void dms_flush_rows(const DmsRows& R, int nbins, uint32_t row_index, uint32_t job, uint32_t n, int cls, uint32_t first,
+                           uint32_t stride)
 {
-	double* row = R.rows + (size_t) L.row * 2 * (size_t) nbins;
-	uint32_t* cnt = R.counts + (size_t) L.row * (size_t) nbins;
-	const uint16_t* list = R.touched + (size_t) L.row * (size_t) nbins;
-	double* h = R.hist + (size_t) L.job * 8 * (size_t) nbins + (cls == 2 ? 4 * (size_t) nbins : 0);
-	unsigned long long* hc = R.hcounts + (size_t) L.job * 2 * (size_t) nbins + (cls == 2 ? (size_t) nbins : 0);
-	for(uint32_t k = 0; k < L.n_touched; k++)
+	double* row = R.rows + (size_t) row_index * 2 * (size_t) nbins;
+	uint32_t* cnt = R.counts + (size_t) row_index * (size_t) nbins;
+	const uint16_t* list = R.touched + (size_t) row_index * (size_t) nbins;
+	double* h = R.hist + (size_t) job * 8 * (size_t) nbins + (cls == 2 ? 4 * (size_t) nbins : 0);
+	unsigned long long* hc = R.hcounts + (size_t) job * 2 * (size_t) nbins + (cls == 2 ? (size_t) nbins : 0);
+	for(uint32_t k = first; k < n; k += stride)
 	{
 		const int b = list[k];
 		const double dt = row[b];
@@ -1780,6 +1783,11 @@ DMS_HD void dms_flush_lane_rows(DmsLane& L, const DmsRows& R, int nbins, int cls
 		row[nbins + b] = 0.0;
 		cnt[b] = 0u;
 	}
+}
+
+DMS_HD void dms_flush_lane_rows(DmsLane& L, const DmsRows& R, int nbins, int cls)
+{
+	dms_flush_rows(R, nbins, L.row, L.job, L.n_touched, cls, 0u, 1u);
 	L.n_touched = 0;
 }
 
@@ -1839,7 +1847,9 @@ DMS_HD void dms_lane_finish_scatter(DmsLane& L, const DmsLaneCtx& c)
 		L.phase = dms_after_scatter(L, c.S, c.P.mass, c.J);
 }
 
-DMS_HD void dms_lane_finish(DmsLane& L, const DmsLaneCtx& c, const dms_event& ic, dms_traj_record* rec, DmsSummary* sum)
+/* Trajectory end up to the per-lane row flush.  Returns the class of the
+ * scratch row still to be flushed (dms_flush_rows), or -1 when there is none. */
+DMS_HD int dms_lane_finish_rows_pending(DmsLane& L, const DmsLaneCtx& c, const dms_event& ic, dms_traj_record* rec, DmsSummary* sum)
 {
 	dms_bin_flush(L, c.R, c.J.nbins);
 	uint32_t flags = 0;
@@ -1851,9 +1861,17 @@ DMS_HD void dms_lane_finish(DmsLane& L, const DmsLaneCtx& c, const dms_event& ic
 		flags = s->flags;
 		reason = s->reason;
 	}
-	if(c.R.per_lane && L.n_touched > 0)
-		dms_flush_lane_rows(L, c.R, c.J.nbins, (flags & DMS_F_CAPTURED) ? 1 : ((reason == DMS_OUTWARD_ESCAPE) ? 2 : 0));
 	L.phase = PH_NEED;
+	if(c.R.per_lane && L.n_touched > 0)
+		return (flags & DMS_F_CAPTURED) ? 1 : ((reason == DMS_OUTWARD_ESCAPE) ? 2 : 0);
+	return -1;
+}
+
+DMS_HD void dms_lane_finish(DmsLane& L, const DmsLaneCtx& c, const dms_event& ic, dms_traj_record* rec, DmsSummary* sum)
+{
+	const int cls = dms_lane_finish_rows_pending(L, c, ic, rec, sum);
+	if(cls >= 0)
+		dms_flush_lane_rows(L, c.R, c.J.nbins, cls);
 }
 
 DMS_HD void dms_lane_start(DmsLane& L, const DmsLaneCtx& c, const dms_event& ic, bool shift_ok, dms_traj_record* rec, DmsSummary* sum)

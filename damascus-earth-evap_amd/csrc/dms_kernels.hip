@@ -156,11 +156,33 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 				L.sc_target = target;
 		}
 		dms_lane_finish_scatter<MX>(L, c);
+		int flush_cls = -1;
 		if(L.phase == PH_FIN)
 		{
-			dms_lane_finish(L, c, a.ics[L.slot], a.records ? a.records + L.slot : nullptr, a.summ ? a.summ + L.slot : nullptr);
+			flush_cls = dms_lane_finish_rows_pending(L, c, a.ics[L.slot], a.records ? a.records + L.slot : nullptr,
+			                                         a.summ ? a.summ + L.slot : nullptr);
 			if(a.job_done)
 				atomicAdd(a.job_done + L.job, 1u);
+		}
+		/* per-lane histogram rows of the finished trajectories: the wave adds
+		 * each row's touched bins to its job's sums, 64 bins at a time */
+		unsigned long long fm = __ballot(flush_cls >= 0);
+		if(fm)
+		{
+			while(fm)
+			{
+				const int s = __ffsll((long long) fm) - 1;
+				fm &= fm - 1;
+				const uint32_t row_s = (uint32_t) __shfl((int) L.row, s);
+				const uint32_t job_s = (uint32_t) __shfl((int) L.job, s);
+				const uint32_t n_s = (uint32_t) __shfl((int) L.n_touched, s);
+				const int cls_s = __shfl(flush_cls, s);
+				dms_flush_rows(c.R, c.J.nbins, row_s, job_s, n_s, cls_s, (uint32_t) lane, 64u);
+			}
+			if(flush_cls >= 0)
+				L.n_touched = 0;
+			/* the cleared rows are reused by their lanes' next trajectories */
+			__threadfence_block();
 		}
 		/* lane refill: one atomic per wave */
 		const unsigned long long nmask = __ballot(L.phase == PH_NEED);
