@@ -25,6 +25,12 @@ HIP events on the kernel's stream.
 
 cpu_baseline: the CPU restatement (oracle/liboracle.so, one trajectory stream
 per host thread) on a bounded sample of the same workload.
+
+Beside the headline line: "mixed" (the same steps on the mixed-precision
+kernel), "batched_uneven" (65 parameter points of very different trajectory
+lengths in one autotuned launch) and "serving" (a Poisson request stream
+through the serving loop, p50/p99 latency).  --no-mixed, --no-uneven,
+--no-serving and --no-cpu leave them out.
 """
 import argparse
 import json
@@ -37,7 +43,7 @@ ROOT = pathlib.Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "damascus-earth-evap_amd"))
 
 # Algorithmic FP64 operations per event (hand count of the reference formulas:
-# +, -, *, /, sqrt = 1; each transcendental = 1).  See DESIGN.md, "Algorithmic work".
+# +, -, *, /, sqrt = 1; each transcendental = 1).
 FLOP_WEIGHTS = {
     "rk_attempt": 266.0,        # one RKF45 attempt: 6 stages, 6 Mass() splines, step control
     "step_interior": 412.0,     # step prep + optical depth with 4 pieces (Parameter point)
@@ -95,7 +101,9 @@ def measure(dmsun, pt, precision, args):
         res = pt.run_job(job, want_records=False)
         dt = time.perf_counter() - t
         single = dict(trajectories_per_s=res["counters"].trajectories / dt, seconds=dt, kernel_ms=res["kernel_ms"],
-                      trajectories=res["counters"].trajectories)
+                      trajectories=res["counters"].trajectories,
+                      longest_trajectory_rk_attempts=res["counters"].max_rk_attempts,
+                      us_per_attempt_of_longest=1e3 * res["kernel_ms"] / max(1, res["counters"].max_rk_attempts))
         print(f"{name} warmup {w} (one job alone): {dt:.2f} s, kernel {res['kernel_ms']:.1f} ms", file=sys.stderr,
               flush=True)
     jobs = [make_workload(dmsun, args.trajectories, k * args.trajectories, precision)[1] for k in range(args.steps)]
@@ -103,7 +111,7 @@ def measure(dmsun, pt, precision, args):
     results = pt.run_batch(jobs, want_records=False)
     wall = time.perf_counter() - t0
     out = dict(traj=0, scat=0, cap=0, flops=0.0, wall=wall, single=single, kern_ms=results[0]["kernel_ms"],
-               launches=results[0]["launches"])
+               launches=results[0]["launches"], max_rk=max(r["counters"].max_rk_attempts for r in results))
     for k, res in enumerate(results):
         c = res["counters"]
         out["traj"] += c.trajectories
@@ -117,6 +125,95 @@ def measure(dmsun, pt, precision, args):
     return out
 
 
+def uneven_points(dmsun, ctx):
+    """64 SI points, m log-spaced over [0.1, 5] GeV x sigma_p log-spaced over
+    [1e-39, 1e-35] cm^2, plus a dark-photon point (m = 0.1 GeV, sigma_e =
+    1e-37 cm^2, light mediator); 1000 x 1000 rate tables built on the GPU."""
+    import numpy as np
+    masses = np.logspace(np.log10(0.1), np.log10(5.0), 8)
+    sigmas = np.logspace(-39.0, -35.0, 8)
+    descs = [dmsun.make_point(float(m), float(s), interpolation_points=1000) for m in masses for s in sigmas]
+    descs.append(dmsun.make_point(0.1, 0.0, sigma_e_cm2=1e-37, interaction=dmsun.DARK_PHOTON,
+                                  form_factor=dmsun.FF_GENERAL, mediator_mass_MeV=0.01, interpolation_points=1000))
+    t = time.perf_counter()
+    pts = [ctx.point(d) for d in descs]
+    return pts, time.perf_counter() - t
+
+
+def measure_uneven(dmsun, ctx, args):
+    """Batched uneven workload: the 65 points' jobs in one persistent launch
+    with lane refill, job order from the calibration autotuner."""
+    pts, t_tables = uneven_points(dmsun, ctx)
+    out = {"workload": f"65 parameter points (64 SI m=0.1-5 GeV x sigma_p=1e-39-1e-35 cm^2, 1 dark photon), "
+                       f"{args.uneven_trajectories} trajectories each, at most 1e5 scatterings and 1e5 RKF45 attempts "
+                       f"(wall-time guard analogue) per trajectory, 1000 bins, one launch",
+           "points": len(pts), "rate_tables_s": t_tables}
+    for name, prec in (("fp64", dmsun.FP64), ("mixed", dmsun.MIXED)):
+        if name == "mixed" and args.no_mixed:
+            continue
+        jobs = [dmsun.make_job(mode=dmsun.MODE_PARAMETER_POINT, max_trajectories=args.uneven_trajectories,
+                               capture_target=0, seed=20260710, point_id=k, max_scatterings=100_000,
+                               rk_attempt_budget=100_000, n_bins=1000, record=0, precision=prec,
+                               halo=dmsun.default_halo((0.0, 248.0, 0.0))) for k in range(len(pts))]
+        t = time.perf_counter()
+        order, cost = dmsun.autotune_multi(pts, jobs)
+        t_cal = time.perf_counter() - t
+        t = time.perf_counter()
+        res = dmsun.run_multi([pts[k] for k in order], [jobs[k] for k in order])
+        wall = time.perf_counter() - t
+        traj = sum(r["counters"].trajectories for r in res)
+        scat = sum(r["counters"].total_scatterings for r in res)
+        trunc = sum(r["counters"].computational_truncations for r in res)
+        leg = {"trajectories_per_s": traj / wall, "scatterings_per_s": scat / wall, "seconds": wall,
+               "calibration_s": t_cal, "trajectories": traj, "scatterings": scat, "truncated": trunc,
+               "kernel_ms": res[0]["kernel_ms"], "cost_per_trajectory_range": [float(cost.min()), float(cost.max())]}
+        if name == "fp64":
+            t = time.perf_counter()
+            dmsun.run_multi(pts, jobs)
+            leg["given_order_seconds"] = time.perf_counter() - t
+        out[name] = leg
+        print(f"uneven {name}: {traj} trajectories, {scat} scatterings in {wall:.2f} s (calibration {t_cal:.2f} s)",
+              file=sys.stderr, flush=True)
+    for p in pts:
+        p.close()
+    return out
+
+
+def measure_serving(args):
+    """Serving loop (dmsun_serve): Poisson parameter-point requests admitted by
+    the device-memory budget, p50/p99 latency and trajectories/s."""
+    import subprocess
+    exe = ROOT / "damascus-earth-evap_amd" / "dmsun_serve"
+    cmd = [str(exe), "--requests", str(args.serve_requests), "--rate", str(args.serve_rate),
+           "--sample-size", "1000", "--sample-size-max", "10000", "--max-trajectories", "200000",
+           "--max-scatterings", "100000", "--rk-budget", "100000", "--interp", "1000", "--bins", "1000", "--quiet"]
+    print("serving: " + " ".join(cmd[1:]), file=sys.stderr, flush=True)
+    p = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+    if p.returncode != 0:
+        raise RuntimeError("dmsun_serve failed: " + p.stderr[-2000:])
+    s = json.loads(p.stdout.strip().splitlines()[-1])
+    s["workload"] = ("Poisson requests, SI m~U_log(0.1, 5) GeV, sigma_p~U_log(1e-38, 1e-35) cm^2, "
+                     "parameter-point mode with a captured-particle target ~U_log(1e3, 1e4), at most 1e5 scatterings "
+                     "and 1e5 RKF45 attempts per trajectory, 1000x1000 rate tables resident")
+    return s
+
+
+def measure_bulk(dmsun, pt, precision):
+    """The same 10^6 trajectories with every trajectory cut at 2000 RKF45
+    attempts: no long tail, all lanes busy for the whole launch.  Shows the
+    kernel's throughput when it is not waiting on a few long trajectories."""
+    _, job = make_workload(dmsun, 1_000_000, 0, precision)
+    job.rk_attempt_budget = 2000
+    res = pt.run_job(job, want_records=False)
+    c = res["counters"]
+    s = res["kernel_ms"] * 1e-3
+    fl = algorithmic_flops(c, False)
+    return {"workload": "the timed workload's first 1e6 trajectories, each cut at 2000 RKF45 attempts",
+            "kernel_ms": res["kernel_ms"], "rk_attempts_per_s": c.n_rk_attempts / s,
+            "achieved_tflops": fl / s / 1e12, "frac": fl / s / 1e12 / PEAK_FP64_TFLOPS,
+            "truncated": c.computational_truncations}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -126,10 +223,15 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=100_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-uneven", action="store_true")
+    ap.add_argument("--uneven-trajectories", type=int, default=4000)
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=16)
+    ap.add_argument("--serve-rate", type=float, default=0.5)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(trajectories are independent: see DESIGN.md for how the path would shard).", file=sys.stderr)
+              "(trajectories are independent: several GPUs would each take a disjoint trajectory-id range of the same keys).", file=sys.stderr)
         sys.exit(2)
 
     import dmsun
@@ -184,6 +286,21 @@ def main():
             "precision": "FP32 rate table, optical depth (Kahan), target sampling, kinematics; "
                          "FP64 RKF45, energy test, time sums",
         }
+    out["tail"] = {
+        "note": "launch time of a parameter-point job is bounded by its longest trajectory, whose RKF45 attempts are "
+                "sequential; us_per_attempt_of_longest = launch time / that trajectory's attempts",
+        "longest_trajectory_rk_attempts": fp64["max_rk"],
+        "us_per_attempt_of_longest": 1e3 * kern_ms / max(1, fp64["max_rk"]),
+        "mixed_longest_trajectory_rk_attempts": mixed["max_rk"] if mixed else None,
+        "mixed_us_per_attempt_of_longest": 1e3 * mixed["kern_ms"] / max(1, mixed["max_rk"]) if mixed else None,
+    }
+    out["bulk"] = {"fp64": measure_bulk(dmsun, pt, dmsun.FP64)}
+    if mixed is not None:
+        out["bulk"]["mixed"] = measure_bulk(dmsun, pt, dmsun.MIXED)
+    if not args.no_uneven:
+        out["batched_uneven"] = measure_uneven(dmsun, ctx, args)
+    if not args.no_serving:
+        out["serving"] = measure_serving(args)
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(args.cpu_sample, threads)

@@ -10,6 +10,7 @@
  * of the trajectories that count.  Results therefore do not depend on launch
  * geometry or chunking.
  */
+#include <algorithm>
 #include <hip/hip_runtime.h>
 
 #include <atomic>
@@ -548,6 +549,8 @@ int account_trajectory(dms_job* job, RoundReplay& R, const DmsSummary& s, double
 	C.n_steps_exterior += s.n_ext;
 	C.n_tau_rejections += s.n_rej;
 	C.n_tv_iterations += s.n_tv;
+	if(s.n_rk > C.max_rk_attempts)
+		C.max_rk_attempts = s.n_rk;
 	if(shift_ok && is_numerical(s.reason))
 		C.numerical_failures++;
 	if(is_truncation(s.reason))
@@ -1580,6 +1583,61 @@ int dms_job_submit_multi(dms_point* const* points, const dms_job_desc* descs, si
 	if(total > DMS_BATCH_MAX_SLOTS)
 		return fail(ctx, DMS_ERR_ARG, "dms_job_submit_multi: the batch holds more than " + std::to_string(DMS_BATCH_MAX_SLOTS) + " trajectories");
 	start_batch(points, descs, n, out);
+	return DMS_OK;
+}
+
+int dms_multi_autotune(dms_point* const* points, const dms_job_desc* descs, size_t n, uint64_t calibration_trajectories,
+                       uint64_t calibration_rk_attempts, uint32_t* order, double* cost_per_trajectory)
+{
+	if(points == nullptr || descs == nullptr || order == nullptr || n == 0 || points[0] == nullptr)
+		return fail(nullptr, DMS_ERR_ARG, "dms_multi_autotune: null argument or empty batch");
+	dms_ctx* ctx = points[0]->ctx;
+	if(calibration_trajectories == 0 || calibration_trajectories * n > DMS_BATCH_MAX_SLOTS || calibration_rk_attempts == 0)
+		return fail(ctx, DMS_ERR_ARG, "dms_multi_autotune: calibration size must be positive and fit one launch");
+	/* the calibration jobs: the same points, keys and limits, trajectory ids
+	 * past each job's own range, an RKF45-attempt cap that bounds the launch */
+	std::vector<dms_job_desc> cal(descs, descs + n);
+	for(size_t k = 0; k < n; k++)
+	{
+		cal[k].max_trajectories = calibration_trajectories;
+		cal[k].capture_target = 0;
+		cal[k].record_trajectories = 0;
+		cal[k].first_trajectory_id = descs[k].first_trajectory_id + descs[k].max_trajectories;
+		if(cal[k].rk_attempt_budget == 0 || cal[k].rk_attempt_budget > calibration_rk_attempts)
+			cal[k].rk_attempt_budget = calibration_rk_attempts;
+	}
+	std::vector<dms_job*> jobs(n, nullptr);
+	const int st = dms_job_submit_multi(points, cal.data(), n, jobs.data());
+	if(st != DMS_OK)
+		return st;
+	std::vector<double> cost(n, 0.0);
+	int status = DMS_OK;
+	for(size_t k = 0; k < n; k++)
+	{
+		if(dms_job_wait(jobs[k], nullptr) != DMS_OK)
+			status = DMS_ERR_DEVICE;
+		else
+		{
+			const dms_counters& c = jobs[k]->C;
+			/* lane iterations: one per RKF45 attempt, about four per scattering */
+			cost[k] = c.trajectories ? ((double) c.n_rk_attempts + 4.0 * (double) c.total_scatterings) / (double) c.trajectories : 0.0;
+		}
+		dms_job_destroy(jobs[k]);
+	}
+	if(status != DMS_OK)
+		return fail(ctx, status, "dms_multi_autotune: calibration launch failed");
+	/* cheapest first: the long trajectories of the costly points start last and
+	 * finish in waves the short ones have left, instead of sharing full waves */
+	std::vector<uint32_t> idx(n);
+	for(size_t k = 0; k < n; k++)
+		idx[k] = (uint32_t) k;
+	std::stable_sort(idx.begin(), idx.end(), [&](uint32_t x, uint32_t y) { return cost[x] < cost[y]; });
+	for(size_t k = 0; k < n; k++)
+	{
+		order[k] = idx[k];
+		if(cost_per_trajectory)
+			cost_per_trajectory[k] = cost[k];
+	}
 	return DMS_OK;
 }
 

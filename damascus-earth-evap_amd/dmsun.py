@@ -92,7 +92,7 @@ class Counters(C.Structure):
                 ("reason_counts", C.c_uint64 * 11), ("n_rk_attempts", C.c_uint64),
                 ("n_steps_interior", C.c_uint64), ("n_steps_exterior", C.c_uint64),
                 ("n_tau_rejections", C.c_uint64), ("n_tv_iterations", C.c_uint64),
-                ("simulated_trajectories", C.c_uint64)]
+                ("simulated_trajectories", C.c_uint64), ("max_rk_attempts", C.c_uint64)]
 
 
 class EvapEvent(C.Structure):
@@ -204,6 +204,8 @@ def _declare(lib):
         "dms_job_submit": (C.c_int, [vp, _P(JobDesc), _P(vp)]),
         "dms_job_submit_batch": (C.c_int, [vp, _P(JobDesc), C.c_size_t, _P(vp)]),
         "dms_job_submit_multi": (C.c_int, [_P(vp), _P(JobDesc), C.c_size_t, _P(vp)]),
+        "dms_multi_autotune": (C.c_int, [_P(vp), _P(JobDesc), C.c_size_t, C.c_uint64, C.c_uint64, _P(C.c_uint32),
+                                         _P(C.c_double)]),
         "dms_job_poll": (C.c_int, [vp, _P(JobStatus)]),
         "dms_job_wait": (C.c_int, [vp, _P(JobStatus)]),
         "dms_job_read": (C.c_int, [vp, _P(Counters), _P(C.c_double), _P(C.c_uint64), C.c_size_t,
@@ -224,7 +226,7 @@ def _declare(lib):
 EXPORTED = ["dms_abi_version", "dms_create", "dms_destroy", "dms_last_error", "dms_set_job_memory_budget", "dms_load_solar_file",
             "dms_solar_eval", "dms_isotopes", "dms_constant", "dms_point_create", "dms_point_destroy",
             "dms_point_rate_grid", "dms_point_cross_sections", "dms_point_rates", "dms_initial_conditions",
-            "dms_simulate_events", "dms_job_submit", "dms_job_submit_batch", "dms_job_submit_multi", "dms_job_poll", "dms_job_wait", "dms_job_read",
+            "dms_simulate_events", "dms_job_submit", "dms_job_submit_batch", "dms_job_submit_multi", "dms_multi_autotune", "dms_job_poll", "dms_job_wait", "dms_job_read",
             "dms_job_timing", "dms_job_destroy", "dms_probe_math"]
 
 _lib = None
@@ -322,6 +324,31 @@ def run_multi(points, jobs, want_records=False):
     finally:
         for k in range(n):
             L.dms_job_destroy(C.c_void_p(hs[k]))
+
+
+def autotune_multi(points, jobs, calibration_trajectories=128, calibration_rk_attempts=5000):
+    """Job order for a multi-point launch (dms_multi_autotune): returns
+    (order, cost_per_trajectory); run_multi([points[k] for k in order], ...)."""
+    L = lib()
+    n = len(jobs)
+    descs = (JobDesc * n)(*jobs)
+    hp = (C.c_void_p * n)(*[p.h.value for p in points])
+    order = (C.c_uint32 * n)()
+    cost = np.zeros(n, dtype=np.float64)
+    points[0].ctx.check(L.dms_multi_autotune(hp, descs, n, calibration_trajectories, calibration_rk_attempts, order, dptr(cost)),
+                        "dms_multi_autotune")
+    return list(order), cost
+
+
+def run_multi_tuned(points, jobs, want_records=False, **calibration):
+    """run_multi in the autotuned job order; results come back in the
+    caller's order."""
+    order, _ = autotune_multi(points, jobs, **calibration)
+    res = run_multi([points[k] for k in order], [jobs[k] for k in order], want_records)
+    out = [None] * len(jobs)
+    for pos, k in enumerate(order):
+        out[k] = res[pos]
+    return out
 
 
 class Point:

@@ -200,6 +200,7 @@ typedef struct
 	uint64_t reason_counts[11];   /* trajectories per dms_reason */
 	uint64_t n_rk_attempts, n_steps_interior, n_steps_exterior, n_tau_rejections, n_tv_iterations;
 	uint64_t simulated_trajectories; /* including speculative ones past the stop point */
+	uint64_t max_rk_attempts;        /* RKF45 attempts of the job's longest trajectory */
 } dms_counters;
 
 /* CompactEvaporationEvent, include/Data_Generation.hpp:48-54 */
@@ -309,6 +310,15 @@ int dms_job_submit_batch(dms_point* point, const dms_job_desc* descs, size_t n, 
  * dms_job_submit on its own point and descriptor.  No reference counterpart
  * (one Generate_Data call per point). */
 int dms_job_submit_multi(dms_point* const* points, const dms_job_desc* descs, size_t n, dms_job** jobs);
+/* Autotuner of a multi-point launch: runs calibration_trajectories
+ * trajectories of every job in one launch (ids past each job's own range, at
+ * most calibration_rk_attempts RKF45 attempts each), measures the lane
+ * iterations each job's trajectories need, and returns in order[0..n) the job
+ * order to pass to dms_job_submit_multi (cheapest first) and, optionally, the
+ * measured cost per trajectory of each job.  The order changes the launch time,
+ * never a job's results.  No reference counterpart. */
+int dms_multi_autotune(dms_point* const* points, const dms_job_desc* descs, size_t n, uint64_t calibration_trajectories,
+                       uint64_t calibration_rk_attempts, uint32_t* order, double* cost_per_trajectory);
 int dms_job_poll(dms_job* job, dms_job_status* status);
 int dms_job_wait(dms_job* job, dms_job_status* status);
 /* hist: 8 x n_bins doubles in the order of bincount.txt columns

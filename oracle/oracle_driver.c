@@ -275,6 +275,8 @@ int or_run_job(const or_point* P, const dms_job_desc* job, int n_threads, dms_co
 			C->n_steps_exterior += R->n_steps_exterior;
 			C->n_tau_rejections += R->n_tau_rejections;
 			C->n_tv_iterations += R->n_tv_iterations;
+			if(R->n_rk_attempts > C->max_rk_attempts)
+				C->max_rk_attempts = R->n_rk_attempts;
 			const int completed_escape = R->reason == DMS_OUTWARD_ESCAPE;
 			if(shift_ok && is_numerical(R->reason))
 				C->numerical_failures++;

@@ -128,3 +128,17 @@ def test_serving_results_same_alone_and_in_the_mix(tmp_path, precision):
     assert s["failed"] == 0 and s["alone_mismatches"] == 0 and s["checked_alone"] == 10
     assert 0 < s["latency_p50_sec"] <= s["latency_p99_sec"] <= s["latency_max_sec"]
     assert s["trajectories"] > 0
+
+
+def test_autotuned_order_keeps_results(multi_points):
+    """The autotuner's calibration orders the jobs by measured cost; the
+    launch in that order gives every job the results of the given order."""
+    jobs = multi_jobs(dmsun.MODE_PARAMETER_POINT, record=0)
+    order, cost = dmsun.autotune_multi(multi_points, jobs, calibration_trajectories=64, calibration_rk_attempts=3000)
+    print(f"autotuned order {order}, cost per trajectory {np.round(cost, 1).tolist()}", flush=True)
+    assert sorted(order) == list(range(len(jobs)))
+    assert np.all(cost > 0) and all(cost[order[i]] <= cost[order[i + 1]] for i in range(len(order) - 1))
+    tuned = dmsun.run_multi_tuned(multi_points, jobs, calibration_trajectories=64, calibration_rk_attempts=3000)
+    plain = dmsun.run_multi(multi_points, jobs)
+    for a, b in zip(tuned, plain):
+        assert_job_equal(a, b)
