@@ -53,6 +53,10 @@ FLOP_WEIGHTS = {
     "trajectory": 150.0,        # trajectory start/finish bookkeeping
 }
 PEAK_FP64_TFLOPS = 78.6
+# HBM bytes of the timed launch of the default workload (4 jobs of 10^6 in one
+# launch): rocprofv3 FETCH_SIZE and WRITE_SIZE passes, scaled by a plain 256 MiB
+# copy (read x2.0, write x1.0); profiles/dms_traj_kernel_parameter_point_m0.3_4x1e6.md.
+PROFILED_TRAFFIC_BYTES = {(4, 1_000_000): 277.1e9 + 160.7e9}
 
 
 def algorithmic_flops(c, capture_mode):
@@ -228,7 +232,18 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=16)
     ap.add_argument("--serve-rate", type=float, default=0.5)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run one plain device-to-device copy of BYTES bytes (scale of the HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_probe > 0:
+        import torch
+        src = torch.ones(args.copy_probe, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": args.copy_probe}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(trajectories are independent: several GPUs would each take a disjoint trajectory-id range of the same keys).", file=sys.stderr)
@@ -268,7 +283,9 @@ def main():
         "capture_fraction": tot_cap / max(1, tot_traj),
         "kernel_ms_per_step": kern_ms / args.steps,
         "roofline": {"bound": "valu-fp64", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP64_TFLOPS,
+                     "traffic": PROFILED_TRAFFIC_BYTES.get((args.steps, args.trajectories)),
+                     "traffic_unit": "bytes per launch (rocprofv3 counters of this workload, not measured live)",
                      "kernel": "dms_traj_kernel", "avg_launch_ms": avg_launch_s * 1e3},
     }
     if mixed is not None:

@@ -111,6 +111,8 @@ def record_array(n):
 
 def records_to_numpy(recs):
     """Structured numpy view (copy) of a ctypes TrajRecord array."""
+    if isinstance(recs, list):
+        recs = (TrajRecord * len(recs))(*recs)
     return np.ctypeslib.as_array(recs).copy()
 
 
@@ -415,7 +417,7 @@ class Point:
         ot = C.c_double(0)
         L.dms_job_timing(h, C.byref(kt), C.byref(kl), C.byref(ot))
         return dict(counters=cnt, hist=hist, counts=counts, events=list(evs)[:n_ev.value],
-                    records=recs[:n_rec.value] if want_records else None,
+                    records=(recs if n_rec.value == len(recs) else recs[:n_rec.value]) if want_records else None,
                     kernel_ms=kt.value, launches=kl.value, other_ms=ot.value)
 
     def run_job(self, job, want_records=True):
