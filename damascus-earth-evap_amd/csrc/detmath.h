@@ -51,6 +51,48 @@ DM_FN double dm_fabs(double x) { return dm_from_bits(dm_bits(x) & 0x7fffffffffff
 DM_FN double dm_min(double a, double b) { return (b < a) ? b : a; }
 DM_FN double dm_max(double a, double b) { return (a < b) ? b : a; }
 DM_FN double dm_sqrt(double x) { return __builtin_sqrt(x); }
+#ifdef __cplusplus
+/* Several correctly rounded quotients a / b with one divisor b.  The gfx950
+ * IEEE division is: scale the operands (v_div_scale), y = rcp(b) refined by
+ * two FMA Newton steps, q = a y, r = fma(-b, q, a), fma(r, y, q), then fix up
+ * special operands.  Where the scaling and the fix-up are the identity -- b in
+ * [2^-200, 2^200] and a zero or in [2^-800, 2^330]: operands, reciprocal,
+ * quotient and remainder normal, exponent gap far below the 768 at which the
+ * hardware scales -- y depends on b alone, so it is computed once
+ * (dm_recip_shared) and each quotient (dm_div_shared) costs three dependent
+ * operations instead of eleven, giving the same bits as a / b.  Callers check
+ * that domain; the host build (CPU replay of the kernel code) divides. */
+struct DmRecip
+{
+	double b, y;
+};
+DM_FN DmRecip dm_recip_shared(double b)
+{
+	DmRecip R;
+	R.b = b;
+#if defined(__HIP_DEVICE_COMPILE__)
+	double y = __builtin_amdgcn_rcp(b);
+	double e = __builtin_fma(-b, y, 1.0);
+	y = __builtin_fma(y, e, y);
+	e = __builtin_fma(-b, y, 1.0);
+	R.y = __builtin_fma(y, e, y);
+#else
+	R.y = 1.0 / b;
+#endif
+	return R;
+}
+DM_FN double dm_div_shared(double a, const DmRecip& R)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+	const double q = a * R.y;
+	const double r = __builtin_fma(-R.b, q, a);
+	return __builtin_fma(r, R.y, q);
+#else
+	return a / R.b;
+#endif
+}
+#endif
+
 DM_FN double dm_clamp_unit(double v) { return dm_max(0.0, dm_min(1.0, v)); }
 DM_FN double dm_clamp_cos(double v) { return dm_max(-1.0, dm_min(1.0, v)); }
 

@@ -260,6 +260,32 @@ DMS_HD double dms_dphi_dt(const DmsProp& p, double r)
 	return p.J / r / r;
 }
 
+/* dms_dv_dt and (want_phi) dms_dphi_dt of one RKF45 stage.  On the device the
+ * seven divisions by r share one reciprocal (dm_div_shared) when every
+ * dividend and r lie in its exact domain: r in [1 km, 1e40], |J| zero or in
+ * [1e-30, 1e50] (j_ok, checked once per attempt) and G M zero or in
+ * [1e-60, 1e100] keep every dividend of the chains in [1e-180, 1e100];
+ * otherwise, and on the host, the reference's divisions. */
+DMS_HD void dms_stage_derivs(const DmsProp& p, double r, double mass, bool j_ok, bool want_phi, double& dv, double& dphi)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+	const double gm = DMS_G_Newton * mass;
+	const bool fast = j_ok & (r >= 1.0 * DMS_km) & (r <= 1.0e40) & ((gm == 0.0) | ((gm >= 1.0e-60) & (gm <= 1.0e100)));
+	if(fast)
+	{
+		const DmRecip R = dm_recip_shared(r);
+		const double jj = p.J * p.J;
+		dv = dm_div_shared(dm_div_shared(dm_div_shared(jj, R), R), R) - dm_div_shared(dm_div_shared(gm, R), R);
+		if(want_phi)
+			dphi = (p.radial || p.J == 0.0) ? 0.0 : dm_div_shared(dm_div_shared(p.J, R), R);
+		return;
+	}
+#endif
+	dv = dms_dv_dt(p, r, mass);
+	if(want_phi)
+		dphi = dms_dphi_dt(p, r);
+}
+
 /* One attempt of Runge_Kutta_45_Step (:1487-1602).  Returns 1 accepted,
  * 0 forced-accepted (retry cap or minimum step: the step "fails"), -1 retry. */
 DMS_HD int dms_rk45_attempt(DmsProp& p, const DmsSolarView& S, int inner_iter)
@@ -267,33 +293,42 @@ DMS_HD int dms_rk45_attempt(DmsProp& p, const DmsSolarView& S, int inner_iter)
 	const double tol0 = 1.0 * DMS_km, tol1 = 1.0e-3 * DMS_km / DMS_sec, tol2 = 1.0e-7;
 	const double h = p.ts;
 	double kr[6], kv[6], kp[6], ri;
+	const double aJ = dm_fabs(p.J);
+	const bool j_ok = (p.J == 0.0) | ((aJ >= 1.0e-30) & (aJ <= 1.0e50));
+	double dv, dph = 0.0;
 	kr[0] = h * p.vr;
-	kv[0] = h * dms_dv_dt(p, p.radius, dms_mass(S, p.radius));
-	kp[0] = h * dms_dphi_dt(p, p.radius);
+	dms_stage_derivs(p, p.radius, dms_mass(S, p.radius), j_ok, true, dv, dph);
+	kv[0] = h * dv;
+	kp[0] = h * dph;
 
 	ri = p.radius + kr[0] / 4.0;
 	kr[1] = h * (p.vr + kv[0] / 4.0);
-	kv[1] = h * dms_dv_dt(p, ri, dms_mass(S, ri));
+	dms_stage_derivs(p, ri, dms_mass(S, ri), j_ok, false, dv, dph);
+	kv[1] = h * dv;
 
 	ri = p.radius + 3.0 / 32.0 * kr[0] + 9.0 / 32.0 * kr[1];
 	kr[2] = h * (p.vr + 3.0 / 32.0 * kv[0] + 9.0 / 32.0 * kv[1]);
-	kv[2] = h * dms_dv_dt(p, ri, dms_mass(S, ri));
-	kp[2] = h * dms_dphi_dt(p, ri);
+	dms_stage_derivs(p, ri, dms_mass(S, ri), j_ok, true, dv, dph);
+	kv[2] = h * dv;
+	kp[2] = h * dph;
 
 	ri = p.radius + 1932.0 / 2197.0 * kr[0] - 7200.0 / 2197.0 * kr[1] + 7296.0 / 2197.0 * kr[2];
 	kr[3] = h * (p.vr + 1932.0 / 2197.0 * kv[0] - 7200.0 / 2197.0 * kv[1] + 7296.0 / 2197.0 * kv[2]);
-	kv[3] = h * dms_dv_dt(p, ri, dms_mass(S, ri));
-	kp[3] = h * dms_dphi_dt(p, ri);
+	dms_stage_derivs(p, ri, dms_mass(S, ri), j_ok, true, dv, dph);
+	kv[3] = h * dv;
+	kp[3] = h * dph;
 
 	ri = p.radius + 439.0 / 216.0 * kr[0] - 8.0 * kr[1] + 3680.0 / 513.0 * kr[2] - 845.0 / 4104.0 * kr[3];
 	kr[4] = h * (p.vr + 439.0 / 216.0 * kv[0] - 8.0 * kv[1] + 3680.0 / 513.0 * kv[2] - 845.0 / 4104.0 * kv[3]);
-	kv[4] = h * dms_dv_dt(p, ri, dms_mass(S, ri));
-	kp[4] = h * dms_dphi_dt(p, ri);
+	dms_stage_derivs(p, ri, dms_mass(S, ri), j_ok, true, dv, dph);
+	kv[4] = h * dv;
+	kp[4] = h * dph;
 
 	ri = p.radius - 8.0 / 27.0 * kr[0] + 2.0 * kr[1] - 3544.0 / 2565.0 * kr[2] + 1859.0 / 4104.0 * kr[3] - 11.0 / 40.0 * kr[4];
 	kr[5] = h * (p.vr - 8.0 / 27.0 * kv[0] + 2.0 * kv[1] - 3544.0 / 2565.0 * kv[2] + 1859.0 / 4104.0 * kv[3] - 11.0 / 40.0 * kv[4]);
-	kv[5] = h * dms_dv_dt(p, ri, dms_mass(S, ri));
-	kp[5] = h * dms_dphi_dt(p, ri);
+	dms_stage_derivs(p, ri, dms_mass(S, ri), j_ok, true, dv, dph);
+	kv[5] = h * dv;
+	kp[5] = h * dph;
 
 	const double r4 = p.radius + 25.0 / 216.0 * kr[0] + 1408.0 / 2565.0 * kr[2] + 2197.0 / 4104.0 * kr[3] - 1.0 / 5.0 * kr[4];
 	const double v4 = p.vr + 25.0 / 216.0 * kv[0] + 1408.0 / 2565.0 * kv[2] + 2197.0 / 4104.0 * kv[3] - 1.0 / 5.0 * kv[4];

@@ -59,6 +59,27 @@ def test_device_math_matches_host(gpu_ctx):
         assert np.array_equal(dev.view(np.uint64), host.view(np.uint64)), name
 
 
+def test_shared_reciprocal_division_is_ieee(gpu_ctx):
+    """dm_div_shared (one refined reciprocal per divisor, three operations per
+    quotient) gives the bits of a / b over its stated domain: b in
+    [2^-200, 2^200], a zero or |a| in [2^-800, 2^330], plus the divisors and
+    dividends of the RKF45 stages (r in [1 km, 1e40], dividends in [1e-180, 1e100])."""
+    rng = np.random.default_rng(11)
+    n = 400000
+    b = np.exp2(rng.uniform(-200, 200, n))
+    a = np.exp2(rng.uniform(-800, 330, n)) * rng.choice([-1.0, 1.0], n)
+    a[::97] = 0.0
+    b2 = np.exp(rng.uniform(np.log(5.07e18), np.log(1e40), n))
+    a2 = np.exp(rng.uniform(np.log(1e-180), np.log(1e100), n)) * rng.choice([-1.0, 1.0], n)
+    # mantissas next to powers of two and all-ones mantissas
+    edge = np.array([np.nextafter(2.0 ** k, 0.0) for k in range(-150, 150)] + [2.0 ** k for k in range(-150, 150)])
+    x = np.concatenate([a, a2, np.tile(edge, 8)[: 2 * edge.size], rng.permutation(np.tile(edge, 2))])
+    y = np.concatenate([b, b2, np.repeat(edge, 2)[: 2 * edge.size], np.tile(edge, 2)])
+    dev = gpu_ctx.probe_math(8, x, y)
+    ref = x / y
+    assert np.array_equal(dev.view(np.uint64), ref.view(np.uint64))
+
+
 def host_detmath(fn, x, y):
     """Host evaluation of the same deterministic functions, via the oracle build."""
     import oracle_lib as O
