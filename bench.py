@@ -314,6 +314,13 @@ def main():
     out["bulk"] = {"fp64": measure_bulk(dmsun, pt, dmsun.FP64)}
     if mixed is not None:
         out["bulk"]["mixed"] = measure_bulk(dmsun, pt, dmsun.MIXED)
+        # kernel speed ratios that do not depend on which trajectory of a launch
+        # happens to be the longest (the two kernels' histories separate through
+        # rounding, so their longest trajectories differ)
+        out["mixed"]["speedup_vs_fp64_bulk"] = (out["bulk"]["mixed"]["rk_attempts_per_s"]
+                                                / out["bulk"]["fp64"]["rk_attempts_per_s"])
+        out["mixed"]["speedup_vs_fp64_per_attempt_of_longest"] = (out["tail"]["us_per_attempt_of_longest"]
+                                                                  / out["tail"]["mixed_us_per_attempt_of_longest"])
     if not args.no_uneven:
         out["batched_uneven"] = measure_uneven(dmsun, ctx, args)
     if not args.no_serving:
