@@ -28,6 +28,13 @@ int main(int argc, char** argv)
 			          << "\nmax_scatterings=" << c.maximum_number_of_scatterings << "\nsnapshot=" << c.snapshot_enabled
 			          << "\nwall=" << c.max_trajectory_wall_time_sec << "\n";
 		}
+		else if(mode == "percentile")
+		{
+			std::vector<double> v;
+			for(int i = 3; i < argc; i++)
+				v.push_back(std::stod(argv[i]));
+			std::printf("p=%.17g\n", dmsun::Percentile(v, std::stod(argv[2])));
+		}
 		else if(mode == "wilson")
 		{
 			const auto e = dmsun::Estimate_Binomial_Rate(std::stoull(argv[2]), std::stoull(argv[3]));

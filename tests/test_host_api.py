@@ -6,6 +6,7 @@ import math
 import pathlib
 import subprocess
 
+import numpy as np
 import pytest
 
 REPO = pathlib.Path(__file__).resolve().parent.parent
@@ -128,3 +129,23 @@ def test_driver_fails_loudly_without_gpu(host_unit):
     p = subprocess.run([str(PKG / "dmsun_run"), str(GOLD / "config_capture_minimal.cfg")], capture_output=True, text=True,
                        cwd=str(REPO))
     assert p.returncode == 1 and "Error" in p.stderr
+
+
+def test_percentile(host_unit):
+    """Latency percentiles of the serving loop: linear interpolation between
+    order statistics (numpy's default), 0 for no samples."""
+    rng = np.random.default_rng(3)
+    for n in (1, 2, 7, 100):
+        x = rng.exponential(2.0, n)
+        for q in (0, 50, 90, 99, 100):
+            got = float(run(host_unit, "percentile", q, *[repr(float(v)) for v in x])["p"])
+            assert got == pytest.approx(np.percentile(x, q), rel=1e-12, abs=1e-15)
+    assert float(run(host_unit, "percentile", 50)["p"]) == 0.0
+
+
+def test_serve_rejects_bad_arguments():
+    exe = PKG / "dmsun_serve"
+    for args in (["--requests", "0"], ["--rate", "-1"], ["--precision", "fp16"], ["--bogus"],
+                 ["--requests", "4", "--check-alone", "5"]):
+        p = subprocess.run([str(exe), *args], capture_output=True, text=True, timeout=60)
+        assert p.returncode == 2, (args, p.stdout, p.stderr)
