@@ -155,7 +155,20 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 			if(lane == s)
 				L.sc_target = target;
 		}
+		const int captured_before = L.captured;
 		dms_lane_finish_scatter<MX>(L, c);
+		/* capture-target chunk: count captures as they happen (the post-scatter
+		 * test), so that the slot limit drops while the captured trajectories
+		 * still orbit */
+		if(a.fill_ctl && L.captured && !captured_before)
+		{
+			const unsigned int ncap = atomicAdd(a.fill_ctl, 1u) + 1u;
+			if(ncap == a.cap_stop)
+			{
+				const unsigned int handed = __hip_atomic_load(a.next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+				atomicMin(a.fill_ctl + 1, handed + a.fill_extra);
+			}
+		}
 		int flush_cls = -1;
 		if(L.phase == PH_FIN)
 		{
@@ -189,15 +202,20 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 		if(nmask)
 		{
 			const int leader = __ffsll((long long) nmask) - 1;
-			unsigned int base = 0;
+			unsigned int base = 0, limit = a.n;
 			if(lane == leader)
+			{
 				base = atomicAdd(a.next, (unsigned int) __popcll(nmask));
+				if(a.fill_ctl)
+					limit = __hip_atomic_load(a.fill_ctl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+			}
 			base = __shfl(base, leader);
+			limit = __shfl(limit, leader);
 			if(L.phase == PH_NEED)
 			{
 				const unsigned int rank = (unsigned int) __popcll(nmask & ((1ull << lane) - 1ull));
 				const unsigned int idx = base + rank;
-				if(idx < a.n)
+				if(idx < a.n && idx < limit)
 				{
 					L.slot = idx;
 					/* job of the slot: last j with job_start[j] <= idx */

@@ -37,6 +37,12 @@ struct DmsTrajArgs
 	 * views and Philox keys {seed, point_id}; null for one point */
 	const DmsPointView* job_points;
 	const uint32_t* job_keys;
+	/* capture-target chunks (may be null): fill_ctl[0] counts the launch's
+	 * trajectories that got captured; when it reaches cap_stop, fill_ctl[1] (the
+	 * slot limit, initially n) drops to the slots handed out so far plus
+	 * fill_extra, and no slot at or past it is started */
+	unsigned int* fill_ctl;
+	uint32_t cap_stop, fill_extra;
 };
 
 __global__ void dms_traj_kernel(DmsTrajArgs a);

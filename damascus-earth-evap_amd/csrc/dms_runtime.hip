@@ -255,7 +255,7 @@ struct ChunkBuffers
 hipError_t ensure(ChunkBuffers& B, size_t n, int nbins, bool rows, bool recs, hipStream_t st)
 {
 	hipError_t e = hipSuccess;
-	if(B.next == nullptr && (e = dalloc(&B.next, 1, st)) != hipSuccess)
+	if(B.next == nullptr && (e = dalloc(&B.next, 4, st)) != hipSuccess) /* slot counter + fill control */
 		return e;
 	if(n > B.cap)
 	{
@@ -347,9 +347,17 @@ struct LaunchJobs
  * count of each job of the launch. */
 using ProgressFn = std::function<void(const std::vector<unsigned int>&)>;
 
+/* Capture-target chunk: stop handing out slots fill_extra slots after the
+ * launch's cap_stop-th captured trajectory.  Slots never started keep an
+ * all-ones summary (reason -1); the host counts only the started prefix. */
+struct FillStop
+{
+	uint32_t cap_stop, fill_extra;
+};
+
 hipError_t run_trajectories(dms_ctx* ctx, hipStream_t st, hipStream_t pst, const dms_point* pt, const DmsJobView& J, ChunkBuffers& B,
                             size_t n, bool rows, bool recs, const LaunchJobs& LJ, const DmsRows* lane_rows, float* ms,
-                            const ProgressFn& progress = nullptr)
+                            const ProgressFn& progress = nullptr, const FillStop* stop = nullptr)
 {
 	DmsTrajArgs a;
 	a.S = ctx->dev;
@@ -378,9 +386,22 @@ hipError_t run_trajectories(dms_ctx* ctx, hipStream_t st, hipStream_t pst, const
 	a.job_done = LJ.d_done;
 	a.job_points = LJ.d_points;
 	a.job_keys = LJ.d_keys;
+	a.fill_ctl = nullptr;
+	a.cap_stop = 0;
+	a.fill_extra = 0;
 	hipError_t e;
 	if((e = hipMemsetAsync(B.next, 0, sizeof(unsigned int), st)) != hipSuccess)
 		return e;
+	if(stop)
+	{
+		a.fill_ctl = B.next + 1;
+		a.cap_stop = stop->cap_stop;
+		a.fill_extra = stop->fill_extra;
+		if((e = hipMemsetD32Async((hipDeviceptr_t) (B.next + 1), 0, 1, st)) != hipSuccess
+		   || (e = hipMemsetD32Async((hipDeviceptr_t) (B.next + 2), (int) n, 1, st)) != hipSuccess
+		   || (e = hipMemsetAsync(B.summ, 0xFF, sizeof(DmsSummary) * n, st)) != hipSuccess)
+			return e;
+	}
 	if(rows && !lane_rows)
 	{
 		if((e = hipMemsetAsync(B.rows, 0, sizeof(double) * 2 * (size_t) J.nbins * n, st)) != hipSuccess)
@@ -424,14 +445,14 @@ hipError_t run_trajectories(dms_ctx* ctx, hipStream_t st, hipStream_t pst, const
 /* single-job launch over slots [0, n) with ids first_id .. */
 hipError_t run_trajectories_single(dms_ctx* ctx, hipStream_t st, hipStream_t pst, const dms_point* pt, const DmsJobView& J,
                                    ChunkBuffers& B, size_t n, bool rows, bool recs, uint64_t first_id, float* ms,
-                                   const ProgressFn& progress = nullptr)
+                                   const ProgressFn& progress = nullptr, const FillStop* stop = nullptr)
 {
 	LaunchJobs LJ;
 	LJ.start.push_back(0);
 	LJ.first_id.push_back(first_id);
 	hipError_t e = LJ.upload(st);
 	if(e == hipSuccess)
-		e = run_trajectories(ctx, st, pst, pt, J, B, n, rows, recs, LJ, nullptr, ms, progress);
+		e = run_trajectories(ctx, st, pst, pt, J, B, n, rows, recs, LJ, nullptr, ms, progress, stop);
 	LJ.release(st);
 	return e;
 }
@@ -677,7 +698,12 @@ void run_job_chunked(dms_job* job, hipStream_t st, hipStream_t pst)
 		{
 			const double rate = (C.trajectories > 0 && C.captured > 0) ? (double) C.captured / (double) C.trajectories : 0.0;
 			const uint64_t need = R.target - R.captured;
-			want = rate > 0.0 ? (uint64_t) (1.25 * (double) need / rate) + 4096 : 65536;
+			/* Every launch waits for its longest trajectory, while the bulk of a
+			 * few hundred thousand more trajectories costs a small fraction of
+			 * that: size chunks so that one more launch is rarely needed.  The
+			 * trajectories past the stop point are discarded (results do not
+			 * depend on the chunking). */
+			want = rate > 0.0 ? (uint64_t) (1.5 * (double) need / rate) + 65536 : 262144;
 			if(!R.unlimited && want > R.max_traj - R.local_total)
 				want = R.max_traj - R.local_total;
 			if(want < R.left_in_round)
@@ -697,8 +723,22 @@ void run_job_chunked(dms_job* job, hipStream_t st, hipStream_t pst)
 		                   next_id, (uint32_t) n, d.r_max_rsun * DMS_rSun, B.ics, B.ok);
 		float kms = 0.0f;
 		const uint64_t base_done = C.trajectories;
+		/* capture target: the launch stops starting trajectories once it has
+		 * captured a quarter more than still needed, plus enough slots to
+		 * close the current sync round twice over */
+		FillStop fs{0u, 0u};
+		const bool fill_stop = R.target != UINT64_MAX;
+		if(fill_stop)
+		{
+			const uint64_t need = R.target > R.captured ? R.target - R.captured : 1;
+			const uint64_t cs = need + need / 4 + 32;
+			const uint64_t fx = 2 * R.interval + 1024;
+			fs.cap_stop = (uint32_t) (cs < 0x7fffffffULL ? cs : 0x7fffffffULL);
+			fs.fill_extra = (uint32_t) (fx < (uint64_t) n ? fx : (uint64_t) n);
+		}
 		hipError_t e = run_trajectories_single(ctx, st, pst, pt, J, B, n, want_rows, want_recs, next_id, &kms,
-		                                       [job, base_done](const std::vector<unsigned int>& d) { job->done_traj = base_done + d[0]; });
+		                                       [job, base_done](const std::vector<unsigned int>& d) { job->done_traj = base_done + d[0]; },
+		                                       fill_stop ? &fs : nullptr);
 		if(e != hipSuccess)
 		{
 			set_err(std::string("trajectory kernel: ") + hipGetErrorString(e));
@@ -718,6 +758,14 @@ void run_job_chunked(dms_job* job, hipStream_t st, hipStream_t pst)
 			set_err("reading trajectory summaries failed");
 			break;
 		}
+		/* started prefix (slots past the fill limit were never started) */
+		if(fill_stop)
+			for(size_t i = 0; i < n; i++)
+				if(hsum[i].reason == -1)
+				{
+					n = i;
+					break;
+				}
 		simulated += n;
 		next_id += n;
 		hcls.assign(n, 0);
