@@ -89,7 +89,14 @@ def cpu_baseline(sample, threads):
     res = pt.run_job(job, threads=threads, want_records=False)
     dt = time.perf_counter() - t0
     c = res["counters"]
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((l.split(":", 1)[1].strip() for l in f if l.startswith("model name")), "")
+    except OSError:
+        pass
     return dict(value=c.trajectories / dt, unit="trajectories/s", cores=threads, kind="port",
+                host=f"{model}; {os.cpu_count()} logical CPUs visible",
                 sample=f"{sample} trajectories of the same workload (ids 0..{sample - 1}), "
                        f"{threads} threads, {dt:.2f} s; scatterings/s {c.total_scatterings / dt:.4g}")
 
