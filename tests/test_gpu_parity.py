@@ -120,6 +120,10 @@ CASES = [
     ("point_dp_general", dmsun.MODE_PARAMETER_POINT,
      dict(mass_GeV=0.1, sigma_p_cm2=0.0, sigma_e_cm2=1e-37, interaction=dmsun.DARK_PHOTON, form_factor=dmsun.FF_GENERAL,
           mediator_mass_MeV=0.01, interpolation_points=300), 24, 10**14),
+    ("point_dp_contact", dmsun.MODE_PARAMETER_POINT,
+     dict(mass_GeV=0.1, sigma_p_cm2=0.0, sigma_e_cm2=1e-37, interaction=dmsun.DARK_PHOTON, form_factor=dmsun.FF_CONTACT,
+          interpolation_points=300), 24, 10**14),
+    ("capture_si_0.5GeV_no_table", dmsun.MODE_CAPTURE, dict(mass_GeV=0.5, sigma_p_cm2=1e-36, interpolation_points=0), 64, 1000),
 ]
 
 
