@@ -225,6 +225,65 @@ def measure_bulk(dmsun, pt, precision):
             "truncated": c.computational_truncations}
 
 
+def validate_mixed(dmsun, ctx, n):
+    """The mixed-precision workload at full size (SI m = 0.5 GeV, sigma_p =
+    1e-37 cm^2, n trajectories, parameter point): FP64 and mixed kernels on the
+    same trajectory ids, per-output differences and speed."""
+    import math
+    import numpy as np
+    from scipy import stats
+
+    pt = ctx.point(dmsun.make_point(0.5, 1e-37, interpolation_points=1000))
+    res = {}
+    for name, prec in (("fp64", dmsun.FP64), ("mixed", dmsun.MIXED)):
+        job = dmsun.make_job(mode=dmsun.MODE_PARAMETER_POINT, max_trajectories=n, capture_target=0,
+                             max_scatterings=10**14, n_bins=1000, record=0, precision=prec,
+                             halo=dmsun.default_halo((0.0, 248.0, 0.0)))
+        t = time.perf_counter()
+        r = pt.run_job(job, want_records=False)
+        r["wall"] = time.perf_counter() - t
+        res[name] = r
+        print(f"validate-mixed {name}: {r['wall']:.2f} s", file=sys.stderr, flush=True)
+    a, b = res["fp64"], res["mixed"]
+    ca, cb = a["counters"], b["counters"]
+    pa, pb = ca.captured / n, cb.captured / n
+    sig = math.sqrt(pa * (1 - pa) / n + pb * (1 - pb) / n)
+
+    def bin_err(s, s2, cnt):
+        if cnt <= 1:
+            return np.zeros_like(s)
+        m = s / cnt
+        return np.sqrt(cnt * np.maximum(0.0, s2 / cnt - m * m))
+
+    chi2, dof = 0.0, 0
+    for base, na, nb in ((0, ca.captured, cb.captured), (4, ca.completed_outward_escapes, cb.completed_outward_escapes)):
+        for q in (0, 1):
+            ha, hb = a["hist"][base + q], b["hist"][base + q]
+            var = bin_err(ha, a["hist"][base + 2 + q], na) ** 2 + bin_err(hb, b["hist"][base + 2 + q], nb) ** 2
+            use = var > 0
+            chi2 += float(np.sum((ha[use] - hb[use]) ** 2 / var[use]))
+            dof += int(use.sum())
+    la = np.array([e.lifetime_unbinding_sec for e in a["events"]])
+    lb = np.array([e.lifetime_unbinding_sec for e in b["events"]])
+    out = {"workload": f"parameter point SI m=0.5 GeV sigma_p=1e-37 cm^2, {n} trajectories, 1000 bins, same ids",
+           "capture_fraction": [pa, pb], "capture_delta_over_sigma": abs(pa - pb) / sig if sig > 0 else 0.0,
+           "hist_chi2_per_dof": chi2 / max(dof, 1), "hist_bins": dof,
+           "evaporation_events": [len(la), len(lb)],
+           "scatterings_mean_rel_diff": (cb.total_scatterings - ca.total_scatterings) / max(1, ca.total_scatterings)}
+    if min(len(la), len(lb)) >= 20:
+        out["evaporation_ks_p"] = float(stats.ks_2samp(la, lb).pvalue)
+        out["evaporation_median_rel_diff"] = float(np.median(lb) / np.median(la) - 1.0)
+        out["evaporation_p90_rel_diff"] = float(np.percentile(lb, 90) / np.percentile(la, 90) - 1.0)
+    for name, r in res.items():
+        c = r["counters"]
+        out[name] = {"seconds": r["wall"], "trajectories_per_s": c.trajectories / r["wall"],
+                     "scatterings_per_s": c.total_scatterings / r["wall"], "longest_trajectory_rk_attempts": c.max_rk_attempts,
+                     "us_per_attempt_of_longest": 1e3 * r["kernel_ms"] / max(1, c.max_rk_attempts)}
+    out["speedup_whole_job"] = a["wall"] / b["wall"]
+    out["speedup_per_attempt_of_longest"] = out["fp64"]["us_per_attempt_of_longest"] / out["mixed"]["us_per_attempt_of_longest"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -241,7 +300,14 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=0.5)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run one plain device-to-device copy of BYTES bytes (scale of the HBM byte counters)")
+    ap.add_argument("--validate-mixed", type=int, default=0, metavar="N",
+                    help="only run the mixed-precision workload (m=0.5 GeV, sigma_p=1e-37) with N trajectories in FP64 "
+                         "and mixed precision and print the per-output differences")
     args = ap.parse_args()
+    if args.validate_mixed > 0:
+        import dmsun
+        print(json.dumps(validate_mixed(dmsun, dmsun.Context(device=0), args.validate_mixed)), flush=True)
+        return
     if args.copy_probe > 0:
         import torch
         src = torch.ones(args.copy_probe, dtype=torch.uint8, device="cuda")
