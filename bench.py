@@ -27,10 +27,12 @@ cpu_baseline: the CPU restatement (oracle/liboracle.so, one trajectory stream
 per host thread) on a bounded sample of the same workload.
 
 Beside the headline line: "mixed" (the same steps on the mixed-precision
-kernel), "batched_uneven" (65 parameter points of very different trajectory
-lengths in one autotuned launch) and "serving" (a Poisson request stream
-through the serving loop, p50/p99 latency).  --no-mixed, --no-uneven,
---no-serving and --no-cpu leave them out.
+kernel), "mixed_validation" (the mixed-precision workload, m = 0.5 GeV and
+sigma_p = 1e-37 cm^2, at 10^6 trajectories in both precisions: per-output
+differences and speed), "batched_uneven" (65 parameter points of very
+different trajectory lengths in one autotuned launch) and "serving" (a Poisson
+request stream through the serving loop, p50/p99 latency).  --no-mixed,
+--no-validate, --no-uneven, --no-serving and --no-cpu leave them out.
 """
 import argparse
 import json
@@ -294,6 +296,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-uneven", action="store_true")
+    ap.add_argument("--no-validate", action="store_true", help="leave out the full-size mixed-precision validation")
     ap.add_argument("--uneven-trajectories", type=int, default=4000)
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=16)
@@ -394,6 +397,8 @@ def main():
                                                 / out["bulk"]["fp64"]["rk_attempts_per_s"])
         out["mixed"]["speedup_vs_fp64_per_attempt_of_longest"] = (out["tail"]["us_per_attempt_of_longest"]
                                                                   / out["tail"]["mixed_us_per_attempt_of_longest"])
+    if not args.no_mixed and not args.no_validate:
+        out["mixed_validation"] = validate_mixed(dmsun, ctx, 1_000_000)
     if not args.no_uneven:
         out["batched_uneven"] = measure_uneven(dmsun, ctx, args)
     if not args.no_serving:
