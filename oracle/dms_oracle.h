@@ -4,7 +4,8 @@
  * A plain-C, scalar FP64 restatement of the DaMaSCUS-SUN-EVAP trajectory path
  * (reference files cited per function in dms_oracle.c).  Only tests/,
  * __graft_entry__.smoke() and the cpu_baseline leg of bench.py load it, and only
- * as the checker or the timed CPU baseline; the product library never links it.
+ * to check the GPU results or as the timed CPU baseline; the product library
+ * never links it.
  *
  * It shares two things with the product: the public struct layouts of
  * include/dms.h (the boundary format) and the deterministic elementary
