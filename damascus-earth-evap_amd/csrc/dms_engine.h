@@ -720,7 +720,7 @@ DMS_HD void dms_rates_at_fractions(const DmsPointView& P, const DmsSolarView& S,
 #pragma unroll
 	for(int i = 0; i < NQ; i++)
 		any_comp = any_comp || comp[i];
-	if(any_comp)
+	if(__builtin_expect(any_comp, 0)) /* no table, or v above it: rare */
 	{
 #pragma unroll
 		for(int i = 0; i < NQ; i++)
@@ -1067,13 +1067,13 @@ DMS_HD int dms_step_post(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 		}                               \
 		return (code);                  \
 	} while(0)
-	if(!L.rk_ok)
+	if(__builtin_expect(!L.rk_ok, 0))
 		DMS_RETURN_AT(ea_t, ea_r, ea_v, DMS_NUMERICAL_FAILURE);
-	if(!dm_isfinite(r_after) || !dm_isfinite(v_after) || !dm_isfinite(actual_dt) || actual_dt <= 0.0)
+	if(__builtin_expect(!dm_isfinite(r_after) || !dm_isfinite(v_after) || !dm_isfinite(actual_dt) || actual_dt <= 0.0, 0))
 		DMS_RETURN_AT(ea_t, ea_r, ea_v, DMS_NON_FINITE_STATE);
-	if(v_after > DMS_V_MAX)
+	if(__builtin_expect(v_after > DMS_V_MAX, 0))
 		DMS_RETURN_AT(ea_t, ea_r, ea_v, DMS_SPEED_LIMIT);
-	if(J.rk_budget > 0 && L.rk_all > J.rk_budget)
+	if(__builtin_expect(J.rk_budget > 0 && L.rk_all > J.rk_budget, 0))
 		DMS_RETURN_AT(ea_t, ea_r, ea_v, DMS_WALL_TIME_LIMIT);
 	const int fast = J.terminate_on_capture;
 	const double od_limit = fast ? DMS_CAPTURE_MODE_MAX_OPTICAL_DEPTH_STEP : DMS_MAX_OPTICAL_DEPTH_STEP;

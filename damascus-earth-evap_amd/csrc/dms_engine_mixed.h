@@ -298,7 +298,7 @@ DMS_HD void dms_rates_at_fractions_mx(const DmsPointView& P, const DmsSolarView&
 #pragma unroll
 	for(int i = 0; i < NQ; i++)
 		any = any || comp[i];
-	if(any)
+	if(__builtin_expect(any, 0)) /* no table, or v above it: rare */
 	{
 #pragma unroll
 		for(int i = 0; i < NQ; i++)

@@ -76,8 +76,10 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 		dms_lane_advance_a<MX>(L, c);
 		/* Sample_Target (src/Simulation_Trajectory.cpp:1122-1158): for each
 		 * scattering lane s the 64 lanes evaluate one target rate each. */
+		/* (the rare branches of the loop are marked so that the compiler lays
+		 * them out of the path a lone long trajectory executes) */
 		unsigned long long m = __ballot(L.phase == PH_SCAT);
-		while(m)
+		while(__builtin_expect(m != 0, 0))
 		{
 			const int s = __ffsll((long long) m) - 1;
 			m &= m - 1;
@@ -170,7 +172,7 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 			}
 		}
 		int flush_cls = -1;
-		if(L.phase == PH_FIN)
+		if(__builtin_expect(L.phase == PH_FIN, 0))
 		{
 			flush_cls = dms_lane_finish_rows_pending(L, c, a.ics[L.slot], a.records ? a.records + L.slot : nullptr,
 			                                         a.summ ? a.summ + L.slot : nullptr);
@@ -180,7 +182,7 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 		/* per-lane histogram rows of the finished trajectories: the wave adds
 		 * each row's touched bins to its job's sums, 64 bins at a time */
 		unsigned long long fm = __ballot(flush_cls >= 0);
-		if(fm)
+		if(__builtin_expect(fm != 0, 0))
 		{
 			while(fm)
 			{
@@ -199,7 +201,7 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 		}
 		/* lane refill: one atomic per wave */
 		const unsigned long long nmask = __ballot(L.phase == PH_NEED);
-		if(nmask)
+		if(__builtin_expect(nmask != 0, 0))
 		{
 			const int leader = __ffsll((long long) nmask) - 1;
 			unsigned int base = 0, limit = a.n;
@@ -244,7 +246,7 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 					L.phase = PH_DONE;
 			}
 		}
-		if(L.phase == PH_START)
+		if(__builtin_expect(L.phase == PH_START, 0))
 		{
 			dms_lane_start(L, c, a.ics[L.slot], a.shift_ok == nullptr || a.shift_ok[L.slot] != 0,
 			               a.records ? a.records + L.slot : nullptr, a.summ ? a.summ + L.slot : nullptr);

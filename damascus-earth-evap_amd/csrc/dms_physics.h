@@ -104,7 +104,7 @@ DMS_HD double dms_spline_at(const DmsSolarView& S, const DmsCubic* T, double x)
 	const double xn = S.x[j + 1];
 	const DmsCubic c = T[j];
 	const bool ok = (j == n - 2 || !(xn <= x)) && (j == 0 || !(xj > x));
-	if(ok)
+	if(__builtin_expect(ok, 1))
 		return dms_cubic(c, x - xj);
 	const int k = dms_interval(S, x);
 	return dms_cubic(T[k], x - S.x[k]);
