@@ -1131,7 +1131,7 @@ DMS_HD int dms_step_post(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 					}
 				}
 			}
-			if(invalid || !mx_isfinite(dtau) || dtau < 0.0f)
+			if(__builtin_expect(invalid || !mx_isfinite(dtau) || dtau < 0.0f, 0))
 				DMS_RETURN_AT(ea_t, ea_r, ea_v, DMS_NUMERICAL_FAILURE);
 			const float scale = fmaxf(dtau, (float) DMS_OPTICAL_DEPTH_ABSOLUTE_TOLERANCE);
 			const float relerr = fabsf(dtau - tau2) * mx_rcp(scale);
@@ -1141,7 +1141,7 @@ DMS_HD int dms_step_post(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 			{
 				L.n_rej++;
 				L.od_retries++;
-				if(L.od_retries > DMS_MAX_OPTICAL_DEPTH_RETRIES)
+				if(__builtin_expect(L.od_retries > DMS_MAX_OPTICAL_DEPTH_RETRIES, 0))
 					DMS_RETURN_AT(L.eb_t, L.eb_r, L.eb_v, DMS_NUMERICAL_FAILURE);
 				L.prop.time = L.pb_time;
 				L.prop.radius = L.pb_radius;
@@ -1159,7 +1159,7 @@ DMS_HD int dms_step_post(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 				return -2;
 			}
 			const float remaining_total = L.mlx_f - L.mlx_c;
-			if(dtau >= remaining_total)
+			if(__builtin_expect(dtau >= remaining_total, 0)) /* a scattering inside this step */
 			{
 				float remaining = remaining_total;
 				int hit = -1;
@@ -1237,7 +1237,7 @@ DMS_HD int dms_step_post(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 	#undef DMS_TRAP
 				}
 			}
-			if(invalid || !dm_isfinite(dtau) || dtau < 0.0)
+			if(__builtin_expect(invalid || !dm_isfinite(dtau) || dtau < 0.0, 0))
 				DMS_RETURN_AT(ea_t, ea_r, ea_v, DMS_NUMERICAL_FAILURE);
 			const double scale = dm_max(dtau, DMS_OPTICAL_DEPTH_ABSOLUTE_TOLERANCE);
 			const double relerr = dm_fabs(dtau - tau2) / scale;
@@ -1247,7 +1247,7 @@ DMS_HD int dms_step_post(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 			{
 				L.n_rej++;
 				L.od_retries++;
-				if(L.od_retries > DMS_MAX_OPTICAL_DEPTH_RETRIES)
+				if(__builtin_expect(L.od_retries > DMS_MAX_OPTICAL_DEPTH_RETRIES, 0))
 					DMS_RETURN_AT(L.eb_t, L.eb_r, L.eb_v, DMS_NUMERICAL_FAILURE);
 				L.prop.time = L.pb_time;
 				L.prop.radius = L.pb_radius;
@@ -1265,7 +1265,7 @@ DMS_HD int dms_step_post(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 				L.prop.ts = dms_rk45_sanitized(factor * actual_dt);
 				return -2;
 			}
-			if(dtau >= L.mlx)
+			if(__builtin_expect(dtau >= L.mlx, 0)) /* a scattering inside this step */
 			{
 				/* locate the scattering piece; pieces with end > start only */
 				double remaining = L.mlx;
@@ -1313,14 +1313,14 @@ DMS_HD int dms_step_post(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 	{
 		double bt = ea_t, br[3] = {ea_r[0], ea_r[1], ea_r[2]}, bv[3] = {ea_v[0], ea_v[1], ea_v[2]};
 		double rb = r_after, vb = v_after;
-		if(L.r_before < J.max_distance && r_after >= J.max_distance && r_after > L.r_before)
+		if(__builtin_expect(L.r_before < J.max_distance && r_after >= J.max_distance && r_after > L.r_before, 0))
 		{
 			const double bf = (J.max_distance - L.r_before) / (r_after - L.r_before);
 			dms_interpolate_event(L.eb_t, L.eb_r, L.eb_v, ea_t, ea_r, ea_v, bf, bt, br, bv);
 			rb = dms_norm(br);
 			vb = dms_norm(bv);
 		}
-		if(rb >= J.max_distance && rb >= L.r_before && vb > dms_local_escape_speed(S, rb))
+		if(__builtin_expect(rb >= J.max_distance && rb >= L.r_before, 0) && vb > dms_local_escape_speed(S, rb))
 		{
 			acc_t = bt;
 			for(int c = 0; c < 3; c++)
@@ -1330,7 +1330,7 @@ DMS_HD int dms_step_post(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 			}
 			reflection = 1;
 		}
-		else if(rb >= J.max_distance && rb >= L.r_before)
+		else if(__builtin_expect(rb >= J.max_distance && rb >= L.r_before, 0))
 		{
 			if(dms_uncaptured_bound(L, S, P.mass, br, bv))
 				DMS_RETURN_AT(bt, br, bv, DMS_NUMERICAL_FAILURE);
