@@ -215,7 +215,7 @@ DMS_HD double dms_prop_radius(const DmsProp& p) { return (p.radius < 0.0) ? 0.0 
 DMS_HD double dms_prop_speed(const DmsProp& p)
 {
 	const double r = (p.radius < 0.0) ? 0.0 : p.radius;
-	if(r == 0 || p.J == 0)
+	if(__builtin_expect(r == 0 || p.J == 0, 0))
 		return dm_fabs(p.vr);
 	return dm_sqrt(p.vr * p.vr + p.J * p.J / r / r);
 }
@@ -225,7 +225,7 @@ DMS_HD void dms_prop_event(const DmsProp& p, double& t, double* r, double* v)
 {
 	const double safe_radius = (dm_isfinite(p.radius) && p.radius > 0.0) ? p.radius : 0.0;
 	t = p.time;
-	if(p.radial || p.J == 0.0 || safe_radius == 0.0)
+	if(__builtin_expect(p.radial || p.J == 0.0 || safe_radius == 0.0, 0))
 	{
 		for(int c = 0; c < 3; c++)
 		{
@@ -271,7 +271,7 @@ DMS_HD void dms_stage_derivs(const DmsProp& p, double r, double mass, bool j_ok,
 #if defined(__HIP_DEVICE_COMPILE__)
 	const double gm = DMS_G_Newton * mass;
 	const bool fast = j_ok & (r >= 1.0 * DMS_km) & (r <= 1.0e40) & ((gm == 0.0) | ((gm >= 1.0e-60) & (gm <= 1.0e100)));
-	if(fast)
+	if(__builtin_expect(fast, 1))
 	{
 		const DmRecip R = dm_recip_shared(r);
 		const double jj = p.J * p.J;
@@ -371,7 +371,7 @@ DMS_HD int dms_rk45_attempt(DmsProp& p, const DmsSolarView& S, int inner_iter)
 		return 1;
 	}
 	const double abs_min_step = 1.0e-8 * DMS_sec;
-	if(h_new < abs_min_step || inner_iter >= DMS_RK45_MAX_INNER_RETRIES)
+	if(__builtin_expect(h_new < abs_min_step || inner_iter >= DMS_RK45_MAX_INNER_RETRIES, 0))
 	{
 		const bool state_ok = dm_isfinite(r4) && dm_isfinite(v4) && dm_isfinite(p4);
 		p.time = p.time + abs_min_step;
@@ -959,16 +959,16 @@ DMS_HD int dms_segment_start(DmsLane& L, const DmsSolarView& S, double dm_mass, 
 template <bool MX = false>
 DMS_HD int dms_step_prep(DmsLane& L, const DmsPointView& P, const DmsSolarView& S, const DmsJobView& J)
 {
-	if(!(L.time_steps < J.max_time_steps))
+	if(__builtin_expect(!(L.time_steps < J.max_time_steps), 0))
 		return DMS_MAX_FREE_STEPS;
-	if(J.rk_budget > 0 && L.rk_all > J.rk_budget)
+	if(__builtin_expect(J.rk_budget > 0 && L.rk_all > J.rk_budget, 0))
 	{
 		double t;
 		dms_prop_event(L.prop, t, L.ev_r, L.ev_v);
 		L.ev_t = t + L.time_origin;
 		return DMS_WALL_TIME_LIMIT;
 	}
-	if(!L.eb_valid)
+	if(__builtin_expect(!L.eb_valid, 0)) /* first step of a segment */
 	{
 		dms_prop_event(L.prop, L.eb_t, L.eb_r, L.eb_v);
 		L.r_before = dms_prop_radius(L.prop);
@@ -1004,7 +1004,7 @@ DMS_HD int dms_step_prep(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 			const float rb = L.rb_cached ? L.rate_before_f
 			                             : dms_rate_total_mx(P, S, (float) (L.r_before * DMS_MX_INV_RSUN), (float) L.v_before);
 			L.rate_before_f = rb;
-			if(!mx_isfinite(rb) || rb < 0.0f)
+			if(__builtin_expect(!mx_isfinite(rb) || rb < 0.0f, 0))
 			{
 				double t;
 				dms_prop_event(L.prop, t, L.ev_r, L.ev_v);
@@ -1017,7 +1017,7 @@ DMS_HD int dms_step_prep(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 		else
 		{
 			L.rate_before = L.rb_cached ? cached_rate : dms_rate_total(P, S, L.r_before, L.v_before);
-			if(!dm_isfinite(L.rate_before) || L.rate_before < 0.0)
+			if(__builtin_expect(!dm_isfinite(L.rate_before) || L.rate_before < 0.0, 0))
 			{
 				double t;
 				dms_prop_event(L.prop, t, L.ev_r, L.ev_v);

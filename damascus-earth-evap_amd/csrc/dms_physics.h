@@ -113,7 +113,7 @@ DMS_HD double dms_spline_at(const DmsSolarView& S, const DmsCubic* T, double x)
 /* Solar_Model::Mass, src/Solar_Model.cpp:168-179 */
 DMS_HD double dms_mass(const DmsSolarView& S, double r)
 {
-	if(!dm_isfinite(r))
+	if(__builtin_expect(!dm_isfinite(r), 0))
 		return dm_nan();
 	if(r < 0.0)
 		r = 0.0;
