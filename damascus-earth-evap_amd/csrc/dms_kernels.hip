@@ -30,6 +30,9 @@ __device__ inline double readlane_d(double x, int l)
 
 __device__ inline double shfl_d(double x, int src) { return __shfl(x, src); }
 
+/* lane l's value (l uniform over the wave): one v_readlane, no LDS round trip */
+__device__ inline float readlane_f(float x, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l)); }
+
 } // namespace
 
 template <bool MX, bool MP>
@@ -99,20 +102,20 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 				const bool bad = (k >= 0) && (!mx_isfinite(rate) || rate < 0.0f);
 				if(__ballot(bad) == 0)
 				{
-					float total = __shfl(rate, DMS_MAX_TARGETS - 1);
+					float total = readlane_f(rate, DMS_MAX_TARGETS - 1);
 					for(int i = 0; i < n_iso; i++)
-						total += __shfl(rate, i);
+						total += readlane_f(rate, i);
 					if(mx_isfinite(total) && total > 0.0f)
 					{
 						const float q = rate * mx_rcp(total);
 						const float xf = (float) xi;
-						float sum = __shfl(q, DMS_MAX_TARGETS - 1);
+						float sum = readlane_f(q, DMS_MAX_TARGETS - 1);
 						if(sum > xf)
 							target = -1;
 						else
 							for(int i = 0; i < n_iso; i++)
 							{
-								sum += __shfl(q, i);
+								sum += readlane_f(q, i);
 								if(sum > xf || i == n_iso - 1)
 								{
 									target = i;
