@@ -28,8 +28,6 @@ __device__ inline double readlane_d(double x, int l)
 	return __hiloint2double(hi, lo);
 }
 
-__device__ inline double shfl_d(double x, int src) { return __shfl(x, src); }
-
 /* lane l's value (l uniform over the wave): one v_readlane, no LDS round trip */
 __device__ inline float readlane_f(float x, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l)); }
 
@@ -86,13 +84,13 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 		{
 			const int s = __ffsll((long long) m) - 1;
 			m &= m - 1;
-			const double r = shfl_d(L.sc_r, s);
-			const double v = shfl_d(L.sc_v, s);
-			const double xi = shfl_d(L.sc_xi, s);
+			const double r = readlane_d(L.sc_r, s);
+			const double v = readlane_d(L.sc_v, s);
+			const double xi = readlane_d(L.sc_xi, s);
 			const int k = (lane < n_iso) ? lane : ((lane == DMS_MAX_TARGETS - 1) ? lane : -1);
 			int target = -2;
 			/* the scattering lane's parameter point (uniform over the wave) */
-			const DmsPointView& Ps = MP ? a.job_points[__shfl((int) L.job, s)] : a.P;
+			const DmsPointView& Ps = MP ? a.job_points[__builtin_amdgcn_readlane((int) L.job, s)] : a.P;
 			if constexpr(MX)
 			{
 				/* FP32 rates (1/s), same draw and order as below */
@@ -191,10 +189,10 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 			{
 				const int s = __ffsll((long long) fm) - 1;
 				fm &= fm - 1;
-				const uint32_t row_s = (uint32_t) __shfl((int) L.row, s);
-				const uint32_t job_s = (uint32_t) __shfl((int) L.job, s);
-				const uint32_t n_s = (uint32_t) __shfl((int) L.n_touched, s);
-				const int cls_s = __shfl(flush_cls, s);
+				const uint32_t row_s = (uint32_t) __builtin_amdgcn_readlane((int) L.row, s);
+				const uint32_t job_s = (uint32_t) __builtin_amdgcn_readlane((int) L.job, s);
+				const uint32_t n_s = (uint32_t) __builtin_amdgcn_readlane((int) L.n_touched, s);
+				const int cls_s = __builtin_amdgcn_readlane(flush_cls, s);
 				dms_flush_rows(c.R, c.J.nbins, row_s, job_s, n_s, cls_s, (uint32_t) lane, 64u);
 			}
 			if(flush_cls >= 0)
@@ -214,8 +212,8 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 				if(a.fill_ctl)
 					limit = __hip_atomic_load(a.fill_ctl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 			}
-			base = __shfl(base, leader);
-			limit = __shfl(limit, leader);
+			base = (unsigned int) __builtin_amdgcn_readlane((int) base, leader);
+			limit = (unsigned int) __builtin_amdgcn_readlane((int) limit, leader);
 			if(L.phase == PH_NEED)
 			{
 				const unsigned int rank = (unsigned int) __popcll(nmask & ((1ull << lane) - 1ull));
