@@ -260,18 +260,52 @@ DMS_HD double dms_dphi_dt(const DmsProp& p, double r)
 	return p.J / r / r;
 }
 
-/* dms_dv_dt and (want_phi) dms_dphi_dt of one RKF45 stage.  On the device the
- * seven divisions by r share one reciprocal (dm_div_shared) when every
- * dividend and r lie in its exact domain: r in [1 km, 1e40], |J| zero or in
- * [1e-30, 1e50] (j_ok, checked once per attempt) and G M zero or in
- * [1e-60, 1e100] keep every dividend of the chains in [1e-180, 1e100];
- * otherwise, and on the host, the reference's divisions. */
-DMS_HD void dms_stage_derivs(const DmsProp& p, double r, double mass, bool j_ok, bool want_phi, double& dv, double& dphi)
+/* x in [2^lo, 2^hi) for exponents of normal numbers lo < hi: one unsigned
+ * compare of the high word, false for negative, NaN and infinite x. */
+DMS_HD bool dms_in_binades(double x, int lo, int hi)
+{
+	const uint32_t w = (uint32_t) (dm_bits(x) >> 32);
+	const uint32_t a = (uint32_t) (lo + 1023) << 20, b = (uint32_t) (hi + 1023) << 20;
+	return w - a < b - a;
+}
+
+/* Mass(r) (dms_mass), dms_dv_dt and (want_phi) dms_dphi_dt of one RKF45
+ * stage.  On the device the common case is Mass() on the interval guess of
+ * dms_spline_at (inside the Sun; M_sun outside) and the seven divisions by r
+ * sharing one reciprocal (dm_div_shared), exact when every dividend and r lie
+ * in its domain: r in [1 km, 1e40], |J| zero or in [1e-30, 1e50] (j_ok,
+ * checked once per attempt) and G M zero or in [1e-60, 1e100] keep every
+ * dividend of the chains in [1e-180, 1e100] (the tests use power-of-two
+ * bounds inside these).  A missed interval guess or operands outside that
+ * domain recompute the stage the reference's way in one rarely taken branch;
+ * the host (CPU replay of the kernel code) always does. */
+DMS_HD void dms_stage(const DmsProp& p, const DmsSolarView& S, double r, bool j_ok, bool want_phi, double& dv, double& dphi)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
+	const double rm = (r < 0.0) ? 0.0 : r;
+	const bool finite = dm_isfinite(r);
+	double mass = finite ? DMS_mSun : dm_nan();
+	bool guess_ok = true;
+	if(finite & (rm <= DMS_rSun)) /* outside the Sun Mass() is M_sun: no table read */
+	{
+		const int n = S.n;
+		const double t = (rm - S.x1) * S.inv_h;
+		/* dms_spline_at's guess (0 for t < 0, n - 2 for t >= n - 2) with the
+		 * conversion kept in range: t clamped to [-1, n] */
+		const double tc = (t >= 0.0) ? dm_min(t, (double) n) : -1.0;
+		const int jt = (int) tc + 1;
+		const int j = (jt > n - 2) ? n - 2 : jt;
+		const double xj = S.x[j];
+		const double xn = S.x[j + 1];
+		const DmsCubic c = S.mass[j];
+		mass = 1.0 * dms_cubic(c, rm - xj);
+		guess_ok = (j == n - 2 || !(xn <= rm)) & (j == 0 || !(xj > rm));
+	}
 	const double gm = DMS_G_Newton * mass;
-	const bool fast = j_ok & (r >= 1.0 * DMS_km) & (r <= 1.0e40) & ((gm == 0.0) | ((gm >= 1.0e-60) & (gm <= 1.0e100)));
-	if(__builtin_expect(fast, 1))
+	/* [2^63, 2^132) lies inside [1 km, 1e40] (1 km = 2^62.14) and
+	 * [2^-199, 2^332) inside [1e-60, 1e100] */
+	const bool fast = j_ok & dms_in_binades(r, 63, 132) & ((gm == 0.0) | dms_in_binades(gm, -199, 332));
+	if(__builtin_expect(guess_ok & fast, 1))
 	{
 		const DmRecip R = dm_recip_shared(r);
 		const double jj = p.J * p.J;
@@ -281,7 +315,8 @@ DMS_HD void dms_stage_derivs(const DmsProp& p, double r, double mass, bool j_ok,
 		return;
 	}
 #endif
-	dv = dms_dv_dt(p, r, mass);
+	const double m = dms_mass(S, r);
+	dv = dms_dv_dt(p, r, m);
 	if(want_phi)
 		dphi = dms_dphi_dt(p, r);
 }
@@ -294,39 +329,39 @@ DMS_HD int dms_rk45_attempt(DmsProp& p, const DmsSolarView& S, int inner_iter)
 	const double h = p.ts;
 	double kr[6], kv[6], kp[6], ri;
 	const double aJ = dm_fabs(p.J);
-	const bool j_ok = (p.J == 0.0) | ((aJ >= 1.0e-30) & (aJ <= 1.0e50));
+	const bool j_ok = (p.J == 0.0) | dms_in_binades(aJ, -99, 166); /* inside [1e-30, 1e50] */
 	double dv, dph = 0.0;
 	kr[0] = h * p.vr;
-	dms_stage_derivs(p, p.radius, dms_mass(S, p.radius), j_ok, true, dv, dph);
+	dms_stage(p, S, p.radius, j_ok, true, dv, dph);
 	kv[0] = h * dv;
 	kp[0] = h * dph;
 
 	ri = p.radius + kr[0] / 4.0;
 	kr[1] = h * (p.vr + kv[0] / 4.0);
-	dms_stage_derivs(p, ri, dms_mass(S, ri), j_ok, false, dv, dph);
+	dms_stage(p, S, ri, j_ok, false, dv, dph);
 	kv[1] = h * dv;
 
 	ri = p.radius + 3.0 / 32.0 * kr[0] + 9.0 / 32.0 * kr[1];
 	kr[2] = h * (p.vr + 3.0 / 32.0 * kv[0] + 9.0 / 32.0 * kv[1]);
-	dms_stage_derivs(p, ri, dms_mass(S, ri), j_ok, true, dv, dph);
+	dms_stage(p, S, ri, j_ok, true, dv, dph);
 	kv[2] = h * dv;
 	kp[2] = h * dph;
 
 	ri = p.radius + 1932.0 / 2197.0 * kr[0] - 7200.0 / 2197.0 * kr[1] + 7296.0 / 2197.0 * kr[2];
 	kr[3] = h * (p.vr + 1932.0 / 2197.0 * kv[0] - 7200.0 / 2197.0 * kv[1] + 7296.0 / 2197.0 * kv[2]);
-	dms_stage_derivs(p, ri, dms_mass(S, ri), j_ok, true, dv, dph);
+	dms_stage(p, S, ri, j_ok, true, dv, dph);
 	kv[3] = h * dv;
 	kp[3] = h * dph;
 
 	ri = p.radius + 439.0 / 216.0 * kr[0] - 8.0 * kr[1] + 3680.0 / 513.0 * kr[2] - 845.0 / 4104.0 * kr[3];
 	kr[4] = h * (p.vr + 439.0 / 216.0 * kv[0] - 8.0 * kv[1] + 3680.0 / 513.0 * kv[2] - 845.0 / 4104.0 * kv[3]);
-	dms_stage_derivs(p, ri, dms_mass(S, ri), j_ok, true, dv, dph);
+	dms_stage(p, S, ri, j_ok, true, dv, dph);
 	kv[4] = h * dv;
 	kp[4] = h * dph;
 
 	ri = p.radius - 8.0 / 27.0 * kr[0] + 2.0 * kr[1] - 3544.0 / 2565.0 * kr[2] + 1859.0 / 4104.0 * kr[3] - 11.0 / 40.0 * kr[4];
 	kr[5] = h * (p.vr - 8.0 / 27.0 * kv[0] + 2.0 * kv[1] - 3544.0 / 2565.0 * kv[2] + 1859.0 / 4104.0 * kv[3] - 11.0 / 40.0 * kv[4]);
-	dms_stage_derivs(p, ri, dms_mass(S, ri), j_ok, true, dv, dph);
+	dms_stage(p, S, ri, j_ok, true, dv, dph);
 	kv[5] = h * dv;
 	kp[5] = h * dph;
 
