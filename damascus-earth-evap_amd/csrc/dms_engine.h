@@ -657,117 +657,154 @@ DMS_HD double dms_rate_at_fraction(const DmsPointView& P, const DmsSolarView& S,
 	return dms_rate_total(P, S, radius, dm_sqrt(dm_max(0.0, vs)));
 }
 
+/* One element of dms_rates_at_fractions up to its table read: radius and
+ * speed (at fraction f of the step, or (xr, xv) for the end-point query), the
+ * lookup cell, and which evaluation it needs (grid: the table; comp: the
+ * direct 64-target sum); res holds the value of an element that needs
+ * neither.  The arithmetic is that of dms_rate_at_fraction / dms_rate_total. */
+struct DmsRateSlot
+{
+	double res, rad, spd, rf, vf;
+	unsigned lo;
+	bool grid, comp;
+};
+
+DMS_HD DmsRateSlot dms_rate_slot(const DmsPointView& P, bool use_grid, bool active, bool at_fraction, const double* br,
+                                 const double* bv, const double* ar, const double* av, double f, double xr, double xv)
+{
+	DmsRateSlot q;
+	double r, v;
+	bool outside = false; /* Scattering_Rate_At_Fraction: radius >= R_sun gives 0 */
+	if(!at_fraction)
+	{
+		r = xr;
+		v = xv;
+	}
+	else
+	{
+		f = dm_clamp_unit(f);
+		double rs = 0.0, vs = 0.0;
+		for(int c = 0; c < 3; c++)
+		{
+			const double pos = br[c] + f * (ar[c] - br[c]);
+			const double vel = bv[c] + f * (av[c] - bv[c]);
+			rs += pos * pos;
+			vs += vel * vel;
+		}
+		r = dm_sqrt(dm_max(0.0, rs));
+		v = dm_sqrt(dm_max(0.0, vs));
+		outside = r >= DMS_rSun;
+	}
+	/* Total_DM_Scattering_Rate */
+	const bool finite = dm_isfinite(r) && dm_isfinite(v);
+	r = (r < 0.0) ? 0.0 : r;
+	v = (v < 0.0) ? 0.0 : v;
+	q.rad = r;
+	q.spd = v;
+	const bool table = use_grid && v <= P.vmax;
+	/* Total_DM_Scattering_Rate_Interpolated */
+	const bool zero_interp = table && (r > DMS_rSun || r < 0.0 || v < 0.0);
+	const double rc = r * P.inv_dr;
+	const double vc = v * P.inv_dv;
+	const bool coords_ok = dm_isfinite(rc) && dm_isfinite(vc) && !(rc < 0.0) && !(vc < 0.0);
+	const bool live = active && !outside && finite;
+	q.grid = live && table && !zero_interp && coords_ok;
+	q.comp = live && !zero_interp && !(table && coords_ok);
+	const double rcs = coords_ok ? dm_min(rc, (double) (P.Nr - 2)) : 0.0;
+	const double vcs = coords_ok ? dm_min(vc, (double) (P.Nv - 2)) : 0.0;
+	const unsigned ri = (unsigned) rcs;
+	const unsigned vi = (unsigned) vcs;
+	q.rf = rc - ri;
+	q.vf = vc - vi;
+	q.lo = q.grid ? ri * (unsigned) P.Nv + vi : 0u;
+	q.res = (!finite && !outside) ? dm_nan() : 0.0;
+	return q;
+}
+
+/* The element's value from the four table nodes of its cell (lookup slots)
+ * or the value it already holds (the others). */
+DMS_HD double dms_rate_slot_bilinear(const DmsRateSlot& q, const double* g)
+{
+	const double lower = g[0] + q.vf * (g[1] - g[0]);
+	const double upper = g[2] + q.vf * (g[3] - g[2]);
+	const double val = lower + q.rf * (upper - lower);
+	const double gv = dm_isfinite(val) ? dm_max(0.0, val) : dm_nan();
+	return q.grid ? gv : q.res;
+}
+
 /* Scattering_Rate_At_Fraction (:327-345) for n fractions of one step, plus
  * optionally Total_DM_Scattering_Rate (src/Solar_Model.cpp:305-322) at one more
  * (xr, xv) -- the rate the next step's preparation needs at the end point.
- * Per element the arithmetic is that of dms_rate_at_fraction / dms_rate_total;
- * the bilinear grid reads of all elements are issued before any is used. */
+ * Per element the arithmetic is that of dms_rate_at_fraction / dms_rate_total. */
 DMS_HD void dms_rates_at_fractions(const DmsPointView& P, const DmsSolarView& S, const double* br, const double* bv,
                                    const double* ar, const double* av, const double* fr, int n, bool use_cached,
                                    double cached_rate, double* out, bool extra, double xr, double xv, double* xout)
 {
-	/* Written without branches on the common path (selects only), so that the
-	 * slots' dependent chains interleave in one instruction stream; slots that
-	 * need the direct 64-target sum (no table, v > v_max, non-finite table
+	/* Elements 1..n-1 and the end-point query are written without branches
+	 * (selects only), their table reads issued together, so that their
+	 * dependent chains interleave in one instruction stream.  Element 0, the
+	 * step's start point, usually has the rate the step preparation used
+	 * (use_cached) and is evaluated only when it has not.  Elements that need
+	 * the direct 64-target sum (no table, v > v_max, non-finite table
 	 * coordinates) take a separate branch afterwards. */
 	constexpr int NQ = DMS_MAX_OPTICAL_DEPTH_PIECES + 2; /* fractions, then the extra query */
-	double res[NQ], rad[NQ], spd[NQ], rf[NQ], vf[NQ];
-	unsigned lo[NQ];
-	bool grid[NQ], comp[NQ];
 	const bool use_grid = P.use_grid != 0;
+	const unsigned nv = (unsigned) P.Nv;
+	DmsRateSlot q[NQ];
 #pragma unroll
-	for(int i = 0; i < NQ; i++)
+	for(int i = 1; i < NQ; i++)
 	{
 		const bool is_extra = (i == NQ - 1);
-		const bool active = is_extra ? extra : (i < n);
-		const bool cached = !is_extra && i == 0 && use_cached;
-		double r, v;
-		bool outside = false; /* Scattering_Rate_At_Fraction: radius >= R_sun gives 0 */
-		if(is_extra)
-		{
-			r = xr;
-			v = xv;
-		}
-		else
-		{
-			const double f = dm_clamp_unit(fr[i < n ? i : 0]);
-			double rs = 0.0, vs = 0.0;
-			for(int c = 0; c < 3; c++)
-			{
-				const double pos = br[c] + f * (ar[c] - br[c]);
-				const double vel = bv[c] + f * (av[c] - bv[c]);
-				rs += pos * pos;
-				vs += vel * vel;
-			}
-			r = dm_sqrt(dm_max(0.0, rs));
-			v = dm_sqrt(dm_max(0.0, vs));
-			outside = r >= DMS_rSun;
-		}
-		/* Total_DM_Scattering_Rate */
-		const bool finite = dm_isfinite(r) && dm_isfinite(v);
-		r = (r < 0.0) ? 0.0 : r;
-		v = (v < 0.0) ? 0.0 : v;
-		rad[i] = r;
-		spd[i] = v;
-		const bool table = use_grid && v <= P.vmax;
-		/* Total_DM_Scattering_Rate_Interpolated */
-		const bool zero_interp = table && (r > DMS_rSun || r < 0.0 || v < 0.0);
-		const double rc = r * P.inv_dr;
-		const double vc = v * P.inv_dv;
-		const bool coords_ok = dm_isfinite(rc) && dm_isfinite(vc) && !(rc < 0.0) && !(vc < 0.0);
-		const bool live = active && !cached && !outside && finite;
-		grid[i] = live && table && !zero_interp && coords_ok;
-		comp[i] = live && !zero_interp && !(table && coords_ok);
-		const double rcs = coords_ok ? dm_min(rc, (double) (P.Nr - 2)) : 0.0;
-		const double vcs = coords_ok ? dm_min(vc, (double) (P.Nv - 2)) : 0.0;
-		const unsigned ri = (unsigned) rcs;
-		const unsigned vi = (unsigned) vcs;
-		rf[i] = rc - ri;
-		vf[i] = vc - vi;
-		lo[i] = grid[i] ? ri * (unsigned) P.Nv + vi : 0u;
-		res[i] = cached ? cached_rate : ((!finite && !outside) ? dm_nan() : 0.0);
+		q[i] = dms_rate_slot(P, use_grid, is_extra ? extra : (i < n), !is_extra, br, bv, ar, av, fr[i < n ? i : 0], xr, xv);
 	}
 	if(use_grid)
 	{
-		/* all table reads in one batch (slots without a lookup read node 0) */
-		const unsigned nv = (unsigned) P.Nv;
 		double g[NQ][4];
 #pragma unroll
-		for(int i = 0; i < NQ; i++)
+		for(int i = 1; i < NQ; i++)
 		{
-			const double* q = P.grid + lo[i];
-			g[i][0] = q[0];
-			g[i][1] = q[1];
-			g[i][2] = q[nv];
-			g[i][3] = q[nv + 1];
+			const double* t = P.grid + q[i].lo;
+			g[i][0] = t[0];
+			g[i][1] = t[1];
+			g[i][2] = t[nv];
+			g[i][3] = t[nv + 1];
 		}
 #pragma unroll
-		for(int i = 0; i < NQ; i++)
+		for(int i = 1; i < NQ; i++)
+			q[i].res = dms_rate_slot_bilinear(q[i], g[i]);
+	}
+	if(use_cached)
+	{
+		q[0].res = cached_rate;
+		q[0].comp = false;
+	}
+	else
+	{
+		q[0] = dms_rate_slot(P, use_grid, 0 < n, true, br, bv, ar, av, fr[0], xr, xv);
+		if(use_grid)
 		{
-			const double lower = g[i][0] + vf[i] * (g[i][1] - g[i][0]);
-			const double upper = g[i][2] + vf[i] * (g[i][3] - g[i][2]);
-			const double val = lower + rf[i] * (upper - lower);
-			const double gv = dm_isfinite(val) ? dm_max(0.0, val) : dm_nan();
-			res[i] = grid[i] ? gv : res[i];
+			const double* t = P.grid + q[0].lo;
+			const double g0[4] = {t[0], t[1], t[nv], t[nv + 1]};
+			q[0].res = dms_rate_slot_bilinear(q[0], g0);
 		}
 	}
 	bool any_comp = false;
 #pragma unroll
 	for(int i = 0; i < NQ; i++)
-		any_comp = any_comp || comp[i];
+		any_comp = any_comp || q[i].comp;
 	if(__builtin_expect(any_comp, 0)) /* no table, or v above it: rare */
 	{
 #pragma unroll
 		for(int i = 0; i < NQ; i++)
-			if(comp[i])
-				res[i] = dms_rate_computed(P, S, rad[i], spd[i]);
+			if(q[i].comp)
+				q[i].res = dms_rate_computed(P, S, q[i].rad, q[i].spd);
 	}
 #pragma unroll
 	for(int i = 0; i < NQ - 1; i++)
 		if(i < n)
-			out[i] = res[i];
+			out[i] = q[i].res;
 	if(extra)
-		*xout = res[NQ - 1];
+		*xout = q[NQ - 1].res;
 }
 
 /* Solve_Linear_Rate_Optical_Depth_Fraction, :416-464 */
