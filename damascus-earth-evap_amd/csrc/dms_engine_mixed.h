@@ -221,96 +221,129 @@ DMS_HD float dms_rate_total_mx(const DmsPointView& P, const DmsSolarView& S, flo
 	return mx_isfinite(val) ? fmaxf(0.0f, val) : __builtin_nanf("");
 }
 
+/* One element of dms_rates_at_fractions_mx up to its table read (FP32, radius
+ * in R_sun): radius and speed at fraction f of the step (b + f d) or the
+ * end-point query (xr, xv), the lookup cell, and which evaluation it needs. */
+struct DmsRateSlotMx
+{
+	float res, rad, spd, rf, vf;
+	unsigned lo;
+	bool grid, comp;
+};
+
+DMS_HD DmsRateSlotMx dms_rate_slot_mx(const DmsPointView& P, bool use_grid, bool active, bool at_fraction, const float* b,
+                                      const float* d, float f, float xr, float xv)
+{
+	DmsRateSlotMx q;
+	float r, v;
+	bool outside = false;
+	if(!at_fraction)
+	{
+		r = xr;
+		v = xv;
+	}
+	else
+	{
+		f = fminf(1.0f, fmaxf(0.0f, f));
+		const float p0 = b[0] + f * d[0], p1 = b[1] + f * d[1], p2 = b[2] + f * d[2];
+		const float q0 = b[3] + f * d[3], q1 = b[4] + f * d[4], q2 = b[5] + f * d[5];
+		r = mx_sqrt(p0 * p0 + p1 * p1 + p2 * p2);
+		v = mx_sqrt(q0 * q0 + q1 * q1 + q2 * q2);
+		outside = r >= 1.0f;
+	}
+	const bool finite = mx_isfinite(r) && mx_isfinite(v);
+	q.rad = r;
+	q.spd = v;
+	const bool table = use_grid && v <= (float) P.vmax;
+	const bool live = active && !outside && finite && !(r > 1.0f);
+	q.grid = live && table;
+	q.comp = live && !table;
+	const float rc = r * (float) (P.Nr - 1);
+	const float vc = v * (float) P.inv_dv;
+	const unsigned ri = q.grid ? (unsigned) fminf(rc, (float) (P.Nr - 2)) : 0u;
+	const unsigned vi = q.grid ? (unsigned) fminf(vc, (float) (P.Nv - 2)) : 0u;
+	q.rf = rc - (float) ri;
+	q.vf = vc - (float) vi;
+	q.lo = ri * (unsigned) P.Nv + vi;
+	q.res = (!finite && !outside) ? __builtin_nanf("") : 0.0f;
+	return q;
+}
+
+DMS_HD float dms_rate_slot_bilinear_mx(const DmsRateSlotMx& q, const float* g)
+{
+	const float lower = g[0] + q.vf * (g[1] - g[0]);
+	const float upper = g[2] + q.vf * (g[3] - g[2]);
+	const float val = lower + q.rf * (upper - lower);
+	return q.grid ? (mx_isfinite(val) ? fmaxf(0.0f, val) : __builtin_nanf("")) : q.res;
+}
+
 /* Scattering_Rate_At_Fraction for n fractions and the next step's rate at the
- * end point (xr, xv), FP32, one batch of table reads. */
+ * end point (xr, xv), FP32.  Elements 1..n-1 and the end-point query take
+ * one batch of table reads; element 0, the step's start point, is evaluated
+ * only when its cached rate is not available (use_cached). */
 DMS_HD void dms_rates_at_fractions_mx(const DmsPointView& P, const DmsSolarView& S, const float* b, const float* a,
                                       const float* fr, int n, bool use_cached, float cached_rate, float* out, bool extra,
                                       float xr, float xv, float* xout)
 {
 	constexpr int NQ = DMS_MAX_OPTICAL_DEPTH_PIECES + 2;
-	float rad[NQ], spd[NQ], res[NQ];
-	bool grid[NQ], comp[NQ];
-	unsigned lo[NQ];
-	float rf[NQ], vf[NQ];
 	const bool use_grid = P.use_grid != 0;
+	const unsigned nv = (unsigned) P.Nv;
 	const float d[6] = {a[0] - b[0], a[1] - b[1], a[2] - b[2], a[3] - b[3], a[4] - b[4], a[5] - b[5]};
+	DmsRateSlotMx q[NQ];
 #pragma unroll
-	for(int i = 0; i < NQ; i++)
+	for(int i = 1; i < NQ; i++)
 	{
 		const bool is_extra = (i == NQ - 1);
-		const bool active = is_extra ? extra : (i < n);
-		const bool cached = !is_extra && i == 0 && use_cached;
-		float r, v;
-		bool outside = false;
-		if(is_extra)
-		{
-			r = xr;
-			v = xv;
-		}
-		else
-		{
-			const float f = fminf(1.0f, fmaxf(0.0f, fr[i < n ? i : 0]));
-			const float p0 = b[0] + f * d[0], p1 = b[1] + f * d[1], p2 = b[2] + f * d[2];
-			const float q0 = b[3] + f * d[3], q1 = b[4] + f * d[4], q2 = b[5] + f * d[5];
-			r = mx_sqrt(p0 * p0 + p1 * p1 + p2 * p2);
-			v = mx_sqrt(q0 * q0 + q1 * q1 + q2 * q2);
-			outside = r >= 1.0f;
-		}
-		const bool finite = mx_isfinite(r) && mx_isfinite(v);
-		rad[i] = r;
-		spd[i] = v;
-		const bool table = use_grid && v <= (float) P.vmax;
-		const bool live = active && !cached && !outside && finite && !(r > 1.0f);
-		grid[i] = live && table;
-		comp[i] = live && !table;
-		const float rc = r * (float) (P.Nr - 1);
-		const float vc = v * (float) P.inv_dv;
-		const unsigned ri = grid[i] ? (unsigned) fminf(rc, (float) (P.Nr - 2)) : 0u;
-		const unsigned vi = grid[i] ? (unsigned) fminf(vc, (float) (P.Nv - 2)) : 0u;
-		rf[i] = rc - (float) ri;
-		vf[i] = vc - (float) vi;
-		lo[i] = ri * (unsigned) P.Nv + vi;
-		res[i] = cached ? cached_rate : ((!finite && !outside) ? __builtin_nanf("") : 0.0f);
+		q[i] = dms_rate_slot_mx(P, use_grid, is_extra ? extra : (i < n), !is_extra, b, d, fr[i < n ? i : 0], xr, xv);
 	}
 	if(use_grid)
 	{
-		const unsigned nv = (unsigned) P.Nv;
 		float g[NQ][4];
 #pragma unroll
-		for(int i = 0; i < NQ; i++)
+		for(int i = 1; i < NQ; i++)
 		{
-			const float* q = P.grid32 + lo[i];
-			g[i][0] = q[0];
-			g[i][1] = q[1];
-			g[i][2] = q[nv];
-			g[i][3] = q[nv + 1];
+			const float* t = P.grid32 + q[i].lo;
+			g[i][0] = t[0];
+			g[i][1] = t[1];
+			g[i][2] = t[nv];
+			g[i][3] = t[nv + 1];
 		}
 #pragma unroll
-		for(int i = 0; i < NQ; i++)
+		for(int i = 1; i < NQ; i++)
+			q[i].res = dms_rate_slot_bilinear_mx(q[i], g[i]);
+	}
+	if(use_cached)
+	{
+		q[0].res = cached_rate;
+		q[0].comp = false;
+	}
+	else
+	{
+		q[0] = dms_rate_slot_mx(P, use_grid, 0 < n, true, b, d, fr[0], xr, xv);
+		if(use_grid)
 		{
-			const float lower = g[i][0] + vf[i] * (g[i][1] - g[i][0]);
-			const float upper = g[i][2] + vf[i] * (g[i][3] - g[i][2]);
-			const float val = lower + rf[i] * (upper - lower);
-			res[i] = grid[i] ? (mx_isfinite(val) ? fmaxf(0.0f, val) : __builtin_nanf("")) : res[i];
+			const float* t = P.grid32 + q[0].lo;
+			const float g0[4] = {t[0], t[1], t[nv], t[nv + 1]};
+			q[0].res = dms_rate_slot_bilinear_mx(q[0], g0);
 		}
 	}
 	bool any = false;
 #pragma unroll
 	for(int i = 0; i < NQ; i++)
-		any = any || comp[i];
+		any = any || q[i].comp;
 	if(__builtin_expect(any, 0)) /* no table, or v above it: rare */
 	{
 #pragma unroll
 		for(int i = 0; i < NQ; i++)
-			if(comp[i])
-				res[i] = dms_rate_computed_mx(P, S, (double) rad[i] * DMS_rSun, (double) spd[i]);
+			if(q[i].comp)
+				q[i].res = dms_rate_computed_mx(P, S, (double) q[i].rad * DMS_rSun, (double) q[i].spd);
 	}
 #pragma unroll
 	for(int i = 0; i < NQ - 1; i++)
 		if(i < n)
-			out[i] = res[i];
+			out[i] = q[i].res;
 	if(extra)
-		*xout = res[NQ - 1];
+		*xout = q[NQ - 1].res;
 }
 
 /* Solve_Linear_Rate_Optical_Depth_Fraction (:416-464) in FP32: the fraction at
