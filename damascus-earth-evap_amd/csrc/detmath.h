@@ -93,6 +93,21 @@ DM_FN double dm_div_shared(double a, const DmRecip& R)
 }
 #endif
 
+/* a / b for a constant b > 0 with y = 1.0 / b (the correctly rounded
+ * reciprocal, folded by the compiler): q = a y lies within one ulp of a / b,
+ * r = fma(-b, q, a) is exact, and fma(r, y, q) is the correctly rounded
+ * quotient (Markstein's theorem) where nothing overflows or underflows:
+ * |a| zero or in [2^-900, 2^900] with |a / b| in the same range.  Three
+ * operations instead of the division's eleven; infinite and NaN quotients
+ * pass through from q.  Callers pass no -0 (whose quotient would be +0). */
+DM_FN double dm_div_const(double a, double b, double y)
+{
+	const double q = a * y;
+	const double r = __builtin_fma(-b, q, a);
+	const double c = __builtin_fma(r, y, q);
+	return dm_isfinite(q) ? c : q;
+}
+
 DM_FN double dm_clamp_unit(double v) { return dm_max(0.0, dm_min(1.0, v)); }
 DM_FN double dm_clamp_cos(double v) { return dm_max(-1.0, dm_min(1.0, v)); }
 

@@ -381,11 +381,24 @@ DMS_HD int dms_rk45_attempt(DmsProp& p, const DmsSolarView& S, int inner_iter)
 		double factor = DMS_RK45_MAX_STEP_FACTOR;
 		const double ee[3] = {e0, e1, e2};
 		const double tt[3] = {tol0, tol1, tol2};
+		/* The reference takes the minimum of 0.84 (tol_i / e_i)^(1/4) over the
+		 * components with e_i > 0, skipping non-finite candidates.  Rounded
+		 * square roots and the product by 0.84 are monotone, so that minimum
+		 * is the candidate of the smallest quotient (in (0, inf], finite
+		 * exactly when its candidate is): one candidate instead of three. */
+		double q = 0.0;
+		bool any = false;
 		for(int i = 0; i < 3; i++)
 		{
 			if(ee[i] <= 0.0)
 				continue;
-			const double candidate = 0.84 * dm_pow_quarter(tt[i] / ee[i]);
+			const double qi = tt[i] / ee[i];
+			q = any ? dm_min(q, qi) : qi;
+			any = true;
+		}
+		if(any)
+		{
+			const double candidate = 0.84 * dm_pow_quarter(q);
 			if(dm_isfinite(candidate))
 				factor = dm_min(factor, candidate);
 		}
@@ -855,7 +868,7 @@ DMS_HD double dms_capture_energy_eV(const DmsSolarView& S, double dm_mass, doubl
 {
 	const double vesc = dms_local_escape_speed(S, radius);
 	const double E = 0.5 * dm_mass * (speed * speed - vesc * vesc);
-	return E / DMS_eV;
+	return dm_div_const(E, DMS_eV, 1.0 / DMS_eV);
 }
 
 /* true where the reference's abort_if_uncaptured_bound lambda fires (:767-778) */
@@ -881,7 +894,7 @@ DMS_HD int dms_update_capture_state(DmsLane& L, const DmsSolarView& S, double dm
 		dE = E - L.prev_cap_E;
 	L.prev_cap_E = E;
 	const int negative = E < 0.0;
-	const double t_now = time / DMS_sec;
+	const double t_now = dm_div_const(time, DMS_sec, 1.0 / DMS_sec);
 	if(!allow_new)
 	{
 		if(L.captured && dm_isfinite(L.ff_ref_E))
@@ -985,11 +998,11 @@ DMS_HD int dms_commit(DmsLane& L, const DmsRows& R, const DmsJobView& J, const D
                       double lt, const double* lr, const double* lv)
 {
 	const double abs_t = lt + L.time_origin;
-	const double t_now = lt / DMS_sec;
+	const double t_now = dm_div_const(lt, DMS_sec, 1.0 / DMS_sec);
 	if(!J.terminate_on_capture)
 	{
-		const double r_now = dms_norm(lr) / DMS_km;
-		const double v_now = dms_norm(lv) / (DMS_km / DMS_sec);
+		const double r_now = dm_div_const(dms_norm(lr), DMS_km, 1.0 / DMS_km);
+		const double v_now = dm_div_const(dms_norm(lv), DMS_km / DMS_sec, 1.0 / (DMS_km / DMS_sec));
 		const double v2 = v_now * v_now;
 		if(L.prev_time_sec >= 0.0)
 		{

@@ -399,6 +399,7 @@ __global__ void dms_probe_kernel(int fn, const double* x, const double* y, doubl
 		case 5: r = dm_cos(a); break;
 		case 6: r = dm_erf(a); break;
 		case 8: r = dm_div_shared(a, dm_recip_shared(y[i])); break;
+		case 9: r = dm_div_const(a, y[i], 1.0 / y[i]); break;
 		default: r = dm_acos(a); break;
 	}
 	out[i] = r;

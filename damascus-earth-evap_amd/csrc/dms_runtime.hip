@@ -1771,7 +1771,7 @@ void dms_job_destroy(dms_job* job)
 
 int dms_probe_math(dms_ctx* ctx, int fn, const double* x, const double* y, double* out, size_t n)
 {
-	if(ctx == nullptr || (n > 0 && (x == nullptr || out == nullptr)) || fn < 0 || fn > 8)
+	if(ctx == nullptr || (n > 0 && (x == nullptr || out == nullptr)) || fn < 0 || fn > 9)
 		return fail(ctx, DMS_ERR_ARG, "dms_probe_math: bad argument");
 	if(ctx->device < 0)
 		return fail(ctx, DMS_ERR_DEVICE, "dms_probe_math: context has no GPU");

@@ -336,7 +336,8 @@ void dms_job_destroy(dms_job* job);
 
 /* Evaluate the deterministic elementary functions on the device for n inputs.
  * fn: 0 sqrt, 1 div (x/y), 2 exp, 3 log, 4 sin, 5 cos, 6 erf, 7 acos,
- * 8 x/y by the shared-reciprocal division of the RKF45 stages. */
+ * 8 x/y by the shared-reciprocal division of the RKF45 stages, 9 x/y by the
+ * constant-divisor division (reciprocal 1/y, product, two FMAs). */
 int dms_probe_math(dms_ctx* ctx, int fn, const double* x, const double* y, double* out, size_t n);
 
 #ifdef __cplusplus

@@ -349,4 +349,12 @@ int rp_simulate(rp_point* p, const dms_job_desc* job, const dms_event* ics, size
 	return 0;
 }
 
+/* dm_div_const over n dividends for one divisor b and reciprocal y (the
+ * engine passes 1.0 / b). */
+void rp_div_const(const double* a, size_t n, double b, double y, double* out)
+{
+	for(size_t i = 0; i < n; i++)
+		out[i] = dm_div_const(a[i], b, y);
+}
+
 } /* extern "C" */

@@ -47,6 +47,7 @@ def lib():
                                        C.c_size_t, _P(C.c_double), _P(C.c_uint64)]),
             "rp_simulate": (C.c_int, [vp, _P(dmsun.JobDesc), _P(dmsun.Event), C.c_size_t, _P(dmsun.TrajRecord),
                                       _P(C.c_double)]),
+            "rp_div_const": (None, [_P(C.c_double), C.c_size_t, C.c_double, C.c_double, _P(C.c_double)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)

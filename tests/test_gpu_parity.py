@@ -80,6 +80,28 @@ def test_shared_reciprocal_division_is_ieee(gpu_ctx):
     assert np.array_equal(dev.view(np.uint64), ref.view(np.uint64))
 
 
+def test_constant_divisor_division_is_ieee(gpu_ctx):
+    """dm_div_const on the device (reciprocal 1/b, product, two FMAs) gives the
+    bits of a / b for the unit divisors of the per-step bookkeeping (s, km,
+    km/s, eV; dms_units.h operation order) over |a| in [2^-800, 2^800], zero,
+    infinities and NaN."""
+    cm = 1.0 / 1.973269804e-14
+    meter = 100.0 * cm
+    km, sec = 1000.0 * meter, 299792458.0 * meter
+    rng = np.random.default_rng(12)
+    n = 200000
+    a = np.exp2(rng.uniform(-800, 800, n)) * rng.choice([-1.0, 1.0], n)
+    a[::101] = 0.0
+    edge = np.array([np.nextafter(2.0 ** k, 0.0) for k in range(-300, 300)] + [2.0 ** k for k in range(-300, 300)])
+    a = np.concatenate([a, edge, -edge, [np.inf, -np.inf, np.nan]])
+    for b in (sec, km, km / sec, 1.0e-9):
+        dev = gpu_ctx.probe_math(9, a, np.full_like(a, b))
+        ref = a / b
+        nan = np.isnan(ref)
+        assert np.array_equal(np.isnan(dev), nan)
+        assert np.array_equal(dev[~nan].view(np.uint64), ref[~nan].view(np.uint64)), b
+
+
 def host_detmath(fn, x, y):
     """Host evaluation of the same deterministic functions, via the oracle build."""
     import oracle_lib as O

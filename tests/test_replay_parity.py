@@ -86,3 +86,54 @@ def test_per_lane_histogram_mode_equals_job_sums(replay, oracle_solar):
     h, c = replay.point(desc).lane_hist(job, ev, ok, nlanes=5)
     assert np.array_equal(h, res["hist"]) and np.array_equal(c, res["counts"])
     assert c.sum() > 0
+
+
+# Unit divisors as the kernel's headers build them (dms_units.h), same operation order.
+UNIT_CM = 1.0 / 1.973269804e-14
+UNIT_METER = 100.0 * UNIT_CM
+UNIT_KM = 1000.0 * UNIT_METER
+UNIT_SEC = 299792458.0 * UNIT_METER
+UNIT_DIVISORS = (UNIT_SEC, UNIT_KM, UNIT_KM / UNIT_SEC, 1.0e-9)
+
+
+def constant_division_inputs(seed=5, n=200000):
+    """Dividends for dm_div_const: log-uniform over its stated domain with both
+    signs, zeros, mantissas next to powers of two, infinities and NaN."""
+    rng = np.random.default_rng(seed)
+    a = np.exp2(rng.uniform(-800, 800, n)) * rng.choice([-1.0, 1.0], n)
+    a[::101] = 0.0
+    edge = np.array([np.nextafter(2.0 ** k, 0.0) for k in range(-300, 300)] + [2.0 ** k for k in range(-300, 300)])
+    special = np.array([np.inf, -np.inf, np.nan, 1.0, -1.0])
+    return np.concatenate([a, edge, -edge, special])
+
+
+def div_const_host(a, b, y):
+    out = np.empty_like(a)
+    RL.lib().rp_div_const(dmsun.dptr(a), a.size, b, y, dmsun.dptr(out))
+    return out
+
+
+def same_bits_or_both_nan(x, ref):
+    nan = np.isnan(ref)
+    return np.array_equal(np.isnan(x), nan) and np.array_equal(x[~nan].view(np.uint64), ref[~nan].view(np.uint64))
+
+
+def test_constant_divisor_division_is_ieee():
+    """dm_div_const (product by the correctly rounded reciprocal, two FMAs) gives
+    the bits of a / b for the unit divisors the per-step bookkeeping divides by
+    (s, km, km/s, eV) over |a| in [2^-800, 2^800], zero, infinities and NaN."""
+    a = constant_division_inputs()
+    for b in UNIT_DIVISORS:
+        assert same_bits_or_both_nan(div_const_host(a, b, 1.0 / b), a / b), b
+
+
+def test_constant_division_check_catches_plausible_mistakes():
+    """The comparison above fails for the product a y without its correction
+    step and for a reciprocal rounded to single precision (one ulp of error in
+    the double reciprocal, by contrast, the correction step absorbs on these
+    inputs)."""
+    a = constant_division_inputs(seed=6, n=50000)
+    uncorrected = [same_bits_or_both_nan(a * (1.0 / b), a / b) for b in UNIT_DIVISORS]
+    single = [same_bits_or_both_nan(div_const_host(a, b, float(np.float32(1.0 / b))), a / b) for b in UNIT_DIVISORS[:3]]
+    assert not any(uncorrected)
+    assert not any(single)
