@@ -58,7 +58,8 @@ DMS_HD double dms_mass_mx(const DmsSolarView& S, double r)
 }
 
 /* One RKF45 attempt (Runge_Kutta_45_Step, src/Simulation_Trajectory.cpp:1487-1602)
- * with the right-hand side J^2/r^3 - G M(r)/r^2, J/r^2 from one reciprocal. */
+ * with the right-hand side J^2/r^3 - G M(r)/r^2, J/r^2 from one refined
+ * reciprocal per stage. */
 DMS_HD int dms_rk45_attempt_mx(DmsProp& p, const DmsSolarView& S, int inner_iter)
 {
 	const double tol0 = 1.0 * DMS_km, tol1 = 1.0e-3 * DMS_km / DMS_sec, tol2 = 1.0e-7;
@@ -67,13 +68,14 @@ DMS_HD int dms_rk45_attempt_mx(DmsProp& p, const DmsSolarView& S, int inner_iter
 	const bool angular = !(p.radial || p.J == 0.0);
 	double kr[6], kv[6], kp[6];
 	auto rhs = [&](double ri, double& dv, double& dphi) {
-		const double rc = (ri < 1.0 * DMS_km) ? 1.0 * DMS_km : ri;
-		const double q = 1.0 / rc;
+		/* one refined reciprocal (within an ulp) of the unclamped radius, which
+		 * dphi/dt uses; dv/dt clamps the radius to 1 km, whose reciprocal is a
+		 * constant */
+		const double qu = dm_recip_shared(ri).y;
+		const double q = (ri < 1.0 * DMS_km) ? 1.0 / DMS_km : qu;
 		const double m = dms_mass_mx(S, ri);
 		dv = (J2 * q - DMS_G_Newton * m) * (q * q);
-		/* dphi/dt uses the unclamped radius; below 1 km it is recomputed */
-		const double qp = (ri < 1.0 * DMS_km) ? 1.0 / ri : q;
-		dphi = (angular && ri > 0.0) ? p.J * (qp * qp) : 0.0;
+		dphi = (angular && ri > 0.0) ? p.J * (qu * qu) : 0.0;
 	};
 	double dv, dp;
 	rhs(p.radius, dv, dp);
@@ -115,10 +117,10 @@ DMS_HD int dms_rk45_attempt_mx(DmsProp& p, const DmsSolarView& S, int inner_iter
 	const bool finite = dm_isfinite(e0) && dm_isfinite(e1) && dm_isfinite(e2);
 	{
 		/* min over components of 0.84 (tol/err)^(1/4) = 0.84 (min tol/err)^(1/4) */
-		double ratio = 1.0e300;
-		ratio = (e0 > 0.0) ? dm_min(ratio, tol0 / e0) : ratio;
-		ratio = (e1 > 0.0) ? dm_min(ratio, tol1 / e1) : ratio;
-		ratio = (e2 > 0.0) ? dm_min(ratio, tol2 / e2) : ratio;
+		double ratio = 1.0e300; /* tol/err by refined reciprocals: an ulp does not matter here */
+		ratio = (e0 > 0.0) ? dm_min(ratio, tol0 * dm_recip_shared(e0).y) : ratio;
+		ratio = (e1 > 0.0) ? dm_min(ratio, tol1 * dm_recip_shared(e1).y) : ratio;
+		ratio = (e2 > 0.0) ? dm_min(ratio, tol2 * dm_recip_shared(e2).y) : ratio;
 		double factor = dm_min(DMS_RK45_MAX_STEP_FACTOR, 0.84 * dm_pow_quarter(ratio));
 		factor = dm_max(DMS_RK45_MIN_STEP_FACTOR, factor);
 		h_new = finite ? dms_rk45_sanitized(factor * h) : DMS_RK45_MIN_STEP_FACTOR * h;
@@ -395,7 +397,7 @@ DMS_HD double dms_prop_event_mx(const DmsProp& p, double& t, double* r, double* 
 	}
 	double sin_phi, cos_phi;
 	dm_sincos(p.phi, &sin_phi, &cos_phi);
-	const double tangential = p.J / safe_radius;
+	const double tangential = p.J * dm_recip_shared(safe_radius).y; /* refined reciprocal, within an ulp */
 	const double vx = p.vr * cos_phi - tangential * sin_phi;
 	const double vy = p.vr * sin_phi + tangential * cos_phi;
 	for(int c = 0; c < 3; c++)
@@ -406,12 +408,13 @@ DMS_HD double dms_prop_event_mx(const DmsProp& p, double& t, double* r, double* 
 	return dm_sqrt(p.vr * p.vr + tangential * tangential);
 }
 
-/* E = m (v^2 - v_esc^2) / 2 in eV from v^2, without the square root of v_esc^2. */
+/* E = m (v^2 - v_esc^2) / 2 in eV from v^2, without the square root of v_esc^2
+ * (outside the Sun 2 G M / r by a refined reciprocal). */
 DMS_HD double dms_capture_energy_mx(const DmsSolarView& S, double dm_mass, double radius, double speed2)
 {
 	double vesc2;
 	if(radius > DMS_rSun)
-		vesc2 = 2.0 * DMS_G_Newton * DMS_mSun / radius;
+		vesc2 = (2.0 * DMS_G_Newton * DMS_mSun) * dm_recip_shared(radius).y;
 	else
 	{
 		const double r = (radius < 0.0) ? 0.0 : radius;
@@ -450,7 +453,7 @@ DMS_HD void dms_commit_mx(DmsLane& L, const DmsRows& R, const DmsJobView& J, con
 		const double drift = dm_fabs(E - L.ff_ref_E);
 		if(dm_isfinite(drift) && drift > L.drift_eV)
 			L.drift_eV = drift;
-		const double rel = drift / dm_max(dm_fabs(L.ff_ref_E), DMS_FREE_ENERGY_DRIFT_REL_E_SCALE_EV);
+		const double rel = drift * dm_recip_shared(dm_max(dm_fabs(L.ff_ref_E), DMS_FREE_ENERGY_DRIFT_REL_E_SCALE_EV)).y;
 		if(dm_isfinite(rel) && rel > L.drift_rel)
 			L.drift_rel = rel;
 	}
