@@ -246,6 +246,46 @@ DMS_HD void dms_prop_event(const DmsProp& p, double& t, double* r, double* v)
 	}
 }
 
+/* x in [2^lo, 2^hi) for exponents of normal numbers lo < hi: one unsigned
+ * compare of the high word, false for negative, NaN and infinite x. */
+DMS_HD bool dms_in_binades(double x, int lo, int hi)
+{
+	const uint32_t w = (uint32_t) (dm_bits(x) >> 32);
+	const uint32_t a = (uint32_t) (lo + 1023) << 20, b = (uint32_t) (hi + 1023) << 20;
+	return w - a < b - a;
+}
+
+/* dms_prop_speed and dms_prop_event of one propagator state.  On the device,
+ * for an angular state with radius in [2^63, 2^132) (inside [1 km, 1e40]) and
+ * |J| in [2^-99, 2^165), the three divisions by the radius share one refined
+ * reciprocal (dm_div_shared: the dividends J^2, J^2 / r and J stay in its
+ * exact domain); otherwise, and on the host, the two functions. */
+DMS_HD double dms_prop_event_speed(const DmsProp& p, double& t, double* r, double* v)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+	if(__builtin_expect(!p.radial & dms_in_binades(p.radius, 63, 132) & dms_in_binades(dm_fabs(p.J), -99, 165), 1))
+	{
+		const DmRecip R = dm_recip_shared(p.radius);
+		const double speed = dm_sqrt(p.vr * p.vr + dm_div_shared(dm_div_shared(p.J * p.J, R), R));
+		t = p.time;
+		double sin_phi, cos_phi;
+		dm_sincos(p.phi, &sin_phi, &cos_phi);
+		const double tangential = dm_div_shared(p.J, R);
+		const double vx = p.vr * cos_phi - tangential * sin_phi;
+		const double vy = p.vr * sin_phi + tangential * cos_phi;
+		for(int c = 0; c < 3; c++)
+		{
+			r[c] = p.radius * (cos_phi * p.ax[c] + sin_phi * p.ay[c]);
+			v[c] = vx * p.ax[c] + vy * p.ay[c];
+		}
+		return speed;
+	}
+#endif
+	const double speed = dms_prop_speed(p);
+	dms_prop_event(p, t, r, v);
+	return speed;
+}
+
 DMS_HD double dms_dv_dt(const DmsProp& p, double r, double mass)
 {
 	if(r < 1.0 * DMS_km)
@@ -258,15 +298,6 @@ DMS_HD double dms_dphi_dt(const DmsProp& p, double r)
 	if(p.radial || p.J == 0.0 || r <= 0.0)
 		return 0.0;
 	return p.J / r / r;
-}
-
-/* x in [2^lo, 2^hi) for exponents of normal numbers lo < hi: one unsigned
- * compare of the high word, false for negative, NaN and infinite x. */
-DMS_HD bool dms_in_binades(double x, int lo, int hi)
-{
-	const uint32_t w = (uint32_t) (dm_bits(x) >> 32);
-	const uint32_t a = (uint32_t) (lo + 1023) << 20, b = (uint32_t) (hi + 1023) << 20;
-	return w - a < b - a;
 }
 
 /* Mass(r) (dms_mass), dms_dv_dt and (want_phi) dms_dphi_dt of one RKF45
@@ -1138,8 +1169,7 @@ DMS_HD int dms_step_post(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 		v_after = dms_prop_event_mx(L.prop, ea_t, ea_r, ea_v);
 	else
 	{
-		v_after = dms_prop_speed(L.prop);
-		dms_prop_event(L.prop, ea_t, ea_r, ea_v);
+		v_after = dms_prop_event_speed(L.prop, ea_t, ea_r, ea_v);
 	}
 #define DMS_RETURN_AT(tt, rr, vv, code) \
 	do                                  \
@@ -1163,6 +1193,10 @@ DMS_HD int dms_step_post(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 	const int fast = J.terminate_on_capture;
 	const double od_limit = fast ? DMS_CAPTURE_MODE_MAX_OPTICAL_DEPTH_STEP : DMS_MAX_OPTICAL_DEPTH_STEP;
 	const int od_count = fast ? DMS_CAPTURE_MODE_OPTICAL_DEPTH_PIECES : DMS_MAX_OPTICAL_DEPTH_PIECES;
+	static_assert((DMS_MAX_OPTICAL_DEPTH_PIECES & (DMS_MAX_OPTICAL_DEPTH_PIECES - 1)) == 0
+	                  && (DMS_CAPTURE_MODE_OPTICAL_DEPTH_PIECES & (DMS_CAPTURE_MODE_OPTICAL_DEPTH_PIECES - 1)) == 0,
+	              "piece counts are powers of two, so x / od_count == x * inv_pieces exactly");
+	const double inv_pieces = fast ? 1.0 / DMS_CAPTURE_MODE_OPTICAL_DEPTH_PIECES : 1.0 / DMS_MAX_OPTICAL_DEPTH_PIECES;
 	int scattering = 0, reflection = 0;
 	bool want_next = false;
 	double rate_next = 0.0;
@@ -1187,7 +1221,7 @@ DMS_HD int dms_step_post(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 				const float isf = (float) is, span = (float) (ie - is);
 #pragma unroll
 				for(int i = 0; i <= DMS_MAX_OPTICAL_DEPTH_PIECES; i++)
-					frf[i] = (i <= od_count) ? isf + (float) i / (float) od_count * span : 0.0f;
+					frf[i] = (i <= od_count) ? isf + (float) i * (float) inv_pieces * span : 0.0f;
 				const float b[6] = {(float) (L.eb_r[0] * DMS_MX_INV_RSUN), (float) (L.eb_r[1] * DMS_MX_INV_RSUN),
 				                    (float) (L.eb_r[2] * DMS_MX_INV_RSUN), (float) L.eb_v[0], (float) L.eb_v[1], (float) L.eb_v[2]};
 				const float a[6] = {(float) (ea_r[0] * DMS_MX_INV_RSUN), (float) (ea_r[1] * DMS_MX_INV_RSUN),
@@ -1202,7 +1236,7 @@ DMS_HD int dms_step_post(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 				if(!invalid)
 				{
 					/* equal pieces of width w: composite trapezoid rules on 4 and 2 pieces */
-					const float w = dt_s * span / (float) od_count;
+					const float w = dt_s * span * (float) inv_pieces;
 					if(od_count >= 4)
 					{
 						const float ends = 0.5f * (rtf[0] + rtf[4]);
@@ -1291,7 +1325,7 @@ DMS_HD int dms_step_post(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 				const double span = ie - is;
 	#pragma unroll
 				for(int i = 0; i <= DMS_MAX_OPTICAL_DEPTH_PIECES; i++)
-					fr[i] = (i <= od_count) ? is + (double) i / (double) od_count * span : 0.0;
+					fr[i] = (i <= od_count) ? is + (double) i * inv_pieces * span : 0.0;
 				/* Scattering_Rate_At_Fraction for all fractions, grid loads issued together */
 				want_next = r_after < DMS_rSun;
 				dms_rates_at_fractions(P, S, L.eb_r, L.eb_v, ea_r, ea_v, fr, od_count + 1, use_cached, L.rate_before, rt,
