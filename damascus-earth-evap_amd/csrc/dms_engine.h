@@ -1459,7 +1459,7 @@ DMS_HD int dms_step_post(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 				L.time_steps++;
 				L.od_retries = 0;
 				if constexpr(MX)
-					dms_commit_mx(L, R, J, S, P.mass, bt, br, bv);
+					dms_commit_mx(L, R, J, S, P.mass, bt, dms_norm(br), dms_dot(bv, bv));
 				else
 					dms_commit(L, R, J, S, P.mass, bt, br, bv);
 				L.ev_t = it + L.time_origin;
@@ -1482,7 +1482,17 @@ DMS_HD int dms_step_post(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 	L.od_retries = 0;
 	int captured_now = 0;
 	if constexpr(MX)
-		dms_commit_mx(L, R, J, S, P.mass, acc_t, acc_r, acc_v);
+	{
+		/* at the step's end point the propagator's FP64 radius and speed; an
+		 * interpolated (scattering) or boundary (reflection) event its own */
+		double cr = r_after, cv2 = v_after * v_after;
+		if(__builtin_expect(scattering || reflection, 0))
+		{
+			cr = dms_norm(acc_r);
+			cv2 = dms_dot(acc_v, acc_v);
+		}
+		dms_commit_mx(L, R, J, S, P.mass, acc_t, cr, cv2);
+	}
 	else
 		captured_now = dms_commit(L, R, J, S, P.mass, acc_t, acc_r, acc_v);
 	L.ev_t = acc_t + L.time_origin;

@@ -381,7 +381,7 @@ DMS_HD void dms_kahan_sub_mx(float& sum, float& comp, float dtau)
 
 /* ---- per-step bookkeeping (FP64, fewer divisions and square roots) -------- */
 
-/* Event_In_3D (:1715-1742) with one division for J/r, plus the speed. */
+/* Event_In_3D (:1715-1742) with one reciprocal for J/r, plus the speed. */
 DMS_HD double dms_prop_event_mx(const DmsProp& p, double& t, double* r, double* v)
 {
 	const double safe_radius = (dm_isfinite(p.radius) && p.radius > 0.0) ? p.radius : 0.0;
@@ -427,12 +427,11 @@ DMS_HD double dms_capture_energy_mx(const DmsSolarView& S, double dm_mass, doubl
 }
 
 /* The commit lambda (:842-874) for a free-flight step: bincount of the previous
- * point and Update_Capture_State(..., allow_new = false). */
+ * point and Update_Capture_State(..., allow_new = false), given the committed
+ * event's radius and squared speed. */
 DMS_HD void dms_commit_mx(DmsLane& L, const DmsRows& R, const DmsJobView& J, const DmsSolarView& S, double dm_mass, double lt,
-                          const double* lr, const double* lv)
+                          double radius, double speed2)
 {
-	const double radius = dms_norm(lr);
-	const double speed2 = dms_dot(lv, lv);
 	const double t_now = lt * (1.0 / DMS_sec);
 	if(!J.terminate_on_capture)
 	{
