@@ -740,20 +740,21 @@ DMS_HD DmsRateSlot dms_rate_slot(const DmsPointView& P, bool use_grid, bool acti
 		outside = r >= DMS_rSun;
 	}
 	/* Total_DM_Scattering_Rate */
-	const bool finite = dm_isfinite(r) && dm_isfinite(v);
+	/* conditions combined with & and | (no short-circuit): selects, not branches */
+	const bool finite = dm_isfinite(r) & dm_isfinite(v);
 	r = (r < 0.0) ? 0.0 : r;
 	v = (v < 0.0) ? 0.0 : v;
 	q.rad = r;
 	q.spd = v;
-	const bool table = use_grid && v <= P.vmax;
+	const bool table = use_grid & (v <= P.vmax);
 	/* Total_DM_Scattering_Rate_Interpolated */
-	const bool zero_interp = table && (r > DMS_rSun || r < 0.0 || v < 0.0);
+	const bool zero_interp = table & ((r > DMS_rSun) | (r < 0.0) | (v < 0.0));
 	const double rc = r * P.inv_dr;
 	const double vc = v * P.inv_dv;
-	const bool coords_ok = dm_isfinite(rc) && dm_isfinite(vc) && !(rc < 0.0) && !(vc < 0.0);
-	const bool live = active && !outside && finite;
-	q.grid = live && table && !zero_interp && coords_ok;
-	q.comp = live && !zero_interp && !(table && coords_ok);
+	const bool coords_ok = dm_isfinite(rc) & dm_isfinite(vc) & !(rc < 0.0) & !(vc < 0.0);
+	const bool live = active & !outside & finite;
+	q.grid = live & table & !zero_interp & coords_ok;
+	q.comp = live & !zero_interp & !(table & coords_ok);
 	const double rcs = coords_ok ? dm_min(rc, (double) (P.Nr - 2)) : 0.0;
 	const double vcs = coords_ok ? dm_min(vc, (double) (P.Nv - 2)) : 0.0;
 	const unsigned ri = (unsigned) rcs;
@@ -779,7 +780,8 @@ DMS_HD double dms_rate_slot_bilinear(const DmsRateSlot& q, const double* g)
 /* Scattering_Rate_At_Fraction (:327-345) for n fractions of one step, plus
  * optionally Total_DM_Scattering_Rate (src/Solar_Model.cpp:305-322) at one more
  * (xr, xv) -- the rate the next step's preparation needs at the end point.
- * Per element the arithmetic is that of dms_rate_at_fraction / dms_rate_total. */
+ * Per element the arithmetic is that of dms_rate_at_fraction / dms_rate_total.
+ * fr and out hold DMS_MAX_OPTICAL_DEPTH_PIECES + 1 elements (fr all defined). */
 DMS_HD void dms_rates_at_fractions(const DmsPointView& P, const DmsSolarView& S, const double* br, const double* bv,
                                    const double* ar, const double* av, const double* fr, int n, bool use_cached,
                                    double cached_rate, double* out, bool extra, double xr, double xv, double* xout)
@@ -799,7 +801,7 @@ DMS_HD void dms_rates_at_fractions(const DmsPointView& P, const DmsSolarView& S,
 	for(int i = 1; i < NQ; i++)
 	{
 		const bool is_extra = (i == NQ - 1);
-		q[i] = dms_rate_slot(P, use_grid, is_extra ? extra : (i < n), !is_extra, br, bv, ar, av, fr[i < n ? i : 0], xr, xv);
+		q[i] = dms_rate_slot(P, use_grid, is_extra ? extra : (i < n), !is_extra, br, bv, ar, av, fr[is_extra ? 0 : i], xr, xv);
 	}
 	if(use_grid)
 	{
@@ -843,10 +845,10 @@ DMS_HD void dms_rates_at_fractions(const DmsPointView& P, const DmsSolarView& S,
 			if(q[i].comp)
 				q[i].res = dms_rate_computed(P, S, q[i].rad, q[i].spd);
 	}
+	/* out has NQ - 1 elements; those from n on are written but not read */
 #pragma unroll
 	for(int i = 0; i < NQ - 1; i++)
-		if(i < n)
-			out[i] = q[i].res;
+		out[i] = q[i].res;
 	if(extra)
 		*xout = q[NQ - 1].res;
 }

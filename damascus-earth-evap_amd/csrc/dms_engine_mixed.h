@@ -253,13 +253,14 @@ DMS_HD DmsRateSlotMx dms_rate_slot_mx(const DmsPointView& P, bool use_grid, bool
 		v = mx_sqrt(q0 * q0 + q1 * q1 + q2 * q2);
 		outside = r >= 1.0f;
 	}
-	const bool finite = mx_isfinite(r) && mx_isfinite(v);
+	/* conditions combined with & (no short-circuit): selects, not branches */
+	const bool finite = mx_isfinite(r) & mx_isfinite(v);
 	q.rad = r;
 	q.spd = v;
-	const bool table = use_grid && v <= (float) P.vmax;
-	const bool live = active && !outside && finite && !(r > 1.0f);
-	q.grid = live && table;
-	q.comp = live && !table;
+	const bool table = use_grid & (v <= (float) P.vmax);
+	const bool live = active & !outside & finite & !(r > 1.0f);
+	q.grid = live & table;
+	q.comp = live & !table;
 	const float rc = r * (float) (P.Nr - 1);
 	const float vc = v * (float) P.inv_dv;
 	const unsigned ri = q.grid ? (unsigned) fminf(rc, (float) (P.Nr - 2)) : 0u;
@@ -296,7 +297,7 @@ DMS_HD void dms_rates_at_fractions_mx(const DmsPointView& P, const DmsSolarView&
 	for(int i = 1; i < NQ; i++)
 	{
 		const bool is_extra = (i == NQ - 1);
-		q[i] = dms_rate_slot_mx(P, use_grid, is_extra ? extra : (i < n), !is_extra, b, d, fr[i < n ? i : 0], xr, xv);
+		q[i] = dms_rate_slot_mx(P, use_grid, is_extra ? extra : (i < n), !is_extra, b, d, fr[is_extra ? 0 : i], xr, xv);
 	}
 	if(use_grid)
 	{
@@ -340,10 +341,10 @@ DMS_HD void dms_rates_at_fractions_mx(const DmsPointView& P, const DmsSolarView&
 			if(q[i].comp)
 				q[i].res = dms_rate_computed_mx(P, S, (double) q[i].rad * DMS_rSun, (double) q[i].spd);
 	}
+	/* out has NQ - 1 elements; those from n on are written but not read */
 #pragma unroll
 	for(int i = 0; i < NQ - 1; i++)
-		if(i < n)
-			out[i] = q[i].res;
+		out[i] = q[i].res;
 	if(extra)
 		*xout = q[NQ - 1].res;
 }
