@@ -405,7 +405,7 @@ DMS_HD int dms_rk45_attempt(DmsProp& p, const DmsSolarView& S, int inner_iter)
 	const double e0 = dm_fabs(r5 - r4), e1 = dm_fabs(v5 - v4), e2 = dm_fabs(p5 - p4);
 	/* RK45_Next_Step_Size, :477-502 */
 	double h_new;
-	if(!dm_isfinite(e0) || !dm_isfinite(e1) || !dm_isfinite(e2))
+	if(!dm_isfinite(e0) | !dm_isfinite(e1) | !dm_isfinite(e2))
 		h_new = DMS_RK45_MIN_STEP_FACTOR * h;
 	else
 	{
@@ -437,7 +437,7 @@ DMS_HD int dms_rk45_attempt(DmsProp& p, const DmsSolarView& S, int inner_iter)
 		h_new = dms_rk45_sanitized(factor * h);
 	}
 	/* RK45_Errors_Within_Tolerance, :466-475 */
-	const bool within = dm_isfinite(e0) && !(e0 > tol0) && dm_isfinite(e1) && !(e1 > tol1) && dm_isfinite(e2) && !(e2 > tol2);
+	const bool within = dm_isfinite(e0) & !(e0 > tol0) & dm_isfinite(e1) & !(e1 > tol1) & dm_isfinite(e2) & !(e2 > tol2);
 	if(within)
 	{
 		p.time = p.time + h;
@@ -587,7 +587,7 @@ DMS_HD bool dms_interior_interval(const double* br, const double* ar, double r_b
 {
 	const bool before_inside = r_before < DMS_rSun;
 	const bool after_inside = r_after < DMS_rSun;
-	if(before_inside && after_inside)
+	if(before_inside & after_inside)
 	{
 		/* the chord's crossings are not used in this case */
 		start = 0.0;
@@ -608,7 +608,7 @@ DMS_HD bool dms_interior_interval(const double* br, const double* ar, double r_b
 		if(qa > 0.0)
 		{
 			const double disc = qb * qb - 4.0 * qa * qc;
-			if(dm_isfinite(disc) && !(disc < 0.0))
+			if(dm_isfinite(disc) & !(disc < 0.0))
 			{
 				const double sd = dm_sqrt(dm_max(0.0, disc));
 				first = (-qb - sd) / (2.0 * qa);
@@ -624,8 +624,8 @@ DMS_HD bool dms_interior_interval(const double* br, const double* ar, double r_b
 		}
 	}
 	const double tol = 1.0e-12;
-	const bool f_in = first >= -tol && first <= 1.0 + tol;
-	const bool s_in = second >= -tol && second <= 1.0 + tol;
+	const bool f_in = (first >= -tol) & (first <= 1.0 + tol);
+	const bool s_in = (second >= -tol) & (second <= 1.0 + tol);
 	bool need_bisection = false;
 	if(!before_inside && after_inside)
 	{
@@ -918,7 +918,7 @@ DMS_HD int dms_update_capture_state(DmsLane& L, const DmsSolarView& S, double dm
                                     double radius, double speed, double time, int allow_new)
 {
 	L.last_E = dm_nan();
-	if(terminate_on_capture && !allow_new && !L.captured)
+	if((terminate_on_capture != 0) & !allow_new & !L.captured)
 		return 0;
 	const double E = dms_capture_energy_eV(S, dm_mass, radius, speed);
 	L.last_E = E;
@@ -930,17 +930,17 @@ DMS_HD int dms_update_capture_state(DmsLane& L, const DmsSolarView& S, double dm
 	const double t_now = dm_div_const(time, DMS_sec, 1.0 / DMS_sec);
 	if(!allow_new)
 	{
-		if(L.captured && dm_isfinite(L.ff_ref_E))
+		if((L.captured != 0) & dm_isfinite(L.ff_ref_E))
 		{
 			const double drift = dm_fabs(E - L.ff_ref_E);
-			if(dm_isfinite(drift) && drift > L.drift_eV)
+			if(dm_isfinite(drift) & (drift > L.drift_eV))
 				L.drift_eV = drift;
 			const double scale = dm_max(dm_fabs(L.ff_ref_E), DMS_FREE_ENERGY_DRIFT_REL_E_SCALE_EV);
 			const double rel = drift / scale;
-			if(dm_isfinite(rel) && rel > L.drift_rel)
+			if(dm_isfinite(rel) & (rel > L.drift_rel))
 				L.drift_rel = rel;
 		}
-		if(L.bound && L.captured)
+		if((L.bound != 0) & (L.captured != 0))
 			L.t_last_bound = t_now;
 		return 0;
 	}
@@ -987,11 +987,11 @@ DMS_HD void dms_bin_flush(DmsLane& L, const DmsRows& R, int nbins)
 
 DMS_HD void dms_accumulate_bincount(DmsLane& L, const DmsRows& R, const DmsJobView& J, double r_km, double v2, double dt_sec)
 {
-	if(!dm_isfinite(dt_sec) || dt_sec <= 0.0)
+	if(!dm_isfinite(dt_sec) | (dt_sec <= 0.0))
 		return;
 	const double bin_max = 2.0 * DMS_R_SUN_KM;
 	int bin = -1;
-	if(dm_isfinite(r_km) && r_km >= 0.0 && r_km < bin_max)
+	if(dm_isfinite(r_km) & (r_km >= 0.0) & (r_km < bin_max))
 	{
 		bin = (int) (r_km / J.bin_width_km);
 		if(bin < 0)
@@ -1079,7 +1079,7 @@ DMS_HD int dms_step_prep(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 {
 	if(__builtin_expect(!(L.time_steps < J.max_time_steps), 0))
 		return DMS_MAX_FREE_STEPS;
-	if(__builtin_expect(J.rk_budget > 0 && L.rk_all > J.rk_budget, 0))
+	if(__builtin_expect((J.rk_budget > 0) & (L.rk_all > J.rk_budget), 0))
 	{
 		double t;
 		dms_prop_event(L.prop, t, L.ev_r, L.ev_v);
@@ -1106,10 +1106,10 @@ DMS_HD int dms_step_prep(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 		const double cap = dms_free_propagation_cap(L.r_before, L.v_before, J.max_distance);
 		L.prop.ts = dm_min(dms_rk45_sanitized(L.prop.ts), cap);
 		const double rvb = (L.r_before > 0.0) ? dms_dot(L.eb_r, L.eb_v) / L.r_before : 0.0;
-		if(L.r_before > DMS_rSun && rvb < 0.0)
+		if((L.r_before > DMS_rSun) & (rvb < 0.0))
 		{
 			const double est = (L.r_before - DMS_rSun) / (-rvb);
-			if(dm_isfinite(est) && est > 0.0)
+			if(dm_isfinite(est) & (est > 0.0))
 				L.prop.ts = dm_min(L.prop.ts, dm_max(est, 1.0e-8 * DMS_sec));
 		}
 	}
@@ -1135,7 +1135,7 @@ DMS_HD int dms_step_prep(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 		else
 		{
 			L.rate_before = L.rb_cached ? cached_rate : dms_rate_total(P, S, L.r_before, L.v_before);
-			if(__builtin_expect(!dm_isfinite(L.rate_before) || L.rate_before < 0.0, 0))
+			if(__builtin_expect(!dm_isfinite(L.rate_before) | (L.rate_before < 0.0), 0))
 			{
 				double t;
 				dms_prop_event(L.prop, t, L.ev_r, L.ev_v);
@@ -1186,11 +1186,11 @@ DMS_HD int dms_step_post(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 	} while(0)
 	if(__builtin_expect(!L.rk_ok, 0))
 		DMS_RETURN_AT(ea_t, ea_r, ea_v, DMS_NUMERICAL_FAILURE);
-	if(__builtin_expect(!dm_isfinite(r_after) || !dm_isfinite(v_after) || !dm_isfinite(actual_dt) || actual_dt <= 0.0, 0))
+	if(__builtin_expect(!dm_isfinite(r_after) | !dm_isfinite(v_after) | !dm_isfinite(actual_dt) | (actual_dt <= 0.0), 0))
 		DMS_RETURN_AT(ea_t, ea_r, ea_v, DMS_NON_FINITE_STATE);
 	if(__builtin_expect(v_after > DMS_V_MAX, 0))
 		DMS_RETURN_AT(ea_t, ea_r, ea_v, DMS_SPEED_LIMIT);
-	if(__builtin_expect(J.rk_budget > 0 && L.rk_all > J.rk_budget, 0))
+	if(__builtin_expect((J.rk_budget > 0) & (L.rk_all > J.rk_budget), 0))
 		DMS_RETURN_AT(ea_t, ea_r, ea_v, DMS_WALL_TIME_LIMIT);
 	const int fast = J.terminate_on_capture;
 	const double od_limit = fast ? DMS_CAPTURE_MODE_MAX_OPTICAL_DEPTH_STEP : DMS_MAX_OPTICAL_DEPTH_STEP;
@@ -1208,7 +1208,7 @@ DMS_HD int dms_step_post(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 	if(dms_interior_interval(L.eb_r, ea_r, L.r_before, r_after, is, ie))
 	{
 		L.n_int++;
-		if(r_after < DMS_rSun && v_after < 0.0)
+		if((r_after < DMS_rSun) & (v_after < 0.0))
 			return DMS_MAX_FREE_STEPS; /* unreachable 'break' of the reference, :964-968 */
 		if constexpr(MX)
 		{
@@ -1319,10 +1319,10 @@ DMS_HD int dms_step_post(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 		{
 			/* Build_Optical_Depth_Pieces, :347-414 */
 			double fr[DMS_MAX_OPTICAL_DEPTH_PIECES + 1], rt[DMS_MAX_OPTICAL_DEPTH_PIECES + 1];
-			const bool use_cached = L.r_before < DMS_rSun && is <= 1.0e-12;
+			const bool use_cached = (L.r_before < DMS_rSun) & (is <= 1.0e-12);
 			double tau2 = 0.0, dtau = 0.0;
 			int invalid = 0, pc = 0;
-			if(ie > is && actual_dt > 0.0)
+			if((ie > is) & (actual_dt > 0.0))
 			{
 				const double span = ie - is;
 	#pragma unroll
@@ -1334,8 +1334,7 @@ DMS_HD int dms_step_post(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 				                       want_next, r_after, v_after, &rate_next);
 	#pragma unroll
 				for(int i = 0; i <= DMS_MAX_OPTICAL_DEPTH_PIECES; i++)
-					if(i <= od_count && (!dm_isfinite(rt[i]) || rt[i] < 0.0))
-						invalid = 1;
+					invalid |= (int) ((i <= od_count) & (!dm_isfinite(rt[i]) | (rt[i] < 0.0)));
 				if(!invalid)
 				{
 	#define DMS_TRAP(s, e) (0.5 * (rt[s] + rt[e]) * (actual_dt * (fr[e] - fr[s])))
@@ -1358,13 +1357,13 @@ DMS_HD int dms_step_post(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 	#undef DMS_TRAP
 				}
 			}
-			if(__builtin_expect(invalid || !dm_isfinite(dtau) || dtau < 0.0, 0))
+			if(__builtin_expect((invalid != 0) | !dm_isfinite(dtau) | (dtau < 0.0), 0))
 				DMS_RETURN_AT(ea_t, ea_r, ea_v, DMS_NUMERICAL_FAILURE);
 			const double scale = dm_max(dtau, DMS_OPTICAL_DEPTH_ABSOLUTE_TOLERANCE);
 			const double relerr = dm_fabs(dtau - tau2) / scale;
 			const bool rej_od = dtau > od_limit;
-			const bool rej_acc = !fast && relerr > DMS_OPTICAL_DEPTH_RELATIVE_TOLERANCE;
-			if(rej_od || rej_acc)
+			const bool rej_acc = !fast & (relerr > DMS_OPTICAL_DEPTH_RELATIVE_TOLERANCE);
+			if(rej_od | rej_acc)
 			{
 				L.n_rej++;
 				L.od_retries++;
@@ -1434,14 +1433,15 @@ DMS_HD int dms_step_post(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 	{
 		double bt = ea_t, br[3] = {ea_r[0], ea_r[1], ea_r[2]}, bv[3] = {ea_v[0], ea_v[1], ea_v[2]};
 		double rb = r_after, vb = v_after;
-		if(__builtin_expect(L.r_before < J.max_distance && r_after >= J.max_distance && r_after > L.r_before, 0))
+		if(__builtin_expect((L.r_before < J.max_distance) & (r_after >= J.max_distance) & (r_after > L.r_before), 0))
 		{
 			const double bf = (J.max_distance - L.r_before) / (r_after - L.r_before);
 			dms_interpolate_event(L.eb_t, L.eb_r, L.eb_v, ea_t, ea_r, ea_v, bf, bt, br, bv);
 			rb = dms_norm(br);
 			vb = dms_norm(bv);
 		}
-		if(__builtin_expect(rb >= J.max_distance && rb >= L.r_before, 0) && vb > dms_local_escape_speed(S, rb))
+		const bool beyond = (rb >= J.max_distance) & (rb >= L.r_before);
+		if(__builtin_expect(beyond, 0) && vb > dms_local_escape_speed(S, rb))
 		{
 			acc_t = bt;
 			for(int c = 0; c < 3; c++)
@@ -1451,7 +1451,7 @@ DMS_HD int dms_step_post(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 			}
 			reflection = 1;
 		}
-		else if(__builtin_expect(rb >= J.max_distance && rb >= L.r_before, 0))
+		else if(__builtin_expect(beyond, 0))
 		{
 			if(dms_uncaptured_bound(L, S, P.mass, br, bv))
 				DMS_RETURN_AT(bt, br, bv, DMS_NUMERICAL_FAILURE);
@@ -1510,10 +1510,10 @@ DMS_HD int dms_step_post(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 		if(dm_isnan(E))
 			E = MX ? dms_capture_energy_mx(S, P.mass, dms_norm(L.ev_r), dms_dot(L.ev_v, L.ev_v))
 			       : dms_capture_energy_eV(S, P.mass, dms_norm(L.ev_r), dms_norm(L.ev_v));
-		if(!L.captured && !(dm_isfinite(E) && E >= 0.0))
+		if(!L.captured & !(dm_isfinite(E) & (E >= 0.0)))
 			return DMS_NUMERICAL_FAILURE;
 	}
-	if(J.terminate_on_capture && captured_now)
+	if((J.terminate_on_capture != 0) & (captured_now != 0))
 		return DMS_CAPTURE_MODE;
 	if(reflection)
 		return DMS_OUTWARD_ESCAPE;
