@@ -75,46 +75,86 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 	for(;;)
 	{
 		dms_lane_advance_a<MX>(L, c);
-		/* Sample_Target (src/Simulation_Trajectory.cpp:1122-1158): for each
-		 * scattering lane s the 64 lanes evaluate one target rate each. */
-		/* (the rare branches of the loop are marked so that the compiler lays
-		 * them out of the path a lone long trajectory executes) */
-		unsigned long long m = __ballot(L.phase == PH_SCAT);
-		while(__builtin_expect(m != 0, 0))
+		/* The wave-level services below (target sampling, scatter completion,
+		 * capture counting, trajectory end, row flush, refill, start) act only
+		 * on lanes in those phases: while every live lane is between RKF45
+		 * attempts, as a lone long trajectory is, the wave skips them at once. */
+		const bool service = __ballot((L.phase != PH_DONE) & (L.phase != PH_PREP) & (L.phase != PH_RK)) != 0;
+		if(service)
 		{
-			const int s = __ffsll((long long) m) - 1;
-			m &= m - 1;
-			const double r = readlane_d(L.sc_r, s);
-			const double v = readlane_d(L.sc_v, s);
-			const double xi = readlane_d(L.sc_xi, s);
-			const int k = (lane < n_iso) ? lane : ((lane == DMS_MAX_TARGETS - 1) ? lane : -1);
-			int target = -2;
-			/* the scattering lane's parameter point (uniform over the wave) */
-			const DmsPointView& Ps = MP ? a.job_points[__builtin_amdgcn_readlane((int) L.job, s)] : a.P;
-			if constexpr(MX)
+			/* Sample_Target (src/Simulation_Trajectory.cpp:1122-1158): for each
+			 * scattering lane s the 64 lanes evaluate one target rate each. */
+			/* (the rare branches of the loop are marked so that the compiler lays
+			 * them out of the path a lone long trajectory executes) */
+			unsigned long long m = __ballot(L.phase == PH_SCAT);
+			while(__builtin_expect(m != 0, 0))
 			{
-				/* FP32 rates (1/s), same draw and order as below */
-				float rate = 0.0f;
+				const int s = __ffsll((long long) m) - 1;
+				m &= m - 1;
+				const double r = readlane_d(L.sc_r, s);
+				const double v = readlane_d(L.sc_v, s);
+				const double xi = readlane_d(L.sc_xi, s);
+				const int k = (lane < n_iso) ? lane : ((lane == DMS_MAX_TARGETS - 1) ? lane : -1);
+				int target = -2;
+				/* the scattering lane's parameter point (uniform over the wave) */
+				const DmsPointView& Ps = MP ? a.job_points[__builtin_amdgcn_readlane((int) L.job, s)] : a.P;
+				if constexpr(MX)
+				{
+					/* FP32 rates (1/s), same draw and order as below */
+					float rate = 0.0f;
+					if(k >= 0)
+						rate = dms_rate_target_mx(Ps, a.S, r, v, k);
+					const bool bad = (k >= 0) && (!mx_isfinite(rate) || rate < 0.0f);
+					if(__ballot(bad) == 0)
+					{
+						float total = readlane_f(rate, DMS_MAX_TARGETS - 1);
+						for(int i = 0; i < n_iso; i++)
+							total += readlane_f(rate, i);
+						if(mx_isfinite(total) && total > 0.0f)
+						{
+							const float q = rate * mx_rcp(total);
+							const float xf = (float) xi;
+							float sum = readlane_f(q, DMS_MAX_TARGETS - 1);
+							if(sum > xf)
+								target = -1;
+							else
+								for(int i = 0; i < n_iso; i++)
+								{
+									sum += readlane_f(q, i);
+									if(sum > xf || i == n_iso - 1)
+									{
+										target = i;
+										break;
+									}
+								}
+						}
+					}
+					if(lane == s)
+						L.sc_target = target;
+					continue;
+				}
+				double rate = 0.0;
 				if(k >= 0)
-					rate = dms_rate_target_mx(Ps, a.S, r, v, k);
-				const bool bad = (k >= 0) && (!mx_isfinite(rate) || rate < 0.0f);
+					rate = dms_rate_target(Ps, a.S, r, v, k);
+				const bool bad = (k >= 0) && (!dm_isfinite(rate) || rate < 0.0);
 				if(__ballot(bad) == 0)
 				{
-					float total = readlane_f(rate, DMS_MAX_TARGETS - 1);
+					/* same summation and comparison order as dms_choose_target */
+					const double re = readlane_d(rate, DMS_MAX_TARGETS - 1);
+					double total = re;
 					for(int i = 0; i < n_iso; i++)
-						total += readlane_f(rate, i);
-					if(mx_isfinite(total) && total > 0.0f)
+						total = total + readlane_d(rate, i);
+					if(dm_isfinite(total) && total > 0.0)
 					{
-						const float q = rate * mx_rcp(total);
-						const float xf = (float) xi;
-						float sum = readlane_f(q, DMS_MAX_TARGETS - 1);
-						if(sum > xf)
+						const double q = rate / total;
+						double sum = readlane_d(q, DMS_MAX_TARGETS - 1);
+						if(sum > xi)
 							target = -1;
 						else
 							for(int i = 0; i < n_iso; i++)
 							{
-								sum += readlane_f(q, i);
-								if(sum > xf || i == n_iso - 1)
+								sum += readlane_d(q, i);
+								if(sum > xi || i == n_iso - 1)
 								{
 									target = i;
 									break;
@@ -124,135 +164,103 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 				}
 				if(lane == s)
 					L.sc_target = target;
-				continue;
 			}
-			double rate = 0.0;
-			if(k >= 0)
-				rate = dms_rate_target(Ps, a.S, r, v, k);
-			const bool bad = (k >= 0) && (!dm_isfinite(rate) || rate < 0.0);
-			if(__ballot(bad) == 0)
+			const int captured_before = L.captured;
+			dms_lane_finish_scatter<MX>(L, c);
+			/* capture-target chunk: count captures as they happen (the post-scatter
+			 * test), so that the slot limit drops while the captured trajectories
+			 * still orbit */
+			if(a.fill_ctl && L.captured && !captured_before)
 			{
-				/* same summation and comparison order as dms_choose_target */
-				const double re = readlane_d(rate, DMS_MAX_TARGETS - 1);
-				double total = re;
-				for(int i = 0; i < n_iso; i++)
-					total = total + readlane_d(rate, i);
-				if(dm_isfinite(total) && total > 0.0)
+				const unsigned int ncap = atomicAdd(a.fill_ctl, 1u) + 1u;
+				if(ncap == a.cap_stop)
 				{
-					const double q = rate / total;
-					double sum = readlane_d(q, DMS_MAX_TARGETS - 1);
-					if(sum > xi)
-						target = -1;
-					else
-						for(int i = 0; i < n_iso; i++)
+					const unsigned int handed = __hip_atomic_load(a.next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+					atomicMin(a.fill_ctl + 1, handed + a.fill_extra);
+				}
+			}
+			int flush_cls = -1;
+			if(__builtin_expect(L.phase == PH_FIN, 0))
+			{
+				flush_cls = dms_lane_finish_rows_pending(L, c, a.ics[L.slot], a.records ? a.records + L.slot : nullptr,
+				                                         a.summ ? a.summ + L.slot : nullptr);
+				if(a.job_done)
+					atomicAdd(a.job_done + L.job, 1u);
+			}
+			/* per-lane histogram rows of the finished trajectories: the wave adds
+			 * each row's touched bins to its job's sums, 64 bins at a time */
+			unsigned long long fm = __ballot(flush_cls >= 0);
+			if(__builtin_expect(fm != 0, 0))
+			{
+				while(fm)
+				{
+					const int s = __ffsll((long long) fm) - 1;
+					fm &= fm - 1;
+					const uint32_t row_s = (uint32_t) __builtin_amdgcn_readlane((int) L.row, s);
+					const uint32_t job_s = (uint32_t) __builtin_amdgcn_readlane((int) L.job, s);
+					const uint32_t n_s = (uint32_t) __builtin_amdgcn_readlane((int) L.n_touched, s);
+					const int cls_s = __builtin_amdgcn_readlane(flush_cls, s);
+					dms_flush_rows(c.R, c.J.nbins, row_s, job_s, n_s, cls_s, (uint32_t) lane, 64u);
+				}
+				if(flush_cls >= 0)
+					L.n_touched = 0;
+				/* the cleared rows are reused by their lanes' next trajectories */
+				__threadfence_block();
+			}
+			/* lane refill: one atomic per wave */
+			const unsigned long long nmask = __ballot(L.phase == PH_NEED);
+			if(__builtin_expect(nmask != 0, 0))
+			{
+				const int leader = __ffsll((long long) nmask) - 1;
+				unsigned int base = 0, limit = a.n;
+				if(lane == leader)
+				{
+					base = atomicAdd(a.next, (unsigned int) __popcll(nmask));
+					if(a.fill_ctl)
+						limit = __hip_atomic_load(a.fill_ctl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+				}
+				base = (unsigned int) __builtin_amdgcn_readlane((int) base, leader);
+				limit = (unsigned int) __builtin_amdgcn_readlane((int) limit, leader);
+				if(L.phase == PH_NEED)
+				{
+					const unsigned int rank = (unsigned int) __popcll(nmask & ((1ull << lane) - 1ull));
+					const unsigned int idx = base + rank;
+					if(idx < a.n && idx < limit)
+					{
+						L.slot = idx;
+						/* job of the slot: last j with job_start[j] <= idx */
+						uint32_t lo = 0, hi = a.n_jobs;
+						while(hi - lo > 1)
 						{
-							sum += readlane_d(q, i);
-							if(sum > xi || i == n_iso - 1)
-							{
-								target = i;
-								break;
-							}
+							const uint32_t mid = (lo + hi) >> 1;
+							if(a.job_start[mid] <= idx)
+								lo = mid;
+							else
+								hi = mid;
 						}
+						const uint32_t j = lo;
+						L.job = j;
+						if constexpr(MP)
+						{
+							c.P = a.job_points[j];
+							c.J.seed = a.job_keys[2 * j];
+							c.J.point_id = a.job_keys[2 * j + 1];
+						}
+						L.id = a.job_first_id[j] + (uint64_t) (idx - a.job_start[j]);
+						L.row = a.R.per_lane ? lane_gid : idx;
+						L.phase = PH_START;
+					}
+					else
+						L.phase = PH_DONE;
 				}
 			}
-			if(lane == s)
-				L.sc_target = target;
-		}
-		const int captured_before = L.captured;
-		dms_lane_finish_scatter<MX>(L, c);
-		/* capture-target chunk: count captures as they happen (the post-scatter
-		 * test), so that the slot limit drops while the captured trajectories
-		 * still orbit */
-		if(a.fill_ctl && L.captured && !captured_before)
-		{
-			const unsigned int ncap = atomicAdd(a.fill_ctl, 1u) + 1u;
-			if(ncap == a.cap_stop)
+			if(__builtin_expect(L.phase == PH_START, 0))
 			{
-				const unsigned int handed = __hip_atomic_load(a.next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-				atomicMin(a.fill_ctl + 1, handed + a.fill_extra);
+				dms_lane_start(L, c, a.ics[L.slot], a.shift_ok == nullptr || a.shift_ok[L.slot] != 0,
+				               a.records ? a.records + L.slot : nullptr, a.summ ? a.summ + L.slot : nullptr);
+				if(L.phase == PH_NEED && a.job_done) /* initial shift failed: recorded, done */
+					atomicAdd(a.job_done + L.job, 1u);
 			}
-		}
-		int flush_cls = -1;
-		if(__builtin_expect(L.phase == PH_FIN, 0))
-		{
-			flush_cls = dms_lane_finish_rows_pending(L, c, a.ics[L.slot], a.records ? a.records + L.slot : nullptr,
-			                                         a.summ ? a.summ + L.slot : nullptr);
-			if(a.job_done)
-				atomicAdd(a.job_done + L.job, 1u);
-		}
-		/* per-lane histogram rows of the finished trajectories: the wave adds
-		 * each row's touched bins to its job's sums, 64 bins at a time */
-		unsigned long long fm = __ballot(flush_cls >= 0);
-		if(__builtin_expect(fm != 0, 0))
-		{
-			while(fm)
-			{
-				const int s = __ffsll((long long) fm) - 1;
-				fm &= fm - 1;
-				const uint32_t row_s = (uint32_t) __builtin_amdgcn_readlane((int) L.row, s);
-				const uint32_t job_s = (uint32_t) __builtin_amdgcn_readlane((int) L.job, s);
-				const uint32_t n_s = (uint32_t) __builtin_amdgcn_readlane((int) L.n_touched, s);
-				const int cls_s = __builtin_amdgcn_readlane(flush_cls, s);
-				dms_flush_rows(c.R, c.J.nbins, row_s, job_s, n_s, cls_s, (uint32_t) lane, 64u);
-			}
-			if(flush_cls >= 0)
-				L.n_touched = 0;
-			/* the cleared rows are reused by their lanes' next trajectories */
-			__threadfence_block();
-		}
-		/* lane refill: one atomic per wave */
-		const unsigned long long nmask = __ballot(L.phase == PH_NEED);
-		if(__builtin_expect(nmask != 0, 0))
-		{
-			const int leader = __ffsll((long long) nmask) - 1;
-			unsigned int base = 0, limit = a.n;
-			if(lane == leader)
-			{
-				base = atomicAdd(a.next, (unsigned int) __popcll(nmask));
-				if(a.fill_ctl)
-					limit = __hip_atomic_load(a.fill_ctl + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-			}
-			base = (unsigned int) __builtin_amdgcn_readlane((int) base, leader);
-			limit = (unsigned int) __builtin_amdgcn_readlane((int) limit, leader);
-			if(L.phase == PH_NEED)
-			{
-				const unsigned int rank = (unsigned int) __popcll(nmask & ((1ull << lane) - 1ull));
-				const unsigned int idx = base + rank;
-				if(idx < a.n && idx < limit)
-				{
-					L.slot = idx;
-					/* job of the slot: last j with job_start[j] <= idx */
-					uint32_t lo = 0, hi = a.n_jobs;
-					while(hi - lo > 1)
-					{
-						const uint32_t mid = (lo + hi) >> 1;
-						if(a.job_start[mid] <= idx)
-							lo = mid;
-						else
-							hi = mid;
-					}
-					const uint32_t j = lo;
-					L.job = j;
-					if constexpr(MP)
-					{
-						c.P = a.job_points[j];
-						c.J.seed = a.job_keys[2 * j];
-						c.J.point_id = a.job_keys[2 * j + 1];
-					}
-					L.id = a.job_first_id[j] + (uint64_t) (idx - a.job_start[j]);
-					L.row = a.R.per_lane ? lane_gid : idx;
-					L.phase = PH_START;
-				}
-				else
-					L.phase = PH_DONE;
-			}
-		}
-		if(__builtin_expect(L.phase == PH_START, 0))
-		{
-			dms_lane_start(L, c, a.ics[L.slot], a.shift_ok == nullptr || a.shift_ok[L.slot] != 0,
-			               a.records ? a.records + L.slot : nullptr, a.summ ? a.summ + L.slot : nullptr);
-			if(L.phase == PH_NEED && a.job_done) /* initial shift failed: recorded, done */
-				atomicAdd(a.job_done + L.job, 1u);
 		}
 		dms_lane_advance_b<MX>(L, c);
 		if(__ballot(L.phase != PH_DONE) == 0)
