@@ -132,9 +132,7 @@ struct DmsRows
 
 DMS_HD double dms_rk45_sanitized(double step)
 {
-	if(!dm_isfinite(step) || step <= 0.0)
-		return 0.1 * DMS_sec;
-	return dm_min(step, 1.0e6 * DMS_sec);
+	return (!dm_isfinite(step) | (step <= 0.0)) ? 0.1 * DMS_sec : dm_min(step, 1.0e6 * DMS_sec);
 }
 
 /* Free_Particle_Propagator constructor, src/Simulation_Trajectory.cpp:1403-1464 */
@@ -403,54 +401,33 @@ DMS_HD int dms_rk45_attempt(DmsProp& p, const DmsSolarView& S, int inner_iter)
 	const double v5 = p.vr + 16.0 / 135.0 * kv[0] + 6656.0 / 12825.0 * kv[2] + 28561.0 / 56430.0 * kv[3] - 9.0 / 50.0 * kv[4] + 2.0 / 55.0 * kv[5];
 	const double p5 = p.phi + 16.0 / 135.0 * kp[0] + 6656.0 / 12825.0 * kp[2] + 28561.0 / 56430.0 * kp[3] - 9.0 / 50.0 * kp[4] + 2.0 / 55.0 * kp[5];
 	const double e0 = dm_fabs(r5 - r4), e1 = dm_fabs(v5 - v4), e2 = dm_fabs(p5 - p4);
-	/* RK45_Next_Step_Size, :477-502 */
-	double h_new;
-	if(!dm_isfinite(e0) | !dm_isfinite(e1) | !dm_isfinite(e2))
-		h_new = DMS_RK45_MIN_STEP_FACTOR * h;
-	else
+	/* RK45_Next_Step_Size, :477-502, and the acceptance, written with selects
+	 * (both arms are plain values; only the forced acceptance branches) */
+	const bool finite_err = dm_isfinite(e0) & dm_isfinite(e1) & dm_isfinite(e2);
+	const double ee[3] = {e0, e1, e2};
+	const double tt[3] = {tol0, tol1, tol2};
+	/* The reference takes the minimum of 0.84 (tol_i / e_i)^(1/4) over the
+	 * components with e_i > 0, skipping non-finite candidates.  Rounded
+	 * square roots and the product by 0.84 are monotone, so that minimum
+	 * is the candidate of the smallest quotient (in (0, inf], finite
+	 * exactly when its candidate is): one candidate instead of three. */
+	double q = 0.0;
+	bool any = false;
+	for(int i = 0; i < 3; i++)
 	{
-		double factor = DMS_RK45_MAX_STEP_FACTOR;
-		const double ee[3] = {e0, e1, e2};
-		const double tt[3] = {tol0, tol1, tol2};
-		/* The reference takes the minimum of 0.84 (tol_i / e_i)^(1/4) over the
-		 * components with e_i > 0, skipping non-finite candidates.  Rounded
-		 * square roots and the product by 0.84 are monotone, so that minimum
-		 * is the candidate of the smallest quotient (in (0, inf], finite
-		 * exactly when its candidate is): one candidate instead of three. */
-		double q = 0.0;
-		bool any = false;
-		for(int i = 0; i < 3; i++)
-		{
-			if(ee[i] <= 0.0)
-				continue;
-			const double qi = tt[i] / ee[i];
-			q = any ? dm_min(q, qi) : qi;
-			any = true;
-		}
-		if(any)
-		{
-			const double candidate = 0.84 * dm_pow_quarter(q);
-			if(dm_isfinite(candidate))
-				factor = dm_min(factor, candidate);
-		}
-		factor = dm_max(DMS_RK45_MIN_STEP_FACTOR, dm_min(factor, DMS_RK45_MAX_STEP_FACTOR));
-		h_new = dms_rk45_sanitized(factor * h);
+		const bool use = !(ee[i] <= 0.0);
+		const double qi = tt[i] / ee[i];
+		q = use ? (any ? dm_min(q, qi) : qi) : q;
+		any = any | use;
 	}
+	const double candidate = 0.84 * dm_pow_quarter(q);
+	double factor = (any & dm_isfinite(candidate)) ? dm_min(DMS_RK45_MAX_STEP_FACTOR, candidate) : DMS_RK45_MAX_STEP_FACTOR;
+	factor = dm_max(DMS_RK45_MIN_STEP_FACTOR, dm_min(factor, DMS_RK45_MAX_STEP_FACTOR));
+	const double h_new = finite_err ? dms_rk45_sanitized(factor * h) : DMS_RK45_MIN_STEP_FACTOR * h;
 	/* RK45_Errors_Within_Tolerance, :466-475 */
-	const bool within = dm_isfinite(e0) & !(e0 > tol0) & dm_isfinite(e1) & !(e1 > tol1) & dm_isfinite(e2) & !(e2 > tol2);
-	if(within)
-	{
-		p.time = p.time + h;
-		p.radius = r4;
-		if(p.radius < 0.0)
-			p.radius = 0.0;
-		p.vr = v4;
-		p.phi = p4;
-		p.ts = h_new;
-		return 1;
-	}
+	const bool within = finite_err & !(e0 > tol0) & !(e1 > tol1) & !(e2 > tol2);
 	const double abs_min_step = 1.0e-8 * DMS_sec;
-	if(__builtin_expect(h_new < abs_min_step || inner_iter >= DMS_RK45_MAX_INNER_RETRIES, 0))
+	if(__builtin_expect(!within & ((h_new < abs_min_step) | (inner_iter >= DMS_RK45_MAX_INNER_RETRIES)), 0))
 	{
 		const bool state_ok = dm_isfinite(r4) && dm_isfinite(v4) && dm_isfinite(p4);
 		p.time = p.time + abs_min_step;
@@ -465,8 +442,12 @@ DMS_HD int dms_rk45_attempt(DmsProp& p, const DmsSolarView& S, int inner_iter)
 		p.ts = abs_min_step;
 		return 0;
 	}
+	p.time = within ? p.time + h : p.time;
+	p.radius = within ? ((r4 < 0.0) ? 0.0 : r4) : p.radius;
+	p.vr = within ? v4 : p.vr;
+	p.phi = within ? p4 : p.phi;
 	p.ts = h_new;
-	return -1;
+	return within ? 1 : -1;
 }
 
 /* ---- helpers of Propagate_Freely, :53-464 ------------------------------------ */
@@ -1142,8 +1123,8 @@ DMS_HD int dms_step_prep(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 				L.ev_t = t + L.time_origin;
 				return DMS_NUMERICAL_FAILURE;
 			}
-			if(L.rate_before > 0.0)
-				L.prop.ts = dm_min(L.prop.ts, od_limit / L.rate_before);
+			const double od_ts = od_limit / L.rate_before;
+			L.prop.ts = (L.rate_before > 0.0) ? dm_min(L.prop.ts, od_ts) : L.prop.ts;
 		}
 	}
 	L.pb_time = L.prop.time;
@@ -1184,14 +1165,17 @@ DMS_HD int dms_step_post(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 		}                               \
 		return (code);                  \
 	} while(0)
-	if(__builtin_expect(!L.rk_ok, 0))
-		DMS_RETURN_AT(ea_t, ea_r, ea_v, DMS_NUMERICAL_FAILURE);
-	if(__builtin_expect(!dm_isfinite(r_after) | !dm_isfinite(v_after) | !dm_isfinite(actual_dt) | (actual_dt <= 0.0), 0))
-		DMS_RETURN_AT(ea_t, ea_r, ea_v, DMS_NON_FINITE_STATE);
-	if(__builtin_expect(v_after > DMS_V_MAX, 0))
-		DMS_RETURN_AT(ea_t, ea_r, ea_v, DMS_SPEED_LIMIT);
-	if(__builtin_expect((J.rk_budget > 0) & (L.rk_all > J.rk_budget), 0))
-		DMS_RETURN_AT(ea_t, ea_r, ea_v, DMS_WALL_TIME_LIMIT);
+	{
+		/* the reference's four checks in its order, under one rarely taken branch */
+		const bool failed = !L.rk_ok;
+		const bool non_finite = !dm_isfinite(r_after) | !dm_isfinite(v_after) | !dm_isfinite(actual_dt) | (actual_dt <= 0.0);
+		const bool too_fast = v_after > DMS_V_MAX;
+		const bool budget = (J.rk_budget > 0) & (L.rk_all > J.rk_budget);
+		if(__builtin_expect(failed | non_finite | too_fast | budget, 0))
+			DMS_RETURN_AT(ea_t, ea_r, ea_v,
+			              failed ? DMS_NUMERICAL_FAILURE
+			                     : non_finite ? DMS_NON_FINITE_STATE : too_fast ? DMS_SPEED_LIMIT : DMS_WALL_TIME_LIMIT);
+	}
 	const int fast = J.terminate_on_capture;
 	const double od_limit = fast ? DMS_CAPTURE_MODE_MAX_OPTICAL_DEPTH_STEP : DMS_MAX_OPTICAL_DEPTH_STEP;
 	const int od_count = fast ? DMS_CAPTURE_MODE_OPTICAL_DEPTH_PIECES : DMS_MAX_OPTICAL_DEPTH_PIECES;
@@ -1976,11 +1960,8 @@ DMS_HD void dms_lane_advance_a(DmsLane& L, const DmsLaneCtx& c)
 		L.inner_iter++;
 		L.rk_all++;
 		const int r = MX ? dms_rk45_attempt_mx(L.prop, c.S, L.inner_iter) : dms_rk45_attempt(L.prop, c.S, L.inner_iter);
-		if(r >= 0)
-		{
-			L.rk_ok = r;
-			L.phase = PH_POST;
-		}
+		L.rk_ok = (r >= 0) ? r : L.rk_ok;
+		L.phase = (r >= 0) ? (int) PH_POST : L.phase;
 	}
 	if(L.phase == PH_POST)
 	{
