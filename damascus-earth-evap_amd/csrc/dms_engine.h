@@ -914,14 +914,12 @@ DMS_HD int dms_update_capture_state(DmsLane& L, const DmsSolarView& S, double dm
 		if((L.captured != 0) & dm_isfinite(L.ff_ref_E))
 		{
 			const double drift = dm_fabs(E - L.ff_ref_E);
-			if(dm_isfinite(drift) & (drift > L.drift_eV))
-				L.drift_eV = drift;
+			L.drift_eV = (dm_isfinite(drift) & (drift > L.drift_eV)) ? drift : L.drift_eV;
 			const double scale = dm_max(dm_fabs(L.ff_ref_E), DMS_FREE_ENERGY_DRIFT_REL_E_SCALE_EV);
 			const double rel = drift / scale;
-			if(dm_isfinite(rel) & (rel > L.drift_rel))
-				L.drift_rel = rel;
+			L.drift_rel = (dm_isfinite(rel) & (rel > L.drift_rel)) ? rel : L.drift_rel;
 		}
-		if((L.bound != 0) & (L.captured != 0))
+		if((L.bound != 0) & (L.captured != 0)) /* a store, not a select: the field lives in scratch */
 			L.t_last_bound = t_now;
 		return 0;
 	}
@@ -1497,12 +1495,11 @@ DMS_HD int dms_step_post(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 		if(!L.captured & !(dm_isfinite(E) & (E >= 0.0)))
 			return DMS_NUMERICAL_FAILURE;
 	}
-	if((J.terminate_on_capture != 0) & (captured_now != 0))
-		return DMS_CAPTURE_MODE;
-	if(reflection)
-		return DMS_OUTWARD_ESCAPE;
-	if(scattering)
-		return -3;
+	{
+		const bool capture_stop = (J.terminate_on_capture != 0) & (captured_now != 0);
+		if(__builtin_expect(capture_stop | (reflection != 0) | (scattering != 0), 0))
+			return capture_stop ? DMS_CAPTURE_MODE : (reflection ? DMS_OUTWARD_ESCAPE : -3);
+	}
 	/* the next step starts from this propagator state: Event_In_3D is ea */
 	L.eb_t = ea_t;
 	for(int c = 0; c < 3; c++)

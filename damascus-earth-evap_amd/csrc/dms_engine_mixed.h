@@ -448,16 +448,14 @@ DMS_HD void dms_commit_mx(DmsLane& L, const DmsRows& R, const DmsJobView& J, con
 	const double E = dms_capture_energy_mx(S, dm_mass, radius, speed2);
 	L.last_E = E;
 	L.prev_cap_E = E;
-	if(L.captured && dm_isfinite(L.ff_ref_E))
+	if((L.captured != 0) & dm_isfinite(L.ff_ref_E))
 	{
 		const double drift = dm_fabs(E - L.ff_ref_E);
-		if(dm_isfinite(drift) && drift > L.drift_eV)
-			L.drift_eV = drift;
+		L.drift_eV = (dm_isfinite(drift) & (drift > L.drift_eV)) ? drift : L.drift_eV;
 		const double rel = drift * dm_recip_shared(dm_max(dm_fabs(L.ff_ref_E), DMS_FREE_ENERGY_DRIFT_REL_E_SCALE_EV)).y;
-		if(dm_isfinite(rel) && rel > L.drift_rel)
-			L.drift_rel = rel;
+		L.drift_rel = (dm_isfinite(rel) & (rel > L.drift_rel)) ? rel : L.drift_rel;
 	}
-	if(L.bound && L.captured)
+	if((L.bound != 0) & (L.captured != 0)) /* a store, not a select: the field lives in scratch */
 		L.t_last_bound = (lt + L.time_origin) * (1.0 / DMS_sec);
 }
 
