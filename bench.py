@@ -58,7 +58,7 @@ PEAK_FP64_TFLOPS = 78.6
 # HBM bytes of the timed launch of the default workload (4 jobs of 10^6 in one
 # launch): rocprofv3 FETCH_SIZE and WRITE_SIZE passes, scaled by a plain 256 MiB
 # copy (read x2.0, write x1.0); profiles/dms_traj_kernel_parameter_point_m0.3_4x1e6.md.
-PROFILED_TRAFFIC_BYTES = {(4, 1_000_000): 275.7e9 + 158.7e9}
+PROFILED_TRAFFIC_BYTES = {(4, 1_000_000): 275.5e9 + 158.2e9}
 
 
 def algorithmic_flops(c, capture_mode):
