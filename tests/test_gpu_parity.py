@@ -167,6 +167,9 @@ def test_trajectories_bit_identical(gpu_ctx, oracle_solar, name, mode, desc_args
     (dmsun.MODE_CAPTURE, dict(mass_GeV=1.0, sigma_p_cm2=1e-35, interpolation_points=1000), dict(capture_target=300)),
     (dmsun.MODE_PARAMETER_POINT, dict(mass_GeV=0.3, sigma_p_cm2=1e-36, interpolation_points=1000),
      dict(max_trajectories=3000, max_scatterings=10**14, n_bins=1000)),
+    # the reference's default histogram (NUM_BINS = 2000, include/Simulation_Trajectory.hpp:27-30)
+    (dmsun.MODE_PARAMETER_POINT, dict(mass_GeV=0.3, sigma_p_cm2=1e-36, interpolation_points=1000),
+     dict(max_trajectories=2000, max_scatterings=10**14, n_bins=2000, first_trajectory_id=40_000)),
 ])
 def test_job_matches_oracle(gpu_ctx, oracle_solar, mode, desc_args, kw):
     desc = dmsun.make_point(**desc_args)
