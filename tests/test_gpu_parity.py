@@ -181,6 +181,30 @@ def test_job_matches_oracle(gpu_ctx, oracle_solar, mode, desc_args, kw):
     assert_job_equal(res, ores)
 
 
+def test_edge_jobs(gpu_ctx, oracle_solar):
+    """A single trajectory, and zero cross sections (no scattering: every
+    trajectory crosses the Sun once and leaves), equal the oracle's runs; a job
+    with neither a trajectory count nor a capture target and a point with a
+    non-positive mass are refused with an error before any launch."""
+    for desc_args, n in [(dict(mass_GeV=0.3, sigma_p_cm2=1e-36, interpolation_points=200), 1),
+                         (dict(mass_GeV=0.3, sigma_p_cm2=0.0, interpolation_points=200), 500)]:
+        print(f"edge job {desc_args} n={n}", flush=True)
+        desc = dmsun.make_point(**desc_args)
+        job = dmsun.make_job(mode=dmsun.MODE_PARAMETER_POINT, max_trajectories=n, capture_target=0,
+                             max_scatterings=10**14, n_bins=1000, rk_attempt_budget=RK_BUDGET)
+        res = gpu_ctx.point(desc).run_job(job)
+        ores = oracle_lib.Point(oracle_solar, desc).run_job(job, record_capacity=n)
+        assert_job_equal(res, ores)
+        if desc_args["sigma_p_cm2"] == 0.0:
+            assert res["counters"].total_scatterings == 0
+    desc = dmsun.make_point(0.3, 1e-36, interpolation_points=200)
+    empty = dmsun.make_job(mode=dmsun.MODE_PARAMETER_POINT, max_trajectories=0, capture_target=0, n_bins=1000)
+    with pytest.raises(RuntimeError):
+        gpu_ctx.point(desc).run_job(empty)
+    with pytest.raises(RuntimeError):
+        gpu_ctx.point(dmsun.make_point(-1.0, 1e-36, interpolation_points=200))
+
+
 def assert_job_equal(res, ores):
     c, oc = res["counters"], ores["counters"]
     for f, _ in dmsun.Counters._fields_:
