@@ -80,6 +80,13 @@ size_t or_sample_target_velocities(uint32_t seed, double temperature, double tar
                                    double* out);
 int or_new_dm_velocity(uint32_t seed, double cos_a, double mDM, double mT, double* vel, const double* vT);
 
+/* Test hooks of the known-answer checks: Solve_Linear_Rate_Optical_Depth_Fraction,
+ * Solar_Interior_Fraction_Interval (returns 0 when the step has no interior
+ * part) and the total cross section of target i (< 0: electron) in cm^2. */
+double or_solve_linear_rate(double rs, double re, double dt, double target_tau);
+int or_interior_interval(const dms_event* b, const dms_event* a, double r_before, double r_after, double* start, double* end);
+double or_point_sigma_target(const or_point* P, int i, double v);
+
 /* Simulation_Data::Generate_Data. */
 void or_probe_math(int fn, const double* x, const double* y, double* out, size_t n);
 int or_run_job(const or_point* P, const dms_job_desc* job, int n_threads, dms_counters* counters,

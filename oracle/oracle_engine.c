@@ -1398,3 +1398,16 @@ int or_new_dm_velocity(uint32_t seed, double cos_a, double mDM, double mT, doubl
 	dm_rng_init(&rng, seed, 0, 0, DMS_STREAM_SIM);
 	return new_dm_velocity(&rng, cos_a, mDM, mT, vel, vT);
 }
+
+/* ---- test hooks: engine helpers exposed for the oracle's known-answer checks -- */
+double or_solve_linear_rate(double rs, double re, double dt, double target_tau)
+{
+	return solve_linear_rate(rs, re, dt, target_tau);
+}
+
+int or_interior_interval(const dms_event* b, const dms_event* a, double r_before, double r_after, double* start, double* end)
+{
+	*start = 0.0;
+	*end = 0.0;
+	return interior_interval(b, a, r_before, r_after, start, end);
+}

@@ -714,3 +714,12 @@ void or_probe_math(int fn, const double* x, const double* y, double* out, size_t
 		out[i] = r;
 	}
 }
+
+/* Test hooks: total cross sections of the point's DM model in cm^2 at DM speed
+ * v (natural units, c = 1): nucleus i (0-based isotope index) or, for i < 0,
+ * the electron. */
+double or_point_sigma_target(const or_point* P, int i, double v)
+{
+	const double s = (i < 0) ? or_sigma_electron(&P->dm, v) : or_sigma_nucleus(&P->dm, P->S, i, v);
+	return s / (OR_cm * OR_cm);
+}

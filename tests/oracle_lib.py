@@ -73,6 +73,10 @@ def lib():
             "or_new_dm_velocity": (C.c_int, [C.c_uint32, C.c_double, C.c_double, C.c_double, _P(C.c_double),
                                              _P(C.c_double)]),
             "or_probe_math": (None, [C.c_int, _P(C.c_double), _P(C.c_double), _P(C.c_double), C.c_size_t]),
+            "or_solve_linear_rate": (C.c_double, [C.c_double, C.c_double, C.c_double, C.c_double]),
+            "or_interior_interval": (C.c_int, [_P(dmsun.Event), _P(dmsun.Event), C.c_double, C.c_double,
+                                               _P(C.c_double), _P(C.c_double)]),
+            "or_point_sigma_target": (C.c_double, [vp, C.c_int, C.c_double]),
             "or_run_job": (C.c_int, [vp, _P(dmsun.JobDesc), C.c_int, _P(dmsun.Counters), _P(C.c_double),
                                      _P(C.c_uint64), _P(dmsun.EvapEvent), C.c_size_t, _P(C.c_size_t),
                                      _P(dmsun.TrajRecord), C.c_size_t, _P(C.c_size_t)]),
