@@ -67,7 +67,7 @@ struct dms_point
 	double* d_sigmaN = nullptr;
 	double* d_grid = nullptr;
 	float* d_grid32 = nullptr;
-	int N = 0;
+	int Nr = 0, Nv = 0; /* rate grid radii x speeds; 0 x 0 without a grid */
 	DmsPointView dev;
 };
 
@@ -143,12 +143,13 @@ DmsPointView host_point_view(const dms_point* pt)
 	v.q_reference = pt->P.q_reference;
 	v.sigma_e = pt->P.sigma_e;
 	v.sigma_N = pt->P.sigma_N;
-	v.use_grid = pt->N >= 2;
-	v.Nr = pt->N;
-	v.Nv = pt->N;
-	v.inv_dr = pt->N >= 2 ? (double) (pt->N - 1) / DMS_rSun : 0.0;
-	v.inv_dv = pt->N >= 2 ? (double) (pt->N - 1) / DMS_V_MAX : 0.0;
-	v.vmax = pt->N >= 2 ? DMS_V_MAX : 0.0;
+	const bool grid = pt->Nr >= 2 && pt->Nv >= 2;
+	v.use_grid = grid;
+	v.Nr = pt->Nr;
+	v.Nv = pt->Nv;
+	v.inv_dr = grid ? (double) (pt->Nr - 1) / DMS_rSun : 0.0;
+	v.inv_dv = grid ? (double) (pt->Nv - 1) / DMS_V_MAX : 0.0;
+	v.vmax = grid ? DMS_V_MAX : 0.0;
 	v.grid = nullptr;
 	v.grid32 = nullptr;
 	return v;
@@ -1307,13 +1308,20 @@ int dms_point_create(dms_ctx* ctx, const dms_point_desc* desc, dms_point** out)
 		const bool unsupported = desc->interaction == DMS_SD || desc->low_mass != 1;
 		return fail(ctx, unsupported ? DMS_ERR_UNSUPPORTED : DMS_ERR_ARG, "dms_point_create: " + msg);
 	}
-	const int N = desc->interpolation_points;
-	if(N > 20000)
+	/* Interpolate_Total_DM_Scattering_Rate(DM, N_radius, N_speed): either count
+	 * below 2 disables the grid (src/Solar_Model.cpp:397-406) */
+	const int Nr = desc->interpolation_points;
+	const int Nv = (desc->speed_points == 0) ? Nr : desc->speed_points;
+	if(Nr > 20000 || Nv > 20000)
 	{
 		delete pt;
-		return fail(ctx, DMS_ERR_ARG, "dms_point_create: interpolation_points too large");
+		return fail(ctx, DMS_ERR_ARG, "dms_point_create: interpolation_points or speed_points too large");
 	}
-	pt->N = (N >= 2) ? N : 0;
+	const bool grid = Nr >= 2 && Nv >= 2;
+	pt->Nr = grid ? Nr : 0;
+	pt->Nv = grid ? Nv : 0;
+	pt->desc.interpolation_points = pt->Nr;
+	pt->desc.speed_points = pt->Nv;
 	std::lock_guard<std::mutex> lock(ctx->mu);
 	hipSetDevice(ctx->device); /* callers may be on any host thread */
 	auto bail = [&](const std::string& what) {
@@ -1327,9 +1335,9 @@ int dms_point_create(dms_ctx* ctx, const dms_point_desc* desc, dms_point** out)
 		return bail("sigma upload failed");
 	pt->dev = host_point_view(pt);
 	pt->dev.sigma_N = pt->d_sigmaN;
-	if(pt->N >= 2)
+	if(grid)
 	{
-		const size_t cells = (size_t) pt->N * (size_t) pt->N;
+		const size_t cells = (size_t) pt->Nr * (size_t) pt->Nv;
 		if(hipMalloc((void**) &pt->d_grid, sizeof(double) * cells) != hipSuccess)
 			return bail("grid allocation failed");
 		if(desc->host_rate_grid != nullptr)
@@ -1341,7 +1349,7 @@ int dms_point_create(dms_ctx* ctx, const dms_point_desc* desc, dms_point** out)
 		{
 			(void) hipGetLastError();
 			hipLaunchKernelGGL(dms_grid_kernel, dim3((unsigned) ((cells + 255) / 256)), dim3(256), 0, ctx->stream, pt->dev, ctx->dev,
-			                   pt->N, pt->N, DMS_V_MAX, pt->d_grid);
+			                   pt->Nr, pt->Nv, DMS_V_MAX, pt->d_grid);
 			if(hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess)
 				return bail("rate-grid kernel failed");
 		}
@@ -1375,13 +1383,24 @@ int dms_point_rate_grid(const dms_point* pt, double* out, size_t n)
 {
 	if(pt == nullptr || out == nullptr)
 		return fail(nullptr, DMS_ERR_ARG, "dms_point_rate_grid: null argument");
-	if(pt->N < 2)
+	if(pt->Nr < 2)
 		return fail(pt->ctx, DMS_ERR_STATE, "dms_point_rate_grid: point has no rate grid");
-	const size_t cells = (size_t) pt->N * (size_t) pt->N;
+	const size_t cells = (size_t) pt->Nr * (size_t) pt->Nv;
 	if(n < cells)
 		return fail(pt->ctx, DMS_ERR_ARG, "dms_point_rate_grid: output too small");
 	hipSetDevice(pt->ctx->device);
 	DMS_HIP(pt->ctx, hipMemcpy(out, pt->d_grid, sizeof(double) * cells, hipMemcpyDeviceToHost));
+	return DMS_OK;
+}
+
+int dms_point_grid_shape(const dms_point* pt, int32_t* n_radius, int32_t* n_speed)
+{
+	if(pt == nullptr)
+		return fail(nullptr, DMS_ERR_ARG, "dms_point_grid_shape: null argument");
+	if(n_radius)
+		*n_radius = pt->Nr;
+	if(n_speed)
+		*n_speed = pt->Nv;
 	return DMS_OK;
 }
 

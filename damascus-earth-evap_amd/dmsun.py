@@ -55,7 +55,7 @@ class PointDesc(C.Structure):
     _fields_ = [("mass_GeV", C.c_double), ("sigma_p_cm2", C.c_double), ("sigma_e_cm2", C.c_double),
                 ("interaction", C.c_int32), ("form_factor", C.c_int32), ("mediator_mass_MeV", C.c_double),
                 ("fn_over_fp", C.c_double), ("low_mass", C.c_int32), ("interpolation_points", C.c_int32),
-                ("host_rate_grid", C.POINTER(C.c_double))]
+                ("speed_points", C.c_int32), ("host_rate_grid", C.POINTER(C.c_double))]
 
 
 class JobDesc(C.Structure):
@@ -128,7 +128,9 @@ def default_halo(vobs=(0.0, 248.0, 0.0)):
 
 
 def make_point(mass_GeV, sigma_p_cm2, sigma_e_cm2=0.0, interaction=SI, form_factor=FF_CONTACT,
-               mediator_mass_MeV=0.0, interpolation_points=1000, fn_over_fp=1.0):
+               mediator_mass_MeV=0.0, interpolation_points=1000, fn_over_fp=1.0, speed_points=0):
+    """interpolation_points x speed_points rate grid (radius-major); speed_points
+    0 is the square grid, either count below 2 no grid (direct rates)."""
     p = PointDesc()
     p.mass_GeV = mass_GeV
     p.sigma_p_cm2 = sigma_p_cm2
@@ -139,6 +141,7 @@ def make_point(mass_GeV, sigma_p_cm2, sigma_e_cm2=0.0, interaction=SI, form_fact
     p.fn_over_fp = fn_over_fp
     p.low_mass = 1
     p.interpolation_points = interpolation_points
+    p.speed_points = speed_points
     p.host_rate_grid = None
     return p
 
@@ -197,6 +200,7 @@ def _declare(lib):
         "dms_point_destroy": (None, [vp]),
         "dms_point_rate_grid": (C.c_int, [vp, _P(C.c_double), C.c_size_t]),
         "dms_point_cross_sections": (C.c_int, [vp, _P(C.c_double), _P(C.c_double)]),
+        "dms_point_grid_shape": (C.c_int, [vp, _P(C.c_int32), _P(C.c_int32)]),
         "dms_point_rates": (C.c_int, [vp, C.c_int, _P(C.c_double), _P(C.c_double), _P(C.c_double),
                                       C.c_size_t]),
         "dms_initial_conditions": (C.c_int, [vp, _P(Halo), C.c_uint32, C.c_uint32, C.c_uint64, C.c_size_t,
@@ -227,7 +231,7 @@ def _declare(lib):
 
 EXPORTED = ["dms_abi_version", "dms_create", "dms_destroy", "dms_last_error", "dms_set_job_memory_budget", "dms_load_solar_file",
             "dms_solar_eval", "dms_isotopes", "dms_constant", "dms_point_create", "dms_point_destroy",
-            "dms_point_rate_grid", "dms_point_cross_sections", "dms_point_rates", "dms_initial_conditions",
+            "dms_point_rate_grid", "dms_point_grid_shape", "dms_point_cross_sections", "dms_point_rates", "dms_initial_conditions",
             "dms_simulate_events", "dms_job_submit", "dms_job_submit_batch", "dms_job_submit_multi", "dms_multi_autotune", "dms_job_poll", "dms_job_wait", "dms_job_read",
             "dms_job_timing", "dms_job_destroy", "dms_probe_math"]
 
@@ -372,11 +376,16 @@ class Point:
         except Exception:
             pass
 
+    def grid_shape(self):
+        nr, nv = C.c_int32(), C.c_int32()
+        self.ctx.check(lib().dms_point_grid_shape(self.h, C.byref(nr), C.byref(nv)), "dms_point_grid_shape")
+        return nr.value, nv.value
+
     def rate_grid(self):
-        n = self.desc.interpolation_points
-        out = np.empty(n * n, dtype=np.float64)
+        nr, nv = self.grid_shape()
+        out = np.empty(nr * nv, dtype=np.float64)
         self.ctx.check(lib().dms_point_rate_grid(self.h, dptr(out), out.size), "dms_point_rate_grid")
-        return out.reshape(n, n)
+        return out.reshape(nr, nv)
 
     def rates(self, which, r, v):
         r = np.ascontiguousarray(r, dtype=np.float64)

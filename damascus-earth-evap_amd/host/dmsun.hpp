@@ -135,7 +135,8 @@ class DM_Particle
 	void Set_FormFactor_DM(const std::string& ff, double mediator);
 	double Sigma_Proton() const { return sigma_proton; }
 	double Sigma_Electron() const { return sigma_electron; }
-	dms_point_desc Descriptor(int interpolation_points) const;
+	/* the point with an N_radius x N_speed rate grid (N_speed 0: N_radius) */
+	dms_point_desc Descriptor(int interpolation_points, int speed_points = 0) const;
 };
 DM_Particle DM_Particle_SI(double mass);
 DM_Particle DM_Particle_Dark_Photon(double mass, double sigma_electron);
@@ -172,10 +173,11 @@ class Solar_Model
 	};
 	const std::vector<Isotope>& Target_Isotopes() const { return isotopes; }
 
-	/* src/Solar_Model.cpp:395-465: builds the N_r x N_v table for DM on the GPU
-	 * (this build tabulates square grids: N_r == N_v). */
+	/* src/Solar_Model.cpp:395-465: builds the N_r x N_v table for DM on the GPU;
+	 * N_r < 2 or N_v < 2 turns the table off (rates are then computed). */
 	void Interpolate_Total_DM_Scattering_Rate(const DM_Particle& DM, unsigned int N_r, unsigned int N_v);
-	unsigned int Scattering_Rate_Interpolation_Radius_Points() const { return interpolation_points; }
+	unsigned int Scattering_Rate_Interpolation_Radius_Points() const { return radius_points; }
+	unsigned int Scattering_Rate_Interpolation_Speed_Points() const { return speed_points; }
 	double Total_DM_Scattering_Rate(const DM_Particle& DM, double r, double v);
 	double Total_DM_Scattering_Rate_Computed(const DM_Particle& DM, double r, double v);
 	void Print_Summary() const;
@@ -190,7 +192,7 @@ class Solar_Model
 	std::vector<Isotope> isotopes;
 	dms_point* point = nullptr;
 	dms_point_desc point_desc{};
-	unsigned int interpolation_points = 0;
+	unsigned int radius_points = 0, speed_points = 0;
 	double eval(int which, double r) const;
 	double rate(const DM_Particle& DM, int which, double r, double v);
 };

@@ -7,6 +7,7 @@
 #include <thread>
 #include <cstring>
 #include <iostream>
+#include <limits>
 #include <stdexcept>
 
 #include "dmsun.hpp"
@@ -153,21 +154,23 @@ double Solar_Model::Number_Density_Nucleus(double r, unsigned int isotope) const
 
 void Solar_Model::Interpolate_Total_DM_Scattering_Rate(const DM_Particle& DM, unsigned int N_r, unsigned int N_v)
 {
-	if(N_r != N_v)
-		throw std::invalid_argument("Interpolate_Total_DM_Scattering_Rate: this build tabulates square grids (N_r == N_v)");
+	const bool grid = N_r >= 2 && N_v >= 2;
+	if(grid && ((unsigned long long) N_r * N_v > (unsigned long long) std::numeric_limits<int>::max()))
+		throw std::invalid_argument("scattering-rate interpolation grid is too large");
 	dms_point* p = nullptr;
-	const dms_point_desc d = DM.Descriptor((int) N_r);
+	const dms_point_desc d = grid ? DM.Descriptor((int) N_r, (int) N_v) : DM.Descriptor(0);
 	check(dms_point_create(ctx, &d, &p), "Interpolate_Total_DM_Scattering_Rate", ctx);
 	if(point)
 		dms_point_destroy(point);
 	point = p;
 	point_desc = d;
-	interpolation_points = (N_r >= 2) ? N_r : 0;
+	radius_points = grid ? N_r : 0;
+	speed_points = grid ? N_v : 0;
 }
 
 dms_point* Solar_Model::Point_For(const DM_Particle& DM)
 {
-	const dms_point_desc d = DM.Descriptor((int) interpolation_points);
+	const dms_point_desc d = DM.Descriptor((int) radius_points, (int) speed_points);
 	if(point && same_point(d, point_desc))
 		return point;
 	/* no table for this particle: direct rates, as the reference without a table */
@@ -178,7 +181,8 @@ dms_point* Solar_Model::Point_For(const DM_Particle& DM)
 		dms_point_destroy(point);
 	point = p;
 	point_desc = direct;
-	interpolation_points = 0;
+	radius_points = 0;
+	speed_points = 0;
 	return point;
 }
 
@@ -196,7 +200,7 @@ double Solar_Model::Total_DM_Scattering_Rate_Computed(const DM_Particle& DM, dou
 void Solar_Model::Print_Summary() const
 {
 	std::cout << "----------------------------------------\n"
-	          << name << "\n\tTarget isotopes:\t" << isotopes.size() << "\n\tRate interpolation points:\t" << interpolation_points
+	          << name << "\n\tTarget isotopes:\t" << isotopes.size() << "\n\tRate interpolation points:\t" << radius_points << " x " << speed_points
 	          << "\n----------------------------------------" << std::endl;
 }
 

@@ -708,7 +708,7 @@ DM_Particle DM_Particle_Dark_Photon(double mass, double sigma_electron)
 	return d;
 }
 
-dms_point_desc DM_Particle::Descriptor(int interpolation_points) const
+dms_point_desc DM_Particle::Descriptor(int interpolation_points, int speed_points) const
 {
 	dms_point_desc p{};
 	p.mass_GeV = mass / units::GeV();
@@ -720,6 +720,7 @@ dms_point_desc DM_Particle::Descriptor(int interpolation_points) const
 	p.fn_over_fp = fn_over_fp;
 	p.low_mass = low_mass ? 1 : 0;
 	p.interpolation_points = interpolation_points;
+	p.speed_points = speed_points;
 	p.host_rate_grid = nullptr;
 	return p;
 }

@@ -35,7 +35,8 @@ bool same_desc(const dms_point_desc& a, const dms_point_desc& b)
 {
 	return a.mass_GeV == b.mass_GeV && a.sigma_p_cm2 == b.sigma_p_cm2 && a.sigma_e_cm2 == b.sigma_e_cm2
 	       && a.interaction == b.interaction && a.form_factor == b.form_factor && a.mediator_mass_MeV == b.mediator_mass_MeV
-	       && a.fn_over_fp == b.fn_over_fp && a.low_mass == b.low_mass && a.interpolation_points == b.interpolation_points;
+	       && a.fn_over_fp == b.fn_over_fp && a.low_mass == b.low_mass && a.interpolation_points == b.interpolation_points
+	       && a.speed_points == b.speed_points;
 }
 } // namespace
 

@@ -31,7 +31,7 @@
 extern "C" {
 #endif
 
-#define DMS_ABI_VERSION 1
+#define DMS_ABI_VERSION 2
 
 enum dms_status
 {
@@ -114,8 +114,15 @@ typedef struct
 	double mediator_mass_MeV;  /* DP "General" only */
 	double fn_over_fp;         /* SI neutron/proton coupling ratio (1 = isospin conserving) */
 	int32_t low_mass;          /* DM_light; only 1 is supported */
-	int32_t interpolation_points; /* N for an N x N rate grid; < 2 disables the grid */
-	const double* host_rate_grid; /* optional N*N radius-major grid to upload instead of building it */
+	/* Rate grid of Interpolate_Total_DM_Scattering_Rate(DM, N_radius, N_speed)
+	 * (src/Solar_Model.cpp:395-465): interpolation_points radii in [0, R_sun]
+	 * by speed_points speeds in [0, 0.75 c], radius-major.  speed_points = 0
+	 * takes N_speed = N_radius (the square grid main.cpp builds from the config
+	 * key interpolation_points, src/main.cpp:80).  Either count below 2
+	 * disables the grid (:397-406): every rate is then computed directly. */
+	int32_t interpolation_points; /* N_radius */
+	int32_t speed_points;         /* N_speed (0: N_radius) */
+	const double* host_rate_grid; /* optional N_radius*N_speed grid to upload instead of building it */
 } dms_point_desc;
 
 /* One batch job: Simulation_Data::Generate_Data (include/Data_Generation.hpp:134). */
@@ -259,12 +266,15 @@ double dms_constant(const char* name);
 /* ---- parameter points ---------------------------------------------------- */
 
 /* DM particle + Solar_Model::Interpolate_Total_DM_Scattering_Rate
- * (src/Solar_Model.cpp:395-465): builds the radius-major N x N grid on the GPU
- * (or uploads host_rate_grid). */
+ * (src/Solar_Model.cpp:395-465): builds the radius-major N_radius x N_speed
+ * grid on the GPU (or uploads host_rate_grid). */
 int dms_point_create(dms_ctx* ctx, const dms_point_desc* desc, dms_point** point);
 void dms_point_destroy(dms_point* point);
-/* Copy the rate grid (N*N doubles, radius-major) to host. */
+/* Copy the rate grid (N_radius*N_speed doubles, radius-major) to host. */
 int dms_point_rate_grid(const dms_point* point, double* out, size_t n);
+/* Grid shape in use (Scattering_Rate_Interpolation_Radius_Points /
+ * _Speed_Points, include/Solar_Model.hpp:74-75): 0 x 0 without a grid. */
+int dms_point_grid_shape(const dms_point* point, int32_t* n_radius, int32_t* n_speed);
 /* Effective proton/electron cross sections in cm^2 (DP derives sigma_p). */
 int dms_point_cross_sections(const dms_point* point, double* sigma_p_cm2, double* sigma_e_cm2);
 /* Solar_Model::Total_DM_Scattering_Rate (+_Computed, per-target rates) evaluated

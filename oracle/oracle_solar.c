@@ -634,18 +634,21 @@ or_point* or_point_create(const or_solar* S, const dms_point_desc* desc, int n_t
 		free(P);
 		return NULL;
 	}
-	const int N = desc->interpolation_points;
-	if(N < 2)
+	/* Interpolate_Total_DM_Scattering_Rate(DM, N_radius, N_speed),
+	 * src/Solar_Model.cpp:395-465: either count below 2 disables the grid */
+	const int Nr = desc->interpolation_points;
+	const int Nv = (desc->speed_points == 0) ? Nr : desc->speed_points;
+	if(Nr < 2 || Nv < 2)
 	{
 		P->use_grid = 0;
 		return P;
 	}
-	P->Nr = N;
-	P->Nv = N;
+	P->Nr = Nr;
+	P->Nv = Nv;
 	P->vmax = 0.75;
-	P->grid = (double*) malloc(sizeof(double) * (size_t) N * (size_t) N);
+	P->grid = (double*) malloc(sizeof(double) * (size_t) Nr * (size_t) Nv);
 	if(desc->host_rate_grid != NULL)
-		memcpy(P->grid, desc->host_rate_grid, sizeof(double) * (size_t) N * (size_t) N);
+		memcpy(P->grid, desc->host_rate_grid, sizeof(double) * (size_t) Nr * (size_t) Nv);
 	else
 	{
 		if(n_threads < 1)
@@ -657,15 +660,15 @@ or_point* or_point_create(const or_solar* S, const dms_point_desc* desc, int n_t
 		for(int k = 0; k < n_threads; k++)
 		{
 			tasks[k].P = P;
-			tasks[k].row0 = (int) ((long long) N * k / n_threads);
-			tasks[k].row1 = (int) ((long long) N * (k + 1) / n_threads);
+			tasks[k].row0 = (int) ((long long) Nr * k / n_threads);
+			tasks[k].row1 = (int) ((long long) Nr * (k + 1) / n_threads);
 			pthread_create(&th[k], NULL, or_grid_worker, &tasks[k]);
 		}
 		for(int k = 0; k < n_threads; k++)
 			pthread_join(th[k], NULL);
 	}
-	P->inv_dr = (double) (N - 1) / OR_rSun;
-	P->inv_dv = (double) (N - 1) / P->vmax;
+	P->inv_dr = (double) (Nr - 1) / OR_rSun;
+	P->inv_dv = (double) (Nv - 1) / P->vmax;
 	P->use_grid = 1;
 	return P;
 }

@@ -27,7 +27,7 @@ struct rp_point
 	dms::PointParams P;
 	std::vector<double> grid;
 	std::vector<float> grid32;
-	int N;
+	int Nr, Nv;
 	DmsPointView view;
 };
 
@@ -79,7 +79,11 @@ rp_point* rp_point_create(rp_ctx* c, const dms_point_desc* d)
 		delete p;
 		return nullptr;
 	}
-	p->N = d->interpolation_points >= 2 ? d->interpolation_points : 0;
+	const int Nr = d->interpolation_points;
+	const int Nv = (d->speed_points == 0) ? Nr : d->speed_points;
+	const bool grid = Nr >= 2 && Nv >= 2;
+	p->Nr = grid ? Nr : 0;
+	p->Nv = grid ? Nv : 0;
 	DmsPointView& v = p->view;
 	v.mass = p->P.mass;
 	v.mu_e = p->P.mu_e;
@@ -89,30 +93,31 @@ rp_point* rp_point_create(rp_ctx* c, const dms_point_desc* d)
 	v.q_reference = p->P.q_reference;
 	v.sigma_e = p->P.sigma_e;
 	v.sigma_N = p->P.sigma_N;
-	v.use_grid = p->N >= 2;
-	v.Nr = p->N;
-	v.Nv = p->N;
-	v.inv_dr = p->N >= 2 ? (double) (p->N - 1) / DMS_rSun : 0.0;
-	v.inv_dv = p->N >= 2 ? (double) (p->N - 1) / DMS_V_MAX : 0.0;
-	v.vmax = p->N >= 2 ? DMS_V_MAX : 0.0;
+	v.use_grid = grid;
+	v.Nr = p->Nr;
+	v.Nv = p->Nv;
+	v.inv_dr = grid ? (double) (p->Nr - 1) / DMS_rSun : 0.0;
+	v.inv_dv = grid ? (double) (p->Nv - 1) / DMS_V_MAX : 0.0;
+	v.vmax = grid ? DMS_V_MAX : 0.0;
 	v.grid = nullptr;
 	v.grid32 = nullptr;
-	if(p->N >= 2)
+	if(grid)
 	{
+		const size_t cells = (size_t) p->Nr * (size_t) p->Nv;
 		if(d->host_rate_grid)
-			p->grid.assign(d->host_rate_grid, d->host_rate_grid + (size_t) p->N * p->N);
+			p->grid.assign(d->host_rate_grid, d->host_rate_grid + cells);
 		else
 		{
 			/* same node construction as dms_grid_kernel */
-			p->grid.resize((size_t) p->N * p->N);
+			p->grid.resize(cells);
 			const DmsSolarView S = c->T.view();
-			const double dv = (DMS_V_MAX - 0.0) / (double) (p->N - 1);
-			for(int i = 0; i < p->N; i++)
-				for(int j = 0; j < p->N; j++)
+			const double dv = (DMS_V_MAX - 0.0) / (double) (p->Nv - 1);
+			for(int i = 0; i < p->Nr; i++)
+				for(int j = 0; j < p->Nv; j++)
 				{
-					const double radius = DMS_rSun * (double) i / (double) (p->N - 1);
+					const double radius = DMS_rSun * (double) i / (double) (p->Nr - 1);
 					const double speed = 0.0 + j * dv;
-					p->grid[(size_t) i * p->N + j] = dms_rate_computed(v, S, radius, speed);
+					p->grid[(size_t) i * p->Nv + j] = dms_rate_computed(v, S, radius, speed);
 				}
 		}
 		v.grid = p->grid.data();

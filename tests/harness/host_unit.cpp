@@ -1,5 +1,7 @@
 /* Host-only checks of the C++ API (no GPU): configuration reader, run
  * statistics and the output files.  Prints key=value lines for the test. */
+#include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <iostream>
 #include <stdexcept>
@@ -27,6 +29,35 @@ int main(int argc, char** argv)
 			          << "\nrho=" << c.halo.rho_GeV_cm3 << "\nv0=" << c.halo.v0_kms << "\nvesc=" << c.halo.vesc_kms
 			          << "\nmax_scatterings=" << c.maximum_number_of_scatterings << "\nsnapshot=" << c.snapshot_enabled
 			          << "\nwall=" << c.max_trajectory_wall_time_sec << "\n";
+		}
+		else if(mode == "grid")
+		{
+			/* tests/test_Solar_Model.cpp:192-224 through the C++ Solar_Model (GPU) */
+			dmsun::Solar_Model SSM(0);
+			dmsun::DM_Particle DM = dmsun::DM_Particle_SI(0.01 * dmsun::units::GeV());
+			DM.Set_Low_Mass_Mode(true);
+			DM.Set_Sigma_Proton(1.0e-36 * dmsun::units::cm() * dmsun::units::cm());
+			const unsigned int radius_points = 31, speed_points = 29;
+			SSM.Interpolate_Total_DM_Scattering_Rate(DM, radius_points, speed_points);
+			std::printf("radius_points=%u\nspeed_points=%u\n", SSM.Scattering_Rate_Interpolation_Radius_Points(),
+			            SSM.Scattering_Rate_Interpolation_Speed_Points());
+			double worst = 0.0;
+			for(unsigned int i : {0u, 7u, 15u, 30u})
+				for(unsigned int j : {0u, 5u, 14u, 28u})
+				{
+					const double r = dmsun::units::rSun() * i / (radius_points - 1);
+					const double v = 0.75 * j / (speed_points - 1);
+					const double want = SSM.Total_DM_Scattering_Rate_Computed(DM, r, v);
+					const double got = SSM.Total_DM_Scattering_Rate(DM, r, v);
+					const double rel = std::fabs(got - want) / std::max(1.0e-300, std::fabs(want));
+					worst = std::max(worst, rel);
+				}
+			std::printf("node_rel_err=%.17g\n", worst);
+			SSM.Interpolate_Total_DM_Scattering_Rate(DM, 1, speed_points);
+			const double r = 0.37 * dmsun::units::rSun(), v = 0.21;
+			std::printf("off_radius_points=%u\noff_speed_points=%u\noff_equal=%d\n",
+			            SSM.Scattering_Rate_Interpolation_Radius_Points(), SSM.Scattering_Rate_Interpolation_Speed_Points(),
+			            (int) (SSM.Total_DM_Scattering_Rate(DM, r, v) == SSM.Total_DM_Scattering_Rate_Computed(DM, r, v)));
 		}
 		else if(mode == "percentile")
 		{

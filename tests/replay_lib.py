@@ -90,10 +90,11 @@ class ReplayPoint:
         return lib().rp_rate(self.h, which, r, v)
 
     def grid(self):
-        n = self.desc.interpolation_points
-        out = np.empty(n * n)
+        nr = self.desc.interpolation_points
+        nv = self.desc.speed_points or nr
+        out = np.empty(nr * nv)
         assert lib().rp_grid(self.h, dmsun.dptr(out), out.size) == 0
-        return out.reshape(n, n)
+        return out.reshape(nr, nv)
 
     def simulate_events(self, job, events, want_rows=False):
         n = len(events)

@@ -127,3 +127,20 @@ def test_driver_writes_progress_snapshots(tmp_path):
     snap = (tmp_path / "snapshots" / "snapshot_progress.txt").read_text()
     kv = dict(l.split(" = ") for l in snap.splitlines() if " = " in l)
     assert kv["finished"] == "1" and int(kv["trajectories_done"]) > 0
+
+
+def test_solar_model_non_square_grid():
+    """tests/test_Solar_Model.cpp:192-224 through the C++ Solar_Model: a 31 x 29
+    grid is exact at its nodes and reports both counts; (1, 29) turns the grid
+    off (0 x 0) and the total rate is the directly computed one."""
+    exe = REPO / "tests" / "harness" / "host_unit"
+    pkg = REPO / "damascus-earth-evap_amd"
+    if not exe.exists():
+        subprocess.run(["g++", "-O1", "-std=c++17", "-o", str(exe), str(REPO / "tests" / "harness" / "host_unit.cpp"),
+                        "-L", str(pkg), "-ldmsun_host", "-ldmsun", f"-Wl,-rpath,{pkg}"], check=True)
+    p = subprocess.run([str(exe), "grid"], capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stdout + p.stderr
+    out = dict(l.split("=", 1) for l in p.stdout.splitlines() if "=" in l)
+    assert out["radius_points"] == "31" and out["speed_points"] == "29"
+    assert float(out["node_rel_err"]) <= 1e-10
+    assert out["off_radius_points"] == "0" and out["off_speed_points"] == "0" and out["off_equal"] == "1"

@@ -20,6 +20,7 @@ HIST_RTOL = 1e-12
 # Per-trajectory RKF45-attempt budget (the reference's wall-time guard analogue);
 # far above the longest trajectory of these cases, it only bounds a runaway lane.
 RK_BUDGET = 20_000_000
+O_rSun = oracle_lib.const("rSun")
 
 
 def rec_bytes(recs):
@@ -114,14 +115,30 @@ def host_detmath(fn, x, y):
 
 @pytest.mark.parametrize("desc_args", [
     dict(mass_GeV=1.0, sigma_p_cm2=1e-35, interpolation_points=200),
+    dict(mass_GeV=0.01, sigma_p_cm2=1e-36, interpolation_points=31, speed_points=29),
+    dict(mass_GeV=0.3, sigma_p_cm2=1e-36, interpolation_points=500, speed_points=180),
     dict(mass_GeV=0.1, sigma_p_cm2=0.0, sigma_e_cm2=1e-37, interaction=dmsun.DARK_PHOTON,
          form_factor=dmsun.FF_GENERAL, mediator_mass_MeV=0.01, interpolation_points=120),
 ])
 def test_rate_grid_bit_identical(gpu_ctx, oracle_solar, desc_args):
     desc = dmsun.make_point(**desc_args)
-    g_dev = gpu_ctx.point(desc).rate_grid()
+    pt = gpu_ctx.point(desc)
+    g_dev = pt.rate_grid()
     g_cpu = oracle_lib.Point(oracle_solar, desc).grid()
+    assert g_dev.shape == g_cpu.shape == (desc.interpolation_points, desc.speed_points or desc.interpolation_points)
+    assert pt.grid_shape() == g_dev.shape
     assert np.array_equal(g_dev.view(np.uint64), g_cpu.view(np.uint64))
+
+
+def test_grid_turned_off_by_a_count_below_two(gpu_ctx, oracle_solar):
+    """src/Solar_Model.cpp:397-406 and tests/test_Solar_Model.cpp:218-224:
+    (1, 29) and (31, 1) leave no grid, so the total rate is the direct sum."""
+    for nr, nv in ((1, 29), (31, 1)):
+        pt = gpu_ctx.point(dmsun.make_point(0.01, 1e-36, interpolation_points=nr, speed_points=nv))
+        assert pt.grid_shape() == (0, 0)
+        r = np.array([0.37 * O_rSun, 0.8 * O_rSun])
+        v = np.array([0.21, 1e-3])
+        assert np.array_equal(pt.rates(0, r, v), pt.rates(1, r, v))
 
 
 def test_initial_conditions_bit_identical(gpu_ctx, oracle_solar):
@@ -146,6 +163,8 @@ CASES = [
      dict(mass_GeV=0.1, sigma_p_cm2=0.0, sigma_e_cm2=1e-37, interaction=dmsun.DARK_PHOTON, form_factor=dmsun.FF_CONTACT,
           interpolation_points=300), 24, 10**14),
     ("capture_si_0.5GeV_no_table", dmsun.MODE_CAPTURE, dict(mass_GeV=0.5, sigma_p_cm2=1e-36, interpolation_points=0), 64, 1000),
+    ("point_si_0.3GeV_grid_500x180", dmsun.MODE_PARAMETER_POINT,
+     dict(mass_GeV=0.3, sigma_p_cm2=1e-36, interpolation_points=500, speed_points=180), 256, 10**14),
 ]
 
 

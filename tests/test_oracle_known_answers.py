@@ -319,25 +319,32 @@ def test_rate_grid_within_ten_percent(point_lowmass):
 
 
 def test_rate_grid_exact_at_nodes(oracle_solar):
-    """tests/test_Solar_Model.cpp:192-242 (square grid of 31 points here)."""
-    N = 31
-    P = O.Point(oracle_solar, dmsun.make_point(0.01, 1e-36, interpolation_points=N))
+    """tests/test_Solar_Model.cpp:192-224: a 31 x 29 grid (radii x speeds) is
+    exact at its nodes; (1, 29) turns the grid off and the direct sum is used."""
+    Nr, Nv = 31, 29
+    P = O.Point(oracle_solar, dmsun.make_point(0.01, 1e-36, interpolation_points=Nr, speed_points=Nv))
+    g = P.grid()
+    assert g.shape == (Nr, Nv)
     for i in (0, 7, 15, 30):
-        for j in (0, 5, 14, 28, 30):
-            r = rSun * i / (N - 1)
-            v = 0.75 * j / (N - 1)
+        for j in (0, 5, 14, 28):
+            r = rSun * i / (Nr - 1)
+            v = 0.75 * j / (Nv - 1)
             exact = P.rate(1, r, v)
             assert abs(P.rate(0, r, v) - exact) <= max(1e-30, 1e-10 * abs(exact))
+            assert abs(g[i, j] - exact) <= max(1e-30, 1e-10 * abs(exact))
     # bilinear interpolation is exact on a plane: the grid's value between
-    # nodes is the weighted mean of the four corners
-    g = P.grid()
-    r = rSun * (7 + 0.25) / (N - 1)
-    v = 0.75 * (5 + 0.5) / (N - 1)
+    # nodes is the weighted mean of the four corners (radius-major, speed fastest)
+    r = rSun * (7 + 0.25) / (Nr - 1)
+    v = 0.75 * (5 + 0.5) / (Nv - 1)
     want = 0.75 * (0.5 * g[7, 5] + 0.5 * g[7, 6]) + 0.25 * (0.5 * g[8, 5] + 0.5 * g[8, 6])
     assert abs(P.rate(0, r, v) - want) <= 1e-12 * want
-    # one point: no grid, the direct sum is used
-    P1 = O.Point(oracle_solar, dmsun.make_point(0.01, 1e-36, interpolation_points=1))
-    assert P1.rate(0, 0.37 * rSun, 0.21) == P1.rate(1, 0.37 * rSun, 0.21)
+    # the same point with the axes' counts exchanged tabulates other nodes
+    Pt = O.Point(oracle_solar, dmsun.make_point(0.01, 1e-36, interpolation_points=Nv, speed_points=Nr))
+    assert Pt.grid().shape == (Nv, Nr) and not np.allclose(Pt.grid()[:, :Nv], g[:Nv, :], rtol=1e-6, atol=0.0)
+    # either count below 2: no grid, the direct sum is used
+    for nr, nv in ((1, Nv), (Nr, 1)):
+        P1 = O.Point(oracle_solar, dmsun.make_point(0.01, 1e-36, interpolation_points=nr, speed_points=nv))
+        assert P1.rate(0, 0.37 * rSun, 0.21) == P1.rate(1, 0.37 * rSun, 0.21)
 
 
 def test_rate_nan_in_nan_out(oracle_solar):

@@ -34,6 +34,7 @@ def test_solar_tables(replay, oracle_solar):
 
 @pytest.mark.parametrize("desc_args", [
     dict(mass_GeV=1.0, sigma_p_cm2=1e-35, interpolation_points=60),
+    dict(mass_GeV=0.01, sigma_p_cm2=1e-36, interpolation_points=31, speed_points=29),
     dict(mass_GeV=0.1, sigma_p_cm2=0.0, sigma_e_cm2=1e-37, interaction=dmsun.DARK_PHOTON,
          form_factor=dmsun.FF_GENERAL, mediator_mass_MeV=0.01, interpolation_points=40),
 ])
