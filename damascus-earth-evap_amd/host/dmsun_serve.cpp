@@ -6,7 +6,7 @@
  *               [--max-trajectories M] [--max-scatterings K] [--rk-budget B]
  *               [--precision fp64|mixed] [--concurrency C] [--memory-budget GiB]
  *               [--per-job-budget GiB] [--seed K] [--interp N] [--bins N]
- *               [--check-alone K] [--json PATH] [--device D] [--quiet]
+ *               [--check-alone K] [--json PATH] [--results PATH] [--device D] [--quiet]
  *
  * Requests arrive as a Poisson process of rate R per second.  Each asks for a
  * spin-independent DM point with the mass log-uniform in [0.1, 5] GeV and the
@@ -16,7 +16,9 @@
  * trajectory (heavy captured particles do not evaporate: K bounds them).
  * Latency is completion minus arrival.  --check-alone K reruns the first K
  * requests one at a time after the mix and compares their results with the
- * ones they got while sharing the GPU.  One JSON line summarises the run.
+ * ones they got while sharing the GPU.  One JSON line summarises the run;
+ * --results writes one JSON line per request: its point and job descriptor,
+ * counters, visit counts, histogram sums and evaporation events.
  */
 #include <algorithm>
 #include <chrono>
@@ -47,10 +49,47 @@ struct Options
 	unsigned int interp = 1000;
 	int bins = 2000;
 	int check_alone = 0;
-	std::string json;
+	std::string json, results;
 	int device = 0;
 	bool quiet = false;
 };
+
+/* one request and its result as a JSON line (doubles round-trip: %.17g) */
+std::string result_json(const dmsun::Request& q, const dmsun::Request_Result& r)
+{
+	const dms_point_desc d = q.DM.Descriptor((int) q.interpolation_points);
+	const dms_counters& c = r.counters;
+	std::ostringstream s;
+	s.precision(17);
+	s << "{\"id\": " << r.id << ", \"status\": " << r.status << ", \"mass_GeV\": " << d.mass_GeV
+	  << ", \"sigma_p_cm2\": " << d.sigma_p_cm2 << ", \"sigma_e_cm2\": " << d.sigma_e_cm2 << ", \"interp\": " << q.interpolation_points
+	  << ", \"sample_size\": " << q.sample_size << ", \"max_trajectories\": " << q.max_trajectories
+	  << ", \"max_scatterings\": " << q.max_scatterings << ", \"rk_budget\": " << q.rk_attempt_budget << ", \"seed\": " << q.seed
+	  << ", \"point_id\": " << q.point_id << ", \"precision\": " << q.precision << ", \"bins\": " << q.n_bins
+	  << ", \"halo\": {\"rho\": " << q.halo.rho_GeV_cm3 << ", \"v0\": " << q.halo.v0_kms << ", \"vesc\": " << q.halo.vesc_kms
+	  << ", \"vobs\": [" << q.halo.vobs_kms[0] << ", " << q.halo.vobs_kms[1] << ", " << q.halo.vobs_kms[2] << "]}"
+	  << ", \"latency_sec\": " << r.Latency() << ", \"counters\": {\"trajectories\": " << c.trajectories
+	  << ", \"captured\": " << c.captured << ", \"completed_outward_escapes\": " << c.completed_outward_escapes
+	  << ", \"free_particles\": " << c.free_particles << ", \"reflected_particles\": " << c.reflected_particles
+	  << ", \"complete_evaporation\": " << c.complete_evaporation << ", \"invalid_survival_captured\": " << c.invalid_survival_captured
+	  << ", \"numerical_failures\": " << c.numerical_failures << ", \"computational_truncations\": " << c.computational_truncations
+	  << ", \"total_scatterings\": " << c.total_scatterings << ", \"sync_rounds\": " << c.sync_rounds
+	  << ", \"capture_target_overshoot\": " << c.capture_target_overshoot << ", \"early_stop_reason\": " << c.early_stop_reason
+	  << ", \"reason_counts\": [";
+	for(int k = 0; k < 11; k++)
+		s << (k ? ", " : "") << c.reason_counts[k];
+	s << "]}, \"events\": [";
+	for(size_t k = 0; k < r.events.size(); k++)
+		s << (k ? ", " : "") << "[" << r.events[k].trajectory_id << ", " << r.events[k].lifetime_unbinding_sec << "]";
+	s << "], \"counts\": [";
+	for(size_t k = 0; k < r.counts.size(); k++)
+		s << (k ? ", " : "") << r.counts[k];
+	s << "], \"hist\": [";
+	for(size_t k = 0; k < r.hist.size(); k++)
+		s << (k ? ", " : "") << r.hist[k];
+	s << "]}";
+	return s.str();
+}
 
 bool same_result(const dmsun::Request_Result& a, const dmsun::Request_Result& b, std::string& why)
 {
@@ -148,6 +187,8 @@ int main(int argc, char** argv)
 				o.check_alone = std::stoi(next());
 			else if(a == "--json")
 				o.json = next();
+			else if(a == "--results")
+				o.results = next();
 			else if(a == "--device")
 				o.device = std::stoi(next());
 			else if(a == "--quiet")
@@ -282,6 +323,12 @@ int main(int argc, char** argv)
 		{
 			std::ofstream f(o.json);
 			f << js.str() << "\n";
+		}
+		if(!o.results.empty())
+		{
+			std::ofstream f(o.results);
+			for(size_t k = 0; k < mix.size(); k++)
+				f << result_json(reqs[mix[k].id], mix[k]) << "\n";
 		}
 		return (failed == 0 && mismatches == 0) ? 0 : 1;
 	}

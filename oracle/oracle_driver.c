@@ -246,10 +246,10 @@ int or_run_job(const or_point* P, const dms_job_desc* job, int n_threads, dms_co
 			{
 				B.first_id = next_id;
 				B.n = chunk;
+				/* chunks never exceed the buffers; a round longer than a chunk
+				 * takes several (cursor == have refills) */
 				if(!unlimited && max_traj - local_total < chunk)
 					B.n = (size_t) (max_traj - local_total);
-				if(B.n < batch - k)
-					B.n = (size_t) (batch - k);
 				run_batch(&B, n_threads);
 				if(B.status != DMS_OK)
 					status = B.status;

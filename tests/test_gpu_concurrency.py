@@ -142,3 +142,43 @@ def test_autotuned_order_keeps_results(multi_points):
     plain = dmsun.run_multi(multi_points, jobs)
     for a, b in zip(tuned, plain):
         assert_job_equal(a, b)
+
+
+def test_serving_requests_match_the_oracle(tmp_path, oracle_solar):
+    """The serving workload's request ranges (SI m ~ U_log(0.1, 5) GeV, sigma_p
+    ~ U_log(1e-38, 1e-35) cm^2, captured-particle targets 1e3 - 2e3) served
+    concurrently by dmsun_serve: every request's counters, visit counts and
+    evaporation events equal the oracle's Generate_Data run of the request's
+    own descriptor, its histogram sums to 1e-12."""
+    res_path = tmp_path / "results.jsonl"
+    cmd = [str(SERVE), "--requests", "4", "--rate", "20", "--sample-size", "1000", "--sample-size-max", "2000",
+           "--max-trajectories", "40000", "--max-scatterings", "500", "--rk-budget", "200000", "--interp", "300",
+           "--bins", "200", "--concurrency", "4", "--seed", "17", "--results", str(res_path), "--quiet"]
+    print(" ".join(cmd), flush=True)
+    p = subprocess.run(cmd, cwd=str(REPO), capture_output=True, text=True, timeout=200)
+    assert p.returncode == 0, p.stdout + p.stderr
+    lines = [json.loads(l) for l in res_path.read_text().splitlines()]
+    assert len(lines) == 4
+    for r in lines:
+        assert r["status"] == 0
+        desc = dmsun.make_point(r["mass_GeV"], r["sigma_p_cm2"], sigma_e_cm2=r["sigma_e_cm2"], interpolation_points=r["interp"])
+        h = r["halo"]
+        halo = dmsun.default_halo(tuple(h["vobs"]))
+        halo.rho_GeV_cm3, halo.v0_kms, halo.vesc_kms = h["rho"], h["v0"], h["vesc"]
+        job = dmsun.make_job(mode=dmsun.MODE_PARAMETER_POINT, max_trajectories=r["max_trajectories"],
+                             capture_target=r["sample_size"], seed=r["seed"], point_id=r["point_id"],
+                             max_scatterings=r["max_scatterings"], rk_attempt_budget=r["rk_budget"], n_bins=r["bins"],
+                             record=0, precision=r["precision"], halo=halo)
+        print(f"request {r['id']}: m={r['mass_GeV']:.4g} GeV sigma_p={r['sigma_p_cm2']:.4g} target {r['sample_size']}",
+              flush=True)
+        o = oracle_lib.Point(oracle_solar, desc, threads=16).run_job(job, threads=16, want_records=False)
+        oc = o["counters"]
+        for f, v in r["counters"].items():
+            got = list(getattr(oc, f)) if f == "reason_counts" else getattr(oc, f)
+            assert got == v, (r["id"], f, got, v)
+        assert r["counters"]["captured"] >= min(r["sample_size"], 1) or r["counters"]["early_stop_reason"] != 0
+        assert np.array_equal(np.array(r["counts"], dtype=np.uint64), o["counts"].ravel())
+        hist = np.array(r["hist"])
+        assert np.all(np.abs(hist - o["hist"].ravel()) <= 1e-12 * np.maximum(np.abs(o["hist"].ravel()), 1e-300))
+        assert [(int(a), float(b)) for a, b in r["events"]] == [(e.trajectory_id, e.lifetime_unbinding_sec)
+                                                                for e in o["events"]]

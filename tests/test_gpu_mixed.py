@@ -26,6 +26,7 @@ import pytest
 from scipy import stats
 
 import dmsun
+import oracle_lib
 
 pytestmark = pytest.mark.gpu
 
@@ -124,3 +125,86 @@ def test_mixed_batch_equals_alone(gpu_ctx):
             assert getattr(alone["counters"], f) == getattr(res["counters"], f), f
         assert np.array_equal(alone["counts"], res["counts"])
         assert np.allclose(alone["hist"], res["hist"], rtol=1e-12, atol=0)
+
+
+def two_sample_gates(ca, cb, ha, hb, ea, eb, n):
+    """The validation gates (validation/README.md:46-58) for two independent
+    samples of n trajectories: capture fraction within 3 sigma and overlapping
+    Wilson intervals, captured radial distributions within total variation
+    0.10, complete-evaporation fractions within 3 sigma, evaporation lifetimes
+    KS p > 1e-3 with median within 20% and 90th percentile within 30%, mean
+    scatterings within 10%.  (A chi^2 of the histograms is not among them:
+    between independent runs of the same FP64 oracle at this size it spreads
+    over 0.7-1.6 per degree of freedom, because one long captured trajectory
+    adds to hundreds of bins at once; it is applied to the paired runs.)"""
+    pa, pb = ca.captured / n, cb.captured / n
+    sig = math.sqrt(pa * (1 - pa) / n + pb * (1 - pb) / n)
+    assert abs(pa - pb) <= 3 * max(sig, 1.0 / n), (pa, pb)
+    wa, wb = wilson(ca.captured, n), wilson(cb.captured, n)
+    assert wa[0] <= wb[1] and wb[0] <= wa[1], (wa, wb)
+    ta, tb = ha[0], hb[0]
+    tv = 0.5 * float(np.abs(ta / ta.sum() - tb / tb.sum()).sum())
+    ea_, eb_ = ca.complete_evaporation / n, cb.complete_evaporation / n
+    sig_e = math.sqrt(ea_ * (1 - ea_) / n + eb_ * (1 - eb_) / n)
+    print(f"independent: capture {pa:.5f} vs {pb:.5f}, captured radial TV {tv:.4f}", flush=True)
+    assert tv <= 0.10
+    assert abs(ea_ - eb_) <= 3 * max(sig_e, 1.0 / n)
+    assert stats.ks_2samp(ea, eb).pvalue > 1e-3
+    for q, tol in ((50, 0.20), (90, 0.30)):
+        qa, qb = np.percentile(ea, q), np.percentile(eb, q)
+        assert abs(qa - qb) <= tol * abs(qa), (q, qa, qb)
+    ma, mb = ca.total_scatterings / n, cb.total_scatterings / n
+    assert abs(ma - mb) <= 0.10 * ma
+
+
+def hist_chi2_per_dof(a, b, na_cap, nb_cap, na_esc, nb_esc):
+    chi2, dof = 0.0, 0
+    for base, na, nb in [(0, na_cap, nb_cap), (4, na_esc, nb_esc)]:
+        for q in (0, 1):
+            var = bin_error(a[base + q], a[base + 2 + q], na) ** 2 + bin_error(b[base + q], b[base + 2 + q], nb) ** 2
+            use = var > 0
+            chi2 += float(np.sum((a[base + q][use] - b[base + q][use]) ** 2 / var[use]))
+            dof += int(use.sum())
+    return chi2 / max(dof, 1), dof
+
+
+def test_mixed_workload_against_the_oracle_cpu_run(gpu_ctx, oracle_solar):
+    """The mixed-precision workload (SI m = 0.5 GeV, sigma_p = 1e-37 cm^2,
+    parameter point) against the oracle's CPU run, the all-FP64 restatement:
+
+      * independent samples (10^5 trajectories each, disjoint ids): the
+        validation gates above;
+      * paired samples (the same 10^5 ids): histogram chi^2/dof < 1.2 with
+        per-bin errors of independent samples (conservative for positively
+        correlated runs), and McNemar's test of the capture flags,
+        |b - c| <= 3 sqrt(b + c) for the b and c trajectories that only one
+        of the two runs captures, which a bias far below the
+        independent-sample resolution fails."""
+    n = 100_000
+    desc = dmsun.make_point(0.5, 1e-37, interpolation_points=1000)
+    halo = dmsun.default_halo((0.0, 248.0, 0.0))
+
+    def job(first, precision, record):
+        return dmsun.make_job(mode=dmsun.MODE_PARAMETER_POINT, max_trajectories=n, capture_target=0, max_scatterings=10**14,
+                              n_bins=500, record=record, precision=precision, first_trajectory_id=first, halo=halo)
+
+    print("mixed workload vs oracle: 1e5 trajectories, paired and independent", flush=True)
+    mx = gpu_ctx.point(desc).run_job(job(0, dmsun.MIXED, 1), want_records=True)
+    opt = oracle_lib.Point(oracle_solar, desc, threads=16)
+    paired = opt.run_job(job(0, dmsun.FP64, 1), threads=16, record_capacity=n)
+    indep = opt.run_job(job(10**7, dmsun.FP64, 0), threads=16, want_records=False)
+    ev = lambda r: np.array([e.lifetime_unbinding_sec for e in r["events"]])
+    cm, cp = mx["counters"], paired["counters"]
+    two_sample_gates(cm, indep["counters"], mx["hist"], indep["hist"], ev(mx), ev(indep), n)
+    chi, dof = hist_chi2_per_dof(mx["hist"], paired["hist"], cm.captured, cp.captured, cm.completed_outward_escapes,
+                                 cp.completed_outward_escapes)
+    a = np.array([bool(r.flags & dmsun.F_CAPTURED) for r in mx["records"]])
+    b = np.array([bool(r.flags & dmsun.F_CAPTURED) for r in paired["records"]])
+    assert a.size == b.size == n
+    only_mixed, only_fp64 = int(np.sum(a & ~b)), int(np.sum(~a & b))
+    print(f"paired: chi2/dof {chi:.3f} over {dof} bins; captured only by mixed {only_mixed}, only by FP64 {only_fp64}, "
+          f"both {int(np.sum(a & b))}", flush=True)
+    assert dof > 50 and chi < 1.2
+    assert abs(only_mixed - only_fp64) <= 3 * math.sqrt(max(1, only_mixed + only_fp64))
+    for c in (cm, cp):
+        assert c.numerical_failures <= 1e-3 * n

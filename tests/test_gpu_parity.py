@@ -270,3 +270,22 @@ def test_batch_rejects_mixed_descriptors(gpu_ctx):
     c = dmsun.make_job(mode=dmsun.MODE_CAPTURE, max_trajectories=0, capture_target=5)
     with pytest.raises(dmsun.DmsError):
         gpu_ctx.point(desc).run_batch([a, c])
+
+
+def test_capture_workload_full_size(gpu_ctx, oracle_solar):
+    """The capture-mode workload at its full size: SI contact m = 1 GeV,
+    sigma_p = 1e-35 cm^2, 10^4 trajectories, at most 1000 scatterings, 1000 x
+    1000 rate grid.  Every record bit for bit, and the run's counters, equal
+    the oracle's Generate_Data run."""
+    desc = dmsun.make_point(1.0, 1e-35, interpolation_points=1000)
+    n = 10_000
+    job = dmsun.make_job(mode=dmsun.MODE_CAPTURE, max_trajectories=n, capture_target=0, max_scatterings=1000,
+                         rk_attempt_budget=RK_BUDGET, record=1, halo=dmsun.default_halo((0.0, 248.0, 0.0)))
+    print("capture workload: 1e4 trajectories", flush=True)
+    res = gpu_ctx.point(desc).run_job(job)
+    ores = oracle_lib.Point(oracle_solar, desc).run_job(job, threads=16, record_capacity=n)
+    assert_job_equal(res, ores)
+    assert len(res["records"]) == len(ores["records"]) == n
+    same = (rec_bytes(res["records"]) == rec_bytes(ores["records"])).all(axis=1)
+    assert same.all(), f"{(~same).sum()} of {n} records differ"
+    assert res["counters"].captured > 100

@@ -627,3 +627,20 @@ def test_bincount_check_catches_plausible_mistakes(oracle_solar):
     assert not check_bincount(1e3 * dt_row, 1e3 * v2_row, nbins, orbit, step)
     shifted = np.roll(dt_row, 40)
     assert not check_bincount(shifted, np.roll(v2_row, 40), nbins, orbit, step)
+
+
+def test_generate_data_round_longer_than_a_chunk(oracle_solar):
+    """A sync round longer than the oracle driver's 4096-trajectory chunks
+    (sigma_p = 1e-38 cm^2: the reference's interval is then far above 4096,
+    src/Data_Generation.cpp:59-74) is simulated in several chunks, in id order:
+    every trajectory once, the counters the records' sums."""
+    P = O.Point(oracle_solar, dmsun.make_point(0.1, 1e-38, interpolation_points=100))
+    n = 10000
+    job = dmsun.make_job(mode=dmsun.MODE_PARAMETER_POINT, max_trajectories=n, capture_target=0, max_scatterings=2000,
+                         n_bins=100, record=1, halo=dmsun.default_halo((0.0, 248.0, 0.0)))
+    res = P.run_job(job, threads=4, record_capacity=n)
+    c, recs = res["counters"], res["records"]
+    assert c.sync_interval > 4096 and c.trajectories == n and len(recs) == n
+    assert [r.trajectory_id for r in recs] == list(range(n))
+    assert c.captured == sum(1 for r in recs if r.flags & dmsun.F_CAPTURED)
+    assert c.total_scatterings == sum(r.n_scatterings for r in recs)
