@@ -57,6 +57,9 @@ struct dms_ctx
 	double *d_pref = nullptr, *d_imass = nullptr;
 	int* d_Z = nullptr;
 	DmsSolarView dev;
+	/* device address of this GPU's copy of the allocation-free entry point's
+	 * scratch array (hipGetSymbolAddress at set-up) */
+	unsigned char* scratch_ptr = nullptr;
 };
 
 struct dms_point
@@ -1045,6 +1048,63 @@ void run_batch_worker(std::shared_ptr<dms_batch> batch)
 
 } // namespace
 
+/* Scratch memory of dms_simulate_events_noalloc: the launch's slot counter and
+ * fill-control words, and its one-job table (first slot, first trajectory id).
+ * Each GPU holds its own copy for the life of the process; host code reaches
+ * it only through the address hipGetSymbolAddress returns for the current
+ * device (the array's name in host code is not a device address). */
+#define DMS_NOALLOC_SCRATCH_BYTES 256
+#define DMS_NOALLOC_MAX_TRAJECTORIES 0x7fffffffu
+static __device__ __align__(256) unsigned char scratch[DMS_NOALLOC_SCRATCH_BYTES];
+
+namespace
+{
+/* byte offsets inside scratch */
+constexpr size_t kScratchNext = 0;     /* unsigned int[4]: slot counter, fill control */
+constexpr size_t kScratchStart = 64;   /* uint32_t job_start[1] */
+constexpr size_t kScratchFirst = 128;  /* uint64_t job_first_id[1] */
+
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[256] = {};
+
+/* once per device, with that device current */
+hipError_t scratch_address(int device, unsigned char** out)
+{
+	std::lock_guard<std::mutex> lock(g_scratch_mu);
+	if(device < 0 || device >= 256)
+		return hipErrorInvalidDevice;
+	if(g_scratch_ptr[device] == nullptr)
+	{
+		void* p = nullptr;
+		const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+		if(e != hipSuccess)
+			return e;
+		g_scratch_ptr[device] = static_cast<unsigned char*>(p);
+	}
+	*out = g_scratch_ptr[device];
+	return hipSuccess;
+}
+} // namespace
+
+/* zero the slot counter and fill control, write the one-job table (a kernel,
+ * not hipMemsetAsync, so that the launch sequence can be captured in a graph) */
+__global__ void dms_noalloc_setup_kernel(unsigned char* scratch_dev, uint32_t n, uint64_t first_id)
+{
+	if(threadIdx.x < 4)
+		reinterpret_cast<unsigned int*>(scratch_dev + kScratchNext)[threadIdx.x] = (threadIdx.x == 2) ? n : 0u;
+	if(threadIdx.x == 0)
+	{
+		*reinterpret_cast<uint32_t*>(scratch_dev + kScratchStart) = 0u;
+		*reinterpret_cast<uint64_t*>(scratch_dev + kScratchFirst) = first_id;
+	}
+}
+
+__global__ void dms_zero_kernel(double* p, size_t n)
+{
+	for(size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t) gridDim.x * blockDim.x)
+		p[i] = 0.0;
+}
+
 /* ======================= C ABI ============================================= */
 
 extern "C" {
@@ -1191,6 +1251,8 @@ int dms_create(dms_ctx** out, int device, const double* raw_rows, size_t n_rows)
 		ctx->dev.iso_pref = ctx->d_pref;
 		ctx->dev.iso_mass = ctx->d_imass;
 		ctx->dev.iso_Z = ctx->d_Z;
+		if(scratch_address(device, &ctx->scratch_ptr) != hipSuccess)
+			return cleanup("hipGetSymbolAddress of the scratch array failed");
 	}
 	*out = ctx;
 	return DMS_OK;
@@ -1535,6 +1597,72 @@ int dms_simulate_events(dms_point* pt, const dms_job_desc* job, const dms_event*
 	hipStreamSynchronize(ctx->stream);
 	if(e != hipSuccess)
 		return fail(ctx, DMS_ERR_DEVICE, std::string("dms_simulate_events: ") + hipGetErrorString(e));
+	return DMS_OK;
+}
+
+int dms_simulate_events_noalloc(dms_point* pt, const dms_job_desc* job, const dms_event* d_initial, size_t n,
+                                dms_traj_record* d_records, double* d_hist_rows, void* stream)
+{
+	if(pt == nullptr || job == nullptr || (n > 0 && (d_initial == nullptr || d_records == nullptr)))
+		return fail(pt ? pt->ctx : nullptr, DMS_ERR_ARG, "dms_simulate_events_noalloc: null argument");
+	dms_ctx* ctx = pt->ctx;
+	const std::string jm = check_job(*job);
+	if(!jm.empty())
+		return fail(ctx, DMS_ERR_ARG, "dms_simulate_events_noalloc: " + jm);
+	if(n > DMS_NOALLOC_MAX_TRAJECTORIES)
+		return fail(ctx, DMS_ERR_ARG,
+		            "dms_simulate_events_noalloc: at most 2^31 - 1 trajectories per call (range of the scratch slot counter)");
+	if(ctx->scratch_ptr == nullptr)
+		return fail(ctx, DMS_ERR_DEVICE, "dms_simulate_events_noalloc: context has no GPU");
+	if(n == 0)
+		return DMS_OK;
+	std::lock_guard<std::mutex> lock(ctx->mu);
+	hipSetDevice(ctx->device); /* callers may be on any host thread */
+	const hipStream_t st = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+	const DmsJobView J = job_view(*job);
+	const bool rows = d_hist_rows != nullptr && !J.terminate_on_capture;
+	unsigned char* s = ctx->scratch_ptr;
+	(void) hipGetLastError();
+	hipLaunchKernelGGL(dms_noalloc_setup_kernel, dim3(1), dim3(64), 0, st, s, (uint32_t) n, J.first_id);
+	if(d_hist_rows)
+	{
+		/* per-trajectory rows start at zero (capture mode leaves them zero) */
+		const size_t cells = 2 * (size_t) J.nbins * n;
+		const size_t blocks = std::min<size_t>((cells + 255) / 256, (size_t) 65536);
+		hipLaunchKernelGGL(dms_zero_kernel, dim3((unsigned) blocks), dim3(256), 0, st, d_hist_rows, cells);
+	}
+	DmsTrajArgs a;
+	a.S = ctx->dev;
+	a.P = pt->dev;
+	a.J = J;
+	a.ics = d_initial;
+	a.shift_ok = nullptr; /* every event is a valid start */
+	a.n = (uint32_t) n;
+	a.records = d_records;
+	a.summ = nullptr;
+	a.R.rows = rows ? d_hist_rows : nullptr;
+	a.R.counts = nullptr;
+	a.R.touched = nullptr;
+	a.R.per_lane = 0;
+	a.R.hist = nullptr;
+	a.R.hcounts = nullptr;
+	a.next = reinterpret_cast<unsigned int*>(s + kScratchNext);
+	a.n_jobs = 1;
+	a.job_start = reinterpret_cast<const uint32_t*>(s + kScratchStart);
+	a.job_first_id = reinterpret_cast<const uint64_t*>(s + kScratchFirst);
+	a.job_done = nullptr;
+	a.job_points = nullptr;
+	a.job_keys = nullptr;
+	a.fill_ctl = nullptr;
+	a.cap_stop = 0;
+	a.fill_extra = 0;
+	hipLaunchKernelGGL(J.mixed ? dms_traj_kernel_mixed : dms_traj_kernel, dim3(traj_grid(ctx, n)), dim3(DMS_TRAJ_BLOCK), ctx->traj_lds,
+	                   st, a);
+	hipError_t e = hipGetLastError();
+	if(e == hipSuccess && stream == nullptr)
+		e = hipStreamSynchronize(st);
+	if(e != hipSuccess)
+		return fail(ctx, DMS_ERR_DEVICE, std::string("dms_simulate_events_noalloc: ") + hipGetErrorString(e));
 	return DMS_OK;
 }
 

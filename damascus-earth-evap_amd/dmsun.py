@@ -205,6 +205,7 @@ def _declare(lib):
                                       C.c_size_t]),
         "dms_initial_conditions": (C.c_int, [vp, _P(Halo), C.c_uint32, C.c_uint32, C.c_uint64, C.c_size_t,
                                              C.c_double, _P(Event), _P(C.c_int32)]),
+        "dms_simulate_events_noalloc": (C.c_int, [vp, _P(JobDesc), vp, C.c_size_t, vp, vp, vp]),
         "dms_simulate_events": (C.c_int, [vp, _P(JobDesc), _P(Event), C.c_size_t, _P(TrajRecord),
                                           _P(C.c_double)]),
         "dms_job_submit": (C.c_int, [vp, _P(JobDesc), _P(vp)]),
@@ -232,7 +233,7 @@ def _declare(lib):
 EXPORTED = ["dms_abi_version", "dms_create", "dms_destroy", "dms_last_error", "dms_set_job_memory_budget", "dms_load_solar_file",
             "dms_solar_eval", "dms_isotopes", "dms_constant", "dms_point_create", "dms_point_destroy",
             "dms_point_rate_grid", "dms_point_grid_shape", "dms_point_cross_sections", "dms_point_rates", "dms_initial_conditions",
-            "dms_simulate_events", "dms_job_submit", "dms_job_submit_batch", "dms_job_submit_multi", "dms_multi_autotune", "dms_job_poll", "dms_job_wait", "dms_job_read",
+            "dms_simulate_events", "dms_simulate_events_noalloc", "dms_job_submit", "dms_job_submit_batch", "dms_job_submit_multi", "dms_multi_autotune", "dms_job_poll", "dms_job_wait", "dms_job_read",
             "dms_job_timing", "dms_job_destroy", "dms_probe_math"]
 
 _lib = None
@@ -393,6 +394,15 @@ class Point:
         out = np.empty_like(r)
         self.ctx.check(lib().dms_point_rates(self.h, which, dptr(r), dptr(v), dptr(out), r.size), "dms_point_rates")
         return out
+
+    def simulate_events_noalloc(self, job, d_events, n, d_records, d_rows=None, stream=None):
+        """dms_simulate_events_noalloc on device buffers (integer device
+        addresses): nothing is allocated; stream None runs synchronously on the
+        context's stream, otherwise the kernels are only enqueued on it."""
+        self.ctx.check(lib().dms_simulate_events_noalloc(self.h, C.byref(job), C.c_void_p(d_events), n,
+                                                          C.c_void_p(d_records), C.c_void_p(d_rows) if d_rows else None,
+                                                          C.c_void_p(stream) if stream else None),
+                       "dms_simulate_events_noalloc")
 
     def simulate_events(self, job, events, n_bins=None, want_rows=False):
         n = len(events)

@@ -296,6 +296,23 @@ int dms_initial_conditions(dms_ctx* ctx, const dms_halo* halo, uint32_t seed, ui
 int dms_simulate_events(dms_point* point, const dms_job_desc* job, const dms_event* initial, size_t n,
                         dms_traj_record* records, double* hist_rows);
 
+/* dms_simulate_events for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  d_initial (n events),
+ * d_records (n records) and d_hist_rows (optional, n x 2 x n_bins doubles,
+ * cleared by the call) are DEVICE pointers on the point's GPU; the kernels'
+ * other memory -- the slot counter and the one-job table -- lives in a
+ * 256-byte array of the library that every call on that GPU shares, so calls
+ * that overlap on one GPU race and must be ordered by the caller (one stream,
+ * or synchronisation).  The call allocates nothing: with stream != NULL (a
+ * hipStream_t) it only enqueues its kernels there; with NULL it runs on the
+ * context's stream and returns when they have finished.  The events must pass
+ * Simulate's argument check (finite, t >= 0, r > 0, v > 0); unlike
+ * dms_simulate_events the host does not read them.  At most 2^31 - 1
+ * trajectories per call. */
+int dms_simulate_events_noalloc(dms_point* point, const dms_job_desc* job, const dms_event* d_initial, size_t n,
+                                dms_traj_record* d_records, double* d_hist_rows, void* stream);
+
 /* ---- jobs (Simulation_Data::Generate_Data, src/Data_Generation.cpp:488-972) -- */
 
 int dms_job_submit(dms_point* point, const dms_job_desc* desc, dms_job** job);
