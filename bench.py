@@ -55,10 +55,14 @@ FLOP_WEIGHTS = {
     "trajectory": 150.0,        # trajectory start/finish bookkeeping
 }
 PEAK_FP64_TFLOPS = 78.6
-# HBM bytes of the timed launch of the default workload (4 jobs of 10^6 in one
-# launch): rocprofv3 FETCH_SIZE and WRITE_SIZE passes, scaled by a plain 256 MiB
-# copy (read x2.0, write x1.0); profiles/dms_traj_kernel_parameter_point_m0.3_4x1e6.md.
-PROFILED_TRAFFIC_BYTES = {(4, 1_000_000): 275.5e9 + 158.2e9}
+# The default timed launch (20 jobs of 10^6 trajectories, the bench's default
+# --steps) profiled with rocprofv3: HBM bytes from the FETCH_SIZE and WRITE_SIZE
+# passes scaled by a plain 256 MiB copy (read x2.0, write x1.0), and the FP64
+# wave instructions weighted by operations (ADD + MUL + TRANS + 2 FMA); see
+# profiles/dms_traj_kernel_parameter_point_m0.3_20x1e6.md.  Other step counts
+# are other launches: their traffic is reported as null (unmeasured).
+PROFILED_LAUNCH = {"steps": 20, "trajectories": 1_000_000, "hbm_bytes": 1413.330804608e9 + 800.797045056e9,
+                   "f64_op_weighted_wave_instructions": 4.148e11}
 
 
 def algorithmic_flops(c, capture_mode):
@@ -289,7 +293,7 @@ def validate_mixed(dmsun, ctx, n):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=4)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--trajectories", type=int, default=1_000_000)
     ap.add_argument("--cpu-sample", type=int, default=100_000)
@@ -336,6 +340,7 @@ def main():
     wall, flops, kern_ms, launches, single = fp64["wall"], fp64["flops"], fp64["kern_ms"], fp64["launches"], fp64["single"]
 
     avg_launch_s = (kern_ms / launches) * 1e-3
+    profiled = (args.steps, args.trajectories) == (PROFILED_LAUNCH["steps"], PROFILED_LAUNCH["trajectories"])
     achieved = flops / launches / avg_launch_s / 1e12
     out = {
         "metric": "trajectories/s (FP64 GPU kernel; mixed-precision kernel and scatterings/s alongside)",
@@ -360,8 +365,12 @@ def main():
         "kernel_ms_per_step": kern_ms / args.steps,
         "roofline": {"bound": "valu-fp64", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP64_TFLOPS,
-                     "traffic": PROFILED_TRAFFIC_BYTES.get((args.steps, args.trajectories)),
-                     "traffic_unit": "bytes per launch (rocprofv3 counters of this workload, not measured live)",
+                     "traffic": PROFILED_LAUNCH["hbm_bytes"] if profiled else None,
+                     "traffic_unit": "HBM bytes per launch (rocprofv3 FETCH_SIZE/WRITE_SIZE of this launch, not live)",
+                     "counter_bound_tflops": (64 * PROFILED_LAUNCH["f64_op_weighted_wave_instructions"] / avg_launch_s / 1e12
+                                              if profiled else None),
+                     "counter_bound_note": "upper bound from the profiled FP64 VALU wave-instruction counters x 64 lanes "
+                                           "(counters do not see inactive lanes)",
                      "kernel": "dms_traj_kernel", "avg_launch_ms": avg_launch_s * 1e3},
     }
     if mixed is not None:
