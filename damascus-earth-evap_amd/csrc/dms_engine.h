@@ -1992,8 +1992,10 @@ DMS_HD void dms_lane_finish_scatter(DmsLane& L, const DmsLaneCtx& c)
 }
 
 /* Trajectory end up to the per-lane row flush.  Returns the class of the
- * scratch row still to be flushed (dms_flush_rows), or -1 when there is none. */
-DMS_HD int dms_lane_finish_rows_pending(DmsLane& L, const DmsLaneCtx& c, const dms_event& ic, dms_traj_record* rec, DmsSummary* sum)
+ * scratch row still to be flushed (dms_flush_rows), or -1 when there is none;
+ * the trajectory's flags and termination reason go to flags_out/reason_out. */
+DMS_HD int dms_lane_finish_rows_pending(DmsLane& L, const DmsLaneCtx& c, const dms_event& ic, dms_traj_record* rec, DmsSummary* sum,
+                                        uint32_t* flags_out = nullptr, int* reason_out = nullptr)
 {
 	dms_bin_flush(L, c.R, c.J.nbins);
 	uint32_t flags = 0;
@@ -2006,6 +2008,10 @@ DMS_HD int dms_lane_finish_rows_pending(DmsLane& L, const DmsLaneCtx& c, const d
 		reason = s->reason;
 	}
 	L.phase = PH_NEED;
+	if(flags_out)
+		*flags_out = flags;
+	if(reason_out)
+		*reason_out = reason;
 	if(c.R.per_lane && L.n_touched > 0)
 		return (flags & DMS_F_CAPTURED) ? 1 : ((reason == DMS_OUTWARD_ESCAPE) ? 2 : 0);
 	return -1;

@@ -33,6 +33,24 @@ __device__ inline float readlane_f(float x, int l) { return __int_as_float(__bui
 
 } // namespace
 
+/* Live progress of a job (the heartbeat's per-rank counts): finished
+ * trajectories, captured ones, completed outward escapes of uncaptured ones
+ * and numerical failures, as the run accounting classifies them
+ * (src/Data_Generation.cpp:764-850). */
+__device__ inline void dms_count_finished(unsigned int* w, uint32_t flags, int reason, int capture_mode)
+{
+	atomicAdd(w, 1u);
+	if(flags & DMS_F_CAPTURED)
+		atomicAdd(w + 1, 1u);
+	else if(reason == DMS_OUTWARD_ESCAPE)
+		atomicAdd(w + 2, 1u);
+	const bool numerical = reason == DMS_NUMERICAL_FAILURE || reason == DMS_NON_FINITE_STATE || reason == DMS_SPEED_LIMIT
+	                       || reason == DMS_ENERGY_DRIFT_ESCAPE || reason == DMS_UNKNOWN
+	                       || (!capture_mode && !(flags & DMS_F_CAPTURED) && (flags & DMS_F_REFLECTION_SHIFT_FAILED));
+	if(numerical)
+		atomicAdd(w + 3, 1u);
+}
+
 template <bool MX, bool MP>
 __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 {
@@ -182,10 +200,12 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 			int flush_cls = -1;
 			if(__builtin_expect(L.phase == PH_FIN, 0))
 			{
+				uint32_t fl = 0;
+				int reason = 0;
 				flush_cls = dms_lane_finish_rows_pending(L, c, a.ics[L.slot], a.records ? a.records + L.slot : nullptr,
-				                                         a.summ ? a.summ + L.slot : nullptr);
+				                                         a.summ ? a.summ + L.slot : nullptr, &fl, &reason);
 				if(a.job_done)
-					atomicAdd(a.job_done + L.job, 1u);
+					dms_count_finished(a.job_done + DMS_PROGRESS_WORDS * L.job, fl, reason, c.J.terminate_on_capture);
 			}
 			/* per-lane histogram rows of the finished trajectories: the wave adds
 			 * each row's touched bins to its job's sums, 64 bins at a time */
@@ -259,7 +279,8 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 				dms_lane_start(L, c, a.ics[L.slot], a.shift_ok == nullptr || a.shift_ok[L.slot] != 0,
 				               a.records ? a.records + L.slot : nullptr, a.summ ? a.summ + L.slot : nullptr);
 				if(L.phase == PH_NEED && a.job_done) /* initial shift failed: recorded, done */
-					atomicAdd(a.job_done + L.job, 1u);
+					dms_count_finished(a.job_done + DMS_PROGRESS_WORDS * L.job, DMS_F_SHIFT_FAILED, DMS_NUMERICAL_FAILURE,
+					                   c.J.terminate_on_capture);
 			}
 		}
 		dms_lane_advance_b<MX>(L, c);

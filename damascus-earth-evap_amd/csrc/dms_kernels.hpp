@@ -13,6 +13,8 @@
 __host__ __device__ inline size_t dms_lds_mass_offset(int n) { return ((size_t) n * sizeof(double) + 31) & ~(size_t) 31; }
 __host__ __device__ inline size_t dms_lds_bytes(int n) { return dms_lds_mass_offset(n) + 2 * (size_t) (n - 1) * sizeof(DmsCubic); }
 
+#define DMS_PROGRESS_WORDS 4
+
 struct DmsTrajArgs
 {
 	DmsSolarView S;
@@ -30,8 +32,9 @@ struct DmsTrajArgs
 	uint32_t n_jobs;
 	const uint32_t* job_start;
 	const uint64_t* job_first_id;
-	/* finished trajectories per job (progress, read by the host while the
-	 * launch runs); may be null */
+	/* live progress per job, DMS_PROGRESS_WORDS counters each (finished,
+	 * captured, completed escapes, numerical failures), read by the host while
+	 * the launch runs; may be null */
 	unsigned int* job_done;
 	/* jobs of different parameter points (multi-point launch): per-job point
 	 * views and Philox keys {seed, point_id}; null for one point */

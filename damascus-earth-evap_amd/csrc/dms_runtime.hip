@@ -91,7 +91,7 @@ struct dms_job
 	std::shared_ptr<dms_batch> batch;
 	bool needs_rerun = false;
 	double batch_size = 1.0;
-	std::atomic<uint64_t> done_traj{0}, done_captured{0}, done_scat{0};
+	std::atomic<uint64_t> done_traj{0}, done_captured{0}, done_scat{0}, done_escaped{0}, done_failed{0};
 	std::atomic<int> finished{0};
 	int status = DMS_OK;
 	std::string err;
@@ -318,9 +318,9 @@ struct LaunchJobs
 	{
 		hipError_t e;
 		if((e = dalloc(&d_start, start.size(), st)) != hipSuccess || (e = dalloc(&d_first, first_id.size(), st)) != hipSuccess
-		   || (e = dalloc(&d_done, start.size(), st)) != hipSuccess)
+		   || (e = dalloc(&d_done, DMS_PROGRESS_WORDS * start.size(), st)) != hipSuccess)
 			return e;
-		if((e = hipMemsetAsync(d_done, 0, sizeof(unsigned int) * start.size(), st)) != hipSuccess)
+		if((e = hipMemsetAsync(d_done, 0, sizeof(unsigned int) * DMS_PROGRESS_WORDS * start.size(), st)) != hipSuccess)
 			return e;
 		if(!points.empty())
 		{
@@ -426,7 +426,7 @@ hipError_t run_trajectories(dms_ctx* ctx, hipStream_t st, hipStream_t pst, const
 	if(e == hipSuccess && progress && LJ.d_done)
 	{
 		/* heartbeat: read the per-job finished counters on a second stream */
-		std::vector<unsigned int> done(LJ.start.size(), 0u);
+		std::vector<unsigned int> done(DMS_PROGRESS_WORDS * LJ.start.size(), 0u);
 		while(hipEventQuery(e1) == hipErrorNotReady)
 		{
 			if(hipMemcpyAsync(done.data(), LJ.d_done, sizeof(unsigned int) * done.size(), hipMemcpyDeviceToHost, pst) == hipSuccess
@@ -741,7 +741,13 @@ void run_job_chunked(dms_job* job, hipStream_t st, hipStream_t pst)
 			fs.fill_extra = (uint32_t) (fx < (uint64_t) n ? fx : (uint64_t) n);
 		}
 		hipError_t e = run_trajectories_single(ctx, st, pst, pt, J, B, n, want_rows, want_recs, next_id, &kms,
-		                                       [job, base_done](const std::vector<unsigned int>& d) { job->done_traj = base_done + d[0]; },
+		                                       [job, base_done, base_cap = C.captured, base_esc = C.completed_outward_escapes,
+		                                        base_fail = C.numerical_failures](const std::vector<unsigned int>& d) {
+			                                       job->done_traj = base_done + d[0];
+			                                       job->done_captured = base_cap + d[1];
+			                                       job->done_escaped = base_esc + d[2];
+			                                       job->done_failed = base_fail + d[3];
+		                                       },
 		                                       fill_stop ? &fs : nullptr);
 		if(e != hipSuccess)
 		{
@@ -815,6 +821,8 @@ void run_job_chunked(dms_job* job, hipStream_t st, hipStream_t pst)
 		job->done_traj = C.trajectories;
 		job->done_captured = C.captured;
 		job->done_scat = C.total_scatterings;
+		job->done_escaped = C.completed_outward_escapes;
+		job->done_failed = C.numerical_failures;
 	}
 	R.finish();
 	C.simulated_trajectories = simulated;
@@ -913,8 +921,13 @@ void run_batch_fixed(std::vector<dms_job*>& jobs, hipStream_t st, hipStream_t ps
 		}
 		hipError_t e = run_trajectories(ctx, st, pst, pt, J, B, n, want_rows, want_recs, LJ, want_rows ? &LR : nullptr, &kms,
 		                                [&jobs](const std::vector<unsigned int>& d) {
-			                                for(size_t k = 0; k < jobs.size() && k < d.size(); k++)
-				                                jobs[k]->done_traj = d[k];
+			                                for(size_t k = 0; k < jobs.size() && DMS_PROGRESS_WORDS * k + 3 < d.size(); k++)
+			                                {
+				                                jobs[k]->done_traj = d[DMS_PROGRESS_WORDS * k];
+				                                jobs[k]->done_captured = d[DMS_PROGRESS_WORDS * k + 1];
+				                                jobs[k]->done_escaped = d[DMS_PROGRESS_WORDS * k + 2];
+				                                jobs[k]->done_failed = d[DMS_PROGRESS_WORDS * k + 3];
+			                                }
 		                                });
 		if(e != hipSuccess)
 		{
@@ -967,6 +980,8 @@ void run_batch_fixed(std::vector<dms_job*>& jobs, hipStream_t st, hipStream_t ps
 			job->done_traj = job->C.trajectories;
 			job->done_captured = job->C.captured;
 			job->done_scat = job->C.total_scatterings;
+			job->done_escaped = job->C.completed_outward_escapes;
+			job->done_failed = job->C.numerical_failures;
 			job->needs_rerun = want_rows && counted < nk;
 		}
 		const double total = (now_ms() - t_start) * 1e-3;
@@ -1843,6 +1858,8 @@ int dms_job_poll(dms_job* job, dms_job_status* st)
 	st->trajectories_done = job->done_traj.load();
 	st->captured = job->done_captured.load();
 	st->scatterings = job->done_scat.load();
+	st->completed_outward_escapes = job->done_escaped.load();
+	st->numerical_failures = job->done_failed.load();
 	st->done = job->finished.load();
 	st->pad = 0;
 	return DMS_OK;

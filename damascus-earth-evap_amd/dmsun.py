@@ -102,7 +102,8 @@ class EvapEvent(C.Structure):
 
 class JobStatus(C.Structure):
     _fields_ = [("trajectories_done", C.c_uint64), ("captured", C.c_uint64), ("scatterings", C.c_uint64),
-                ("done", C.c_int32), ("pad", C.c_int32)]
+                ("done", C.c_int32), ("pad", C.c_int32), ("completed_outward_escapes", C.c_uint64),
+                ("numerical_failures", C.c_uint64)]
 
 
 def record_array(n):

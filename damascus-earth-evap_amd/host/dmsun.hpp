@@ -244,9 +244,11 @@ class Simulation_Data
 	               unsigned long int max_free_steps = 1000000000000ul);
 	void Set_Bins(int n_bins) { number_of_bins = n_bins; }
 	void Set_RK_Attempt_Budget(uint64_t budget) { rk_attempt_budget = budget; }
-	/* Progress snapshots (heartbeat analogue, include/Snapshot_IO.hpp:27-57):
-	 * every interval seconds while Generate_Data runs, <dir>/snapshot_progress.txt
-	 * is rewritten with the job's live counters. */
+	/* Progress snapshots (heartbeat analogue, include/Snapshot_IO.hpp:27-57,
+	 * src/Snapshot_IO.cpp:508-646): every interval seconds while Generate_Data
+	 * runs, <dir>/snapshot_<t>s.txt with the job's live device counters; at the
+	 * first checkpoint after the job finished, the report with its final
+	 * counters and histograms. */
 	void Set_Snapshots(const std::string& dir, double interval_seconds)
 	{
 		snapshot_dir = dir;
@@ -385,8 +387,25 @@ std::map<std::string, std::vector<std::string>> Parse_Config_Text(const std::str
 /* Output files of Write_Output_Files (src/Data_Generation.cpp:253-348,1212-1300). */
 void Write_Bincount_File(const std::string& path, const Simulation_Data& data, double mass_gev, double sigma_cm2);
 void Write_Evaporation_Times_File(const std::string& path, const Simulation_Data& data, double mass_gev, double sigma_cm2);
-/* One progress snapshot (key = value lines), written through a temporary file. */
-void Write_Progress_Snapshot(const std::string& path, const dms_job_status& status, double elapsed_sec, uint64_t captured_target);
+/* One snapshot report (src/Snapshot_IO.cpp:508-646): the reference's
+ * snapshot_<t>s.txt header, its device-status section (one "rank": the GPU
+ * job) and, when hist is given (8 x n_bins in bincount.txt column order), the
+ * histogram section; written through a temporary file. */
+struct Snapshot_Report
+{
+	long long time_label_s = 0;
+	double interval_s = 0.0;
+	uint64_t run_id = 0;
+	double mass_GeV = 0.0, sigma_cm2 = 0.0;
+	uint64_t total_trajectories = 0, captured_particles = 0, valid_trajectories = 0, numerical_failures = 0;
+	uint64_t scatterings = 0;
+	bool done = false;
+	double elapsed_s = 0.0;
+	const std::vector<double>* hist = nullptr;
+	int n_bins = 0;
+	double bin_width_km = 0.0;
+};
+void Write_Snapshot_Report(const std::string& path, const Snapshot_Report& report);
 
 } // namespace dmsun
 

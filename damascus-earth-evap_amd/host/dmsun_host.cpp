@@ -302,23 +302,37 @@ void Simulation_Data::Generate_Data(const DM_Particle& DM, Solar_Model& solar_mo
 	dms_job* job = nullptr;
 	check(dms_job_submit(p, &j, &job), "Simulation_Data::Generate_Data", solar_model.Context());
 	dms_job_status st;
-	if(snapshot_interval > 0.0 && !snapshot_dir.empty() && Ensure_Directory(snapshot_dir))
+	const bool snapshots = snapshot_interval > 0.0 && !snapshot_dir.empty() && Ensure_Directory(snapshot_dir);
+	const std::string snap_root = snapshot_dir.empty() ? "" : (snapshot_dir.back() == '/' ? snapshot_dir : snapshot_dir + "/");
+	const auto t0 = std::chrono::steady_clock::now();
+	auto elapsed = [&]() { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
+	Snapshot_Report rep;
+	rep.interval_s = snapshot_interval;
+	rep.run_id = (uint64_t) std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::system_clock::now().time_since_epoch()).count();
+	rep.mass_GeV = DM.mass / units::GeV();
+	rep.sigma_cm2 = (DM.interaction == DMS_DARK_PHOTON ? DM.sigma_electron : DM.sigma_proton) / (units::cm() * units::cm());
+	rep.n_bins = number_of_bins;
+	auto snapshot_path = [&](long long label) { return snap_root + "snapshot_" + std::to_string(label) + "s.txt"; };
+	int snapshot_index = 0;
+	if(snapshots)
 	{
-		const std::string path = (snapshot_dir.back() == '/' ? snapshot_dir : snapshot_dir + "/") + "snapshot_progress.txt";
-		const auto t0 = std::chrono::steady_clock::now();
-		auto elapsed = [&]() { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
-		double next = snapshot_interval;
+		/* SnapshotTimeLabelSeconds: checkpoint k at k x interval seconds of wall time */
 		while(dms_job_poll(job, &st) == DMS_OK && !st.done)
 		{
-			if(elapsed() >= next)
+			if(elapsed() >= (snapshot_index + 1) * snapshot_interval)
 			{
-				Write_Progress_Snapshot(path, st, elapsed(), requested_captured_particles);
-				next += snapshot_interval;
+				snapshot_index++;
+				rep.time_label_s = (long long) (snapshot_index * snapshot_interval);
+				rep.total_trajectories = st.trajectories_done;
+				rep.captured_particles = st.captured;
+				rep.valid_trajectories = st.captured + st.completed_outward_escapes;
+				rep.numerical_failures = st.numerical_failures;
+				rep.scatterings = st.scatterings;
+				rep.elapsed_s = elapsed();
+				Write_Snapshot_Report(snapshot_path(rep.time_label_s), rep);
 			}
 			std::this_thread::sleep_for(std::chrono::milliseconds(50));
 		}
-		dms_job_poll(job, &st);
-		Write_Progress_Snapshot(path, st, elapsed(), requested_captured_particles);
 	}
 	const int ws = dms_job_wait(job, &st);
 	if(ws != DMS_OK)
@@ -339,6 +353,21 @@ void Simulation_Data::Generate_Data(const DM_Particle& DM, Solar_Model& solar_mo
 	check(rs, "Simulation_Data::Generate_Data: dms_job_read", solar_model.Context());
 	ev.resize(n_ev);
 	Set_Results(cnt, h, c, ev);
+	if(snapshots)
+	{
+		/* the first checkpoint after the job finished carries its final results */
+		snapshot_index = std::max(snapshot_index + 1, (int) std::ceil(elapsed() / snapshot_interval));
+		rep.time_label_s = (long long) (snapshot_index * snapshot_interval);
+		rep.total_trajectories = cnt.trajectories;
+		rep.captured_particles = cnt.captured;
+		rep.valid_trajectories = cnt.captured + cnt.completed_outward_escapes;
+		rep.numerical_failures = cnt.numerical_failures;
+		rep.scatterings = cnt.total_scatterings;
+		rep.done = true;
+		rep.elapsed_s = elapsed();
+		rep.hist = capture_mode ? nullptr : &h;
+		Write_Snapshot_Report(snapshot_path(rep.time_label_s), rep);
+	}
 }
 
 } // namespace dmsun

@@ -319,20 +319,64 @@ void Write_Evaporation_Times_File(const std::string& path, const Simulation_Data
 	}
 }
 
-void Write_Progress_Snapshot(const std::string& path, const dms_job_status& st, double elapsed_sec, uint64_t captured_target)
+void Write_Snapshot_Report(const std::string& path, const Snapshot_Report& r)
 {
 	const std::string tmp = path + ".tmp." + std::to_string(getpid());
 	{
 		std::ofstream f(tmp, std::ios::out | std::ios::trunc);
 		if(!f.is_open())
 			return;
-		f << "# DaMaSCUS-SUN progress snapshot\n";
-		f << "trajectories_done = " << st.trajectories_done << "\n";
-		f << "captured_particles = " << st.captured << "\n";
-		f << "captured_target = " << captured_target << "\n";
-		f << "total_scatterings = " << st.scatterings << "\n";
-		f << "elapsed_seconds = " << std::fixed << std::setprecision(3) << elapsed_sec << "\n";
-		f << "finished = " << (st.done ? 1 : 0) << "\n";
+		f << "# snapshot_status = merged\n";
+		f << "# snapshot_run_id = " << r.run_id << "\n";
+		f << "# Cumulative snapshot report\n";
+		f << "# ready_ranks = 0\n";
+		f << "# missing_ranks = \n";
+		f << "# snapshot_target_wall_time_s = " << r.time_label_s << "\n";
+		f << "# snapshot_interval_s = " << std::fixed << std::setprecision(3) << r.interval_s << "\n";
+		f << "# DM_mass_GeV = " << std::scientific << std::setprecision(6) << r.mass_GeV << "\n";
+		f << "# DM_sigma_cm2 = " << std::scientific << std::setprecision(6) << r.sigma_cm2 << "\n";
+		f << "# total_trajectories = " << r.total_trajectories << "\n";
+		f << "# captured_particles = " << r.captured_particles << "\n";
+		f << "# valid_trajectories = " << r.valid_trajectories << "\n";
+		f << "# numerical_failures = " << r.numerical_failures << "\n";
+		f << "# unresolved_not_captured_trajectories = "
+		  << (r.total_trajectories > r.valid_trajectories ? r.total_trajectories - r.valid_trajectories : 0) << "\n";
+		const BinomialRateEstimate raw = Estimate_Binomial_Rate(r.total_trajectories, r.captured_particles);
+		const BinomialRateEstimate valid = Estimate_Binomial_Rate(r.valid_trajectories, r.captured_particles);
+		f << std::fixed << std::setprecision(8);
+		f << "# capture_rate = " << raw.rate << "\n";
+		f << "# capture_rate_raw = " << raw.rate << "\n";
+		f << "# capture_rate_err = " << raw.standard_error << "\n";
+		f << "# capture_rate_raw_err = " << raw.standard_error << "\n";
+		f << "# capture_rate_CI_95_lower = " << raw.ci_lower << "\n";
+		f << "# capture_rate_CI_95_upper = " << raw.ci_upper << "\n";
+		f << "# capture_rate_raw_CI_95_lower = " << raw.ci_lower << "\n";
+		f << "# capture_rate_raw_CI_95_upper = " << raw.ci_upper << "\n";
+		f << "# capture_rate_valid = " << valid.rate << "\n";
+		f << "# capture_rate_valid_err = " << valid.standard_error << "\n";
+		f << "# capture_rate_valid_CI_95_lower = " << valid.ci_lower << "\n";
+		f << "# capture_rate_valid_CI_95_upper = " << valid.ci_upper << "\n";
+		/* the device runs the job's trajectories in one launch: one status line */
+		f << "#\n# [MPI rank status]\n";
+		f << "# rank  state  trajectory_id  trajectory_wall_s  simulated_elapsed_s  scatterings  observed_at_wall_s\n";
+		f << "# 0\t" << (r.done ? "done" : "running_uncaptured") << "\t" << r.total_trajectories << "\t" << std::scientific
+		  << std::setprecision(10) << r.elapsed_s << "\t" << 0.0 << "\t" << r.scatterings << "\t" << r.elapsed_s << "\n";
+		if(r.hist && r.n_bins > 0 && r.hist->size() >= 8 * (size_t) r.n_bins)
+		{
+			const std::vector<double>& h = *r.hist;
+			const size_t nb = (size_t) r.n_bins;
+			auto H = [&](int q, size_t b) { return h[(size_t) q * nb + b]; };
+			const double N_cap = (double) r.captured_particles;
+			const double N_nc = (double) (r.valid_trajectories - std::min(r.valid_trajectories, r.captured_particles));
+			f << "#\n# [Bincount histogram]\n";
+			f << "# bin_index  cap_dt[s]  cap_v2dt[km2/s]  cap_err_dt[s]  cap_err_v2dt[km2/s]  not_cap_dt[s]  not_cap_v2dt[km2/s]  "
+			     "not_cap_err_dt[s]  not_cap_err_v2dt[km2/s]\n";
+			for(size_t b = 0; b < nb; b++)
+				f << b << "\t" << std::scientific << std::setprecision(10) << H(0, b) << "\t" << H(1, b) << "\t"
+				  << Snapshot_Bin_Error(H(0, b), H(2, b), N_cap) << "\t" << Snapshot_Bin_Error(H(1, b), H(3, b), N_cap) << "\t"
+				  << H(4, b) << "\t" << H(5, b) << "\t" << Snapshot_Bin_Error(H(4, b), H(6, b), N_nc) << "\t"
+				  << Snapshot_Bin_Error(H(5, b), H(7, b), N_nc) << "\n";
+		}
 	}
 	std::rename(tmp.c_str(), path.c_str());
 }

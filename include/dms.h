@@ -220,6 +220,9 @@ typedef struct
 	double lifetime_unbinding_sec;
 } dms_evap_event;
 
+/* Live progress of a job (the heartbeat's view, include/Snapshot_IO.hpp:27-57):
+ * while its launch runs, counts of the trajectories the device has finished;
+ * once done, the job's counters. */
 typedef struct
 {
 	uint64_t trajectories_done;
@@ -227,6 +230,8 @@ typedef struct
 	uint64_t scatterings;
 	int32_t done;
 	int32_t pad;
+	uint64_t completed_outward_escapes; /* uncaptured trajectories that left (capture-classified with the captured) */
+	uint64_t numerical_failures;
 } dms_job_status;
 
 typedef struct dms_ctx dms_ctx;

@@ -118,29 +118,82 @@ def test_progress_is_live_and_monotone(gpu_ctx):
     assert len(set(v for v in seen if 0 < v < 60000)) >= 2  # intermediate progress was visible
 
 
-def test_driver_writes_progress_snapshots(tmp_path):
+def snapshot_reports(directory):
+    """snapshot_<t>s.txt reports in checkpoint order: (label, header keys, histogram rows)."""
+    out = []
+    for f in directory.glob("snapshot_*s.txt"):
+        label = int(f.name[len("snapshot_"):-len("s.txt")])
+        lines = f.read_text().splitlines()
+        kv = {l[2:].split(" = ")[0]: l.split(" = ", 1)[1] for l in lines if l.startswith("# ") and " = " in l}
+        rows = [list(map(float, l.split("\t"))) for l in lines if l and not l.startswith("#")]
+        out.append((label, kv, np.array(rows)))
+    return sorted(out, key=lambda t: t[0])
+
+
+def wilson(n, k):
+    if n == 0:
+        return 0.0, 0.0, 0.0, 0.0
+    p = min(k, n) / n
+    z = 1.96
+    den = 1 + z * z / n
+    c = p + z * z / (2 * n)
+    h = z * np.sqrt(p * (1 - p) / n + z * z / (4 * n * n))
+    return p, np.sqrt(p * (1 - p) / n), max(0.0, (c - h) / den), min(1.0, (c + h) / den)
+
+
+def run_with_snapshots(tmp_path, sample, nb):
     cfg = tmp_path / "snap.cfg"
-    cfg.write_text(CFG.format(mode="snap", run_mode="Parameter point", sample=30, maxt=800, out=tmp_path)
+    cfg.write_text(CFG.format(mode="snap", run_mode="Parameter point", sample=sample, maxt=0, out=tmp_path)
                    + "snapshot_enabled = true;\nsnapshot_interval = 1;\n")
-    p = subprocess.run([str(RUN), str(cfg), "--bins", "100"], cwd=str(REPO), capture_output=True, text=True, timeout=120)
+    p = subprocess.run([str(RUN), str(cfg), "--bins", str(nb)], cwd=str(REPO), capture_output=True, text=True, timeout=200)
     assert p.returncode == 0, p.stderr
-    snap = (tmp_path / "snapshots" / "snapshot_progress.txt").read_text()
-    kv = dict(l.split(" = ") for l in snap.splitlines() if " = " in l)
-    assert kv["finished"] == "1" and int(kv["trajectories_done"]) > 0
+    return snapshot_reports(tmp_path / "snapshots")
 
 
-def test_solar_model_non_square_grid():
-    """tests/test_Solar_Model.cpp:192-224 through the C++ Solar_Model: a 31 x 29
-    grid is exact at its nodes and reports both counts; (1, 29) turns the grid
-    off (0 x 0) and the total rate is the directly computed one."""
-    exe = REPO / "tests" / "harness" / "host_unit"
-    pkg = REPO / "damascus-earth-evap_amd"
-    if not exe.exists():
-        subprocess.run(["g++", "-O1", "-std=c++17", "-o", str(exe), str(REPO / "tests" / "harness" / "host_unit.cpp"),
-                        "-L", str(pkg), "-ldmsun_host", "-ldmsun", f"-Wl,-rpath,{pkg}"], check=True)
-    p = subprocess.run([str(exe), "grid"], capture_output=True, text=True, timeout=120)
-    assert p.returncode == 0, p.stdout + p.stderr
-    out = dict(l.split("=", 1) for l in p.stdout.splitlines() if "=" in l)
-    assert out["radius_points"] == "31" and out["speed_points"] == "29"
-    assert float(out["node_err_over_tolerance"]) <= 1.0
-    assert out["off_radius_points"] == "0" and out["off_speed_points"] == "0" and out["off_equal"] == "1"
+def test_final_snapshot_report_matches_the_oracle(oracle_solar, tmp_path):
+    """The snapshot report at the first checkpoint after the job (the
+    reference's report keys, src/Snapshot_IO.cpp:508-646) carries the job's
+    final counters, capture-rate estimates with Wilson intervals and
+    histograms, which equal the oracle's Generate_Data run."""
+    nb = 100
+    reps = run_with_snapshots(tmp_path, 5000, nb)
+    label, kv, rows = reps[-1]
+    o = oracle_job(oracle_solar, dmsun.MODE_PARAMETER_POINT, 5000, 0, nb)
+    c = o["counters"]
+    print(f"{len(reps)} snapshot reports; final {label} s: {kv['total_trajectories']} trajectories", flush=True)
+    assert kv["snapshot_status"] == "merged" and int(kv["snapshot_target_wall_time_s"]) == label
+    assert int(kv["total_trajectories"]) == c.trajectories and int(kv["captured_particles"]) == c.captured
+    valid = c.captured + c.completed_outward_escapes
+    assert int(kv["valid_trajectories"]) == valid and int(kv["numerical_failures"]) == c.numerical_failures
+    assert int(kv["unresolved_not_captured_trajectories"]) == c.trajectories - valid
+    for prefix, n in (("capture_rate_raw", c.trajectories), ("capture_rate_valid", valid)):
+        rate, err, lo, hi = wilson(n, c.captured)
+        for suffix, want in (("", rate), ("_err", err), ("_CI_95_lower", lo), ("_CI_95_upper", hi)):
+            assert abs(float(kv[prefix + suffix]) - want) <= 6e-9, (prefix + suffix, kv[prefix + suffix], want)
+    assert rows.shape == (nb, 9)
+    h = o["hist"]
+    for col, q in ((1, 0), (2, 1), (5, 4), (6, 5)):
+        assert np.allclose(rows[:, col], h[q], rtol=2e-10, atol=0.0)
+
+
+def test_live_snapshot_reports_are_consistent(tmp_path):
+    """While a longer job runs (10^5 captured particles), a report every second
+    from the device's live counts: non-decreasing totals, captured <= valid <=
+    total, no histogram section until the final report.  (Live counts include
+    the trajectories a capture-target launch runs past its stop point, which
+    the final round accounting leaves out, so the final report's totals can
+    be below the last live ones.)"""
+    reps = run_with_snapshots(tmp_path, 100000, 100)
+    print(f"{len(reps)} snapshot reports at {[r[0] for r in reps]} s", flush=True)
+    assert len(reps) >= 2
+    keys = ["total_trajectories", "captured_particles", "valid_trajectories", "numerical_failures"]
+    prev = [0] * 4
+    for label, kv, rows in reps[:-1]:
+        cur = [int(kv[k]) for k in keys]
+        assert all(a <= b for a, b in zip(prev, cur)), (label, prev, cur)
+        assert cur[1] <= cur[2] <= cur[0] and int(kv["snapshot_target_wall_time_s"]) == label
+        prev = cur
+    assert all(r[2].size == 0 for r in reps[:-1]) and reps[-1][2].shape == (100, 9)
+    final = reps[-1][1]
+    assert int(final["captured_particles"]) >= 100000 and reps[-1][0] > reps[-2][0]
+    assert int(final["captured_particles"]) <= int(final["valid_trajectories"]) <= int(final["total_trajectories"])
