@@ -197,21 +197,25 @@ def measure_uneven(dmsun, ctx, args):
 
 
 def measure_serving(args):
-    """Serving loop (dmsun_serve): Poisson parameter-point requests admitted by
-    the device-memory budget, p50/p99 latency and trajectories/s."""
+    """Serving loop (dmsun_serve): Poisson parameter-point requests whose
+    trajectories join one resident launch as slots; p50/p99 latency,
+    trajectories/s, and the first requests rerun alone and compared."""
     import subprocess
     exe = ROOT / "damascus-earth-evap_amd" / "dmsun_serve"
     cmd = [str(exe), "--requests", str(args.serve_requests), "--rate", str(args.serve_rate),
-           "--sample-size", "1000", "--sample-size-max", "10000", "--max-trajectories", "200000",
-           "--max-scatterings", "100000", "--rk-budget", "100000", "--interp", "1000", "--bins", "1000", "--quiet"]
+           "--sample-size", "1000", "--sample-size-max", "100000", "--max-trajectories", "2000000",
+           "--max-scatterings", "100000", "--rk-budget", "100000", "--interp", "1000", "--bins", "1000",
+           "--check-alone", str(args.serve_check_alone), "--quiet"]
     print("serving: " + " ".join(cmd[1:]), file=sys.stderr, flush=True)
-    p = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
+    p = subprocess.run(cmd, capture_output=True, text=True, timeout=900)
     if p.returncode != 0:
         raise RuntimeError("dmsun_serve failed: " + p.stderr[-2000:])
     s = json.loads(p.stdout.strip().splitlines()[-1])
     s["workload"] = ("Poisson requests, SI m~U_log(0.1, 5) GeV, sigma_p~U_log(1e-38, 1e-35) cm^2, "
-                     "parameter-point mode with a captured-particle target ~U_log(1e3, 1e4), at most 1e5 scatterings "
-                     "and 1e5 RKF45 attempts per trajectory, 1000x1000 rate tables resident")
+                     "parameter-point mode with a captured-particle target ~U_log(1e3, 1e5) (at most 2e6 trajectories), "
+                     "at most 1e5 scatterings and 1e5 RKF45 attempts per trajectory, 1000x1000 rate tables resident; "
+                     "every request's trajectories join one resident launch as slots; the first "
+                     f"{args.serve_check_alone} requests rerun alone and compared")
     return s
 
 
@@ -304,7 +308,8 @@ def main():
     ap.add_argument("--uneven-trajectories", type=int, default=4000)
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=16)
-    ap.add_argument("--serve-rate", type=float, default=0.5)
+    ap.add_argument("--serve-rate", type=float, default=1.0)
+    ap.add_argument("--serve-check-alone", type=int, default=4)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run one plain device-to-device copy of BYTES bytes (scale of the HBM byte counters)")
     ap.add_argument("--validate-mixed", type=int, default=0, metavar="N",

@@ -35,7 +35,8 @@ enum DmsPhase
 	PH_RK = 5,
 	PH_POST = 6,
 	PH_SCAT = 7,
-	PH_FIN = 8
+	PH_FIN = 8,
+	PH_WAIT = 9 /* serving launch: holds a slot index not yet published */
 };
 
 /* Per-trajectory summary the host needs for run accounting. */

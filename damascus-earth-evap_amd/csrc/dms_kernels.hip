@@ -51,7 +51,239 @@ __device__ inline void dms_count_finished(unsigned int* w, uint32_t flags, int r
 		atomicAdd(w + 3, 1u);
 }
 
-template <bool MX, bool MP>
+/* ---- serving launch ------------------------------------------------------ */
+
+/* A structure in host memory read with system-scope loads (past every device
+ * cache: the host rewrites request and slot entries while the launch runs). */
+template <typename T>
+__device__ __forceinline__ T dms_sys_load(const T* p)
+{
+	static_assert(sizeof(T) % 8 == 0, "8-byte words");
+	unsigned long long w[sizeof(T) / 8];
+	const unsigned long long* q = reinterpret_cast<const unsigned long long*>(p);
+#pragma unroll
+	for(int i = 0; i < (int) (sizeof(T) / 8); i++)
+		w[i] = __hip_atomic_load(q + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+	T out;
+	__builtin_memcpy(&out, w, sizeof(T));
+	return out;
+}
+
+/* lane l's copy of a per-lane structure, broadcast to the wave */
+template <typename T>
+__device__ __forceinline__ T dms_readlane_struct(const T& v, int l)
+{
+	static_assert(sizeof(T) % 4 == 0, "4-byte words");
+	int w[sizeof(T) / 4];
+	__builtin_memcpy(w, &v, sizeof(T));
+#pragma unroll
+	for(int i = 0; i < (int) (sizeof(T) / 4); i++)
+		w[i] = __builtin_amdgcn_readlane(w[i], l);
+	T out;
+	__builtin_memcpy(&out, w, sizeof(T));
+	return out;
+}
+
+/* constant-clock ticks (100 MHz) between reads of the host's published count */
+#define DMS_SERVE_POLL_TICKS 200ull
+
+struct DmsServePoll
+{
+	uint32_t tick, idle;
+};
+
+/* Hand-back of finished slots (lanes with `completing` set; the wave calls it
+ * together).  The histogram rows of these trajectories are already in their
+ * round's sums.  Order: row atomics, then the round count (agent fence); the
+ * summary, then the completion-queue entry (system fence).  The slot that
+ * completes a round copies the round's sums to its host mailbox, clears them
+ * for the buffer's next round and stamps the mailbox. */
+__device__ __forceinline__ void dms_serve_complete(const DmsTrajArgs& a, const DmsLane& L, uint32_t round, const DmsSummary& sm,
+                                                   bool completing, int lane)
+{
+	if(__ballot(completing) == 0)
+		return;
+	const DmsServeView& V = a.V;
+	__threadfence();
+	bool closer = false;
+	if(completing)
+	{
+		V.results[L.slot] = sm;
+		if(round != DMS_SERVE_NO_ROUND)
+		{
+			const uint32_t size = __hip_atomic_load(V.round_size + round, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+			closer = atomicAdd(V.round_done + round, 1u) + 1u == size;
+		}
+	}
+	__threadfence_system();
+	if(completing)
+	{
+		const unsigned long long t = atomicAdd(V.counters + 1, 1ull);
+		__hip_atomic_store(V.cq + (uint32_t) (t & V.ring_mask), L.slot + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+	}
+	unsigned long long rm = __ballot(closer);
+	while(rm)
+	{
+		const int s = __ffsll((long long) rm) - 1;
+		rm &= rm - 1;
+		const uint32_t b = (uint32_t) __builtin_amdgcn_readlane((int) round, s);
+		const int nb = a.J.nbins;
+		/* every slot of the round counted itself after its row atomics */
+		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+		unsigned long long* dh = reinterpret_cast<unsigned long long*>(a.R.hist + (size_t) b * 8 * (size_t) nb);
+		unsigned long long* mh = reinterpret_cast<unsigned long long*>(V.mb_hist + (size_t) b * 8 * (size_t) nb);
+		for(int i = lane; i < 8 * nb; i += 64)
+		{
+			mh[i] = __hip_atomic_load(dh + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+			dh[i] = 0ull;
+		}
+		unsigned long long* dc = a.R.hcounts + (size_t) b * 2 * (size_t) nb;
+		unsigned long long* mc = V.mb_counts + (size_t) b * 2 * (size_t) nb;
+		for(int i = lane; i < 2 * nb; i += 64)
+		{
+			mc[i] = __hip_atomic_load(dc + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+			dc[i] = 0ull;
+		}
+		if(lane == 0)
+			__hip_atomic_store(V.round_done + b, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		__threadfence_system();
+		if(lane == 0)
+		{
+			const unsigned long long g = __hip_atomic_load(V.round_gen + b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+			__hip_atomic_store(V.round_stamp + b, g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+		}
+	}
+}
+
+/* Lane refill of a serving launch.  Lanes that need work take slot indices
+ * from the counter at once and hold them (PH_WAIT) until the host has
+ * published them; a published slot starts its trajectory with initial
+ * conditions drawn in-lane (the computation of dms_ic_kernel: same Philox
+ * stream, same bits).  A slot of a cancelled request, or one whose initial
+ * shift fails, is handed back at once (`completing`, summary in sm).  After
+ * the stop flag, lanes holding unpublished slots end.  A wave with nothing
+ * but waiting lanes backs off between reads of the host's published count. */
+template <bool MX>
+__device__ __forceinline__ void dms_serve_refill(const DmsTrajArgs& a, DmsLaneCtx& c, DmsLane& L, uint32_t& round, DmsSummary& sm,
+                                                 bool& completing, DmsServePoll& poll, int lane)
+{
+	const DmsServeView& V = a.V;
+	const unsigned long long nmask = __ballot(L.phase == PH_NEED);
+	if(nmask != 0)
+	{
+		const int leader = __ffsll((long long) nmask) - 1;
+		unsigned long long base = 0;
+		if(lane == leader)
+			base = atomicAdd(V.counters, (unsigned long long) __popcll(nmask));
+		base = ((unsigned long long) (uint32_t) __builtin_amdgcn_readlane((int) (base >> 32), leader) << 32)
+		       | (uint32_t) __builtin_amdgcn_readlane((int) (uint32_t) base, leader);
+		if(L.phase == PH_NEED)
+		{
+			L.id = V.base + base + (unsigned long long) __popcll(nmask & ((1ull << lane) - 1ull));
+			L.phase = PH_WAIT;
+		}
+	}
+	const unsigned long long wmask = __ballot(L.phase == PH_WAIT);
+	if(wmask == 0)
+	{
+		poll.idle = 0;
+		return;
+	}
+	const bool all_wait = __ballot((L.phase != PH_WAIT) & (L.phase != PH_DONE)) == 0;
+	const int leader = __ffsll((long long) wmask) - 1;
+	unsigned long long pub = 0, stop = 0;
+	if(lane == leader)
+	{
+		/* The published count and the stop flag come from their device-memory
+		 * mirror; at most one wave of the launch reads the host's copies per
+		 * DMS_SERVE_POLL_TICKS of the constant clock (the wave that claims the
+		 * read time), because reads of host memory from thousands of waiting
+		 * waves slow the memory traffic of the running lanes.  The slot and
+		 * request entries are read only after the count has arrived (their
+		 * loads depend on it), with system-scope loads that do not hit in the
+		 * device caches; the host writes them before it raises the count. */
+		const unsigned long long now = wall_clock64();
+		unsigned long long last = __hip_atomic_load(V.counters + 4, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		if(now - last >= DMS_SERVE_POLL_TICKS
+		   && __hip_atomic_compare_exchange_strong(V.counters + 4, &last, now, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+		                                           __HIP_MEMORY_SCOPE_AGENT))
+		{
+			pub = __hip_atomic_load(V.ctl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+			stop = __hip_atomic_load(V.ctl + 8, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+			atomicMax(V.counters + 2, pub);
+			if(stop)
+				__hip_atomic_store(V.counters + 3, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		}
+		else
+		{
+			pub = __hip_atomic_load(V.counters + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+			stop = __hip_atomic_load(V.counters + 3, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		}
+	}
+	pub = ((unsigned long long) (uint32_t) __builtin_amdgcn_readlane((int) (pub >> 32), leader) << 32)
+	      | (uint32_t) __builtin_amdgcn_readlane((int) (uint32_t) pub, leader);
+	stop = (unsigned long long) (uint32_t) __builtin_amdgcn_readlane((int) (uint32_t) stop, leader);
+	bool started = false;
+	if(L.phase == PH_WAIT)
+	{
+		if(L.id < pub)
+		{
+			const uint32_t pos = (uint32_t) L.id & V.ring_mask;
+			const DmsServeSlot d = dms_sys_load(V.slots + pos);
+			const DmsServeReq* q = V.reqs + d.req;
+			L.slot = pos;
+			L.row = (uint32_t) (blockIdx.x * blockDim.x + threadIdx.x);
+			L.job = d.req;
+			L.id = d.id;
+			round = d.round;
+			started = true;
+			if(__hip_atomic_load(&q->cancel, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM) != 0u)
+			{
+				sm.reason = DMS_SERVE_CANCELLED;
+				sm.flags = 0;
+				sm.n_scat = 0;
+				sm.t_capture_sec = -1.0;
+				sm.t_unbind_sec = -1.0;
+				sm.n_rk = 0;
+				sm.n_int = sm.n_ext = sm.n_rej = sm.n_tv = 0;
+				L.phase = PH_NEED;
+				completing = true;
+			}
+			else
+			{
+				c.P = dms_sys_load(&q->P);
+				const DmsHaloView H = dms_sys_load(&q->H);
+				const unsigned long long keys = __hip_atomic_load(reinterpret_cast<const unsigned long long*>(&q->seed),
+				                                                  __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+				c.J.seed = (uint32_t) keys;
+				c.J.point_id = (uint32_t) (keys >> 32);
+				dm_rng rng;
+				dm_rng_init(&rng, c.J.seed, c.J.point_id, L.id, DMS_STREAM_IC);
+				dms_event e;
+				dms_initial_condition(H, &rng, &e);
+				const bool shift_ok = dms_kepler_shift(e.r, e.v, c.J.max_distance);
+				dms_lane_start(L, c, e, shift_ok, nullptr, &sm);
+				if(L.phase == PH_NEED) /* initial shift failed: recorded in sm */
+					completing = true;
+			}
+		}
+		else if(stop)
+			L.phase = PH_DONE;
+	}
+	if(all_wait && __ballot(started) == 0)
+	{
+		/* nothing published for this wave yet: back off, up to ~50 us between
+		 * reads of the mirror */
+		if(poll.idle < 16)
+			poll.idle++;
+		for(uint32_t k = 0; k < poll.idle; k++)
+			__builtin_amdgcn_s_sleep(127);
+	}
+	else
+		poll.idle = 0;
+}
+
+template <bool MX, bool MP, bool SV = false>
 __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 {
 	const int lane = threadIdx.x & 63;
@@ -90,6 +322,11 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 	L.sc_target = -2;
 	const uint32_t lane_gid = blockIdx.x * blockDim.x + threadIdx.x;
 	const int n_iso = a.S.n_iso;
+	/* serving launch: the slot's round buffer, the summary of a trajectory
+	 * being handed back, and the wave's polling state */
+	uint32_t sv_round = DMS_SERVE_NO_ROUND;
+	DmsSummary sv_sum;
+	DmsServePoll sv_poll = {0u, 0u};
 	for(;;)
 	{
 		dms_lane_advance_a<MX>(L, c);
@@ -97,7 +334,17 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 		 * capture counting, trajectory end, row flush, refill, start) act only
 		 * on lanes in those phases: while every live lane is between RKF45
 		 * attempts, as a lone long trajectory is, the wave skips them at once. */
-		const bool service = __ballot((L.phase != PH_DONE) & (L.phase != PH_PREP) & (L.phase != PH_RK)) != 0;
+		bool service;
+		if constexpr(SV)
+		{
+			/* lanes waiting for published slots look for them on every 16th
+			 * pass only, so that a lone trajectory beside them keeps the fast
+			 * path */
+			const bool busy = __ballot((L.phase != PH_DONE) & (L.phase != PH_PREP) & (L.phase != PH_RK) & (L.phase != PH_WAIT)) != 0;
+			service = busy || (__ballot(L.phase == PH_WAIT) != 0 && ((++sv_poll.tick) & 15u) == 0u);
+		}
+		else
+			service = __ballot((L.phase != PH_DONE) & (L.phase != PH_PREP) & (L.phase != PH_RK)) != 0;
 		if(service)
 		{
 			/* Sample_Target (src/Simulation_Trajectory.cpp:1122-1158): for each
@@ -114,8 +361,12 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 				const double xi = readlane_d(L.sc_xi, s);
 				const int k = (lane < n_iso) ? lane : ((lane == DMS_MAX_TARGETS - 1) ? lane : -1);
 				int target = -2;
-				/* the scattering lane's parameter point (uniform over the wave) */
-				const DmsPointView& Ps = MP ? a.job_points[__builtin_amdgcn_readlane((int) L.job, s)] : a.P;
+				/* the scattering lane's parameter point (uniform over the wave):
+				 * a serving launch's lanes hold their points in registers */
+				DmsPointView Pv;
+				if constexpr(SV)
+					Pv = dms_readlane_struct(c.P, s);
+				const DmsPointView& Ps = SV ? Pv : (MP ? a.job_points[__builtin_amdgcn_readlane((int) L.job, s)] : a.P);
 				if constexpr(MX)
 				{
 					/* FP32 rates (1/s), same draw and order as below */
@@ -198,14 +449,23 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 				}
 			}
 			int flush_cls = -1;
-			if(__builtin_expect(L.phase == PH_FIN, 0))
+			const bool finishing = L.phase == PH_FIN;
+			if(__builtin_expect(finishing, 0))
 			{
 				uint32_t fl = 0;
 				int reason = 0;
-				flush_cls = dms_lane_finish_rows_pending(L, c, a.ics[L.slot], a.records ? a.records + L.slot : nullptr,
-				                                         a.summ ? a.summ + L.slot : nullptr, &fl, &reason);
-				if(a.job_done)
-					dms_count_finished(a.job_done + DMS_PROGRESS_WORDS * L.job, fl, reason, c.J.terminate_on_capture);
+				if constexpr(SV)
+				{
+					const dms_event ic0 = {};
+					flush_cls = dms_lane_finish_rows_pending(L, c, ic0, nullptr, &sv_sum);
+				}
+				else
+				{
+					flush_cls = dms_lane_finish_rows_pending(L, c, a.ics[L.slot], a.records ? a.records + L.slot : nullptr,
+					                                         a.summ ? a.summ + L.slot : nullptr, &fl, &reason);
+					if(a.job_done)
+						dms_count_finished(a.job_done + DMS_PROGRESS_WORDS * L.job, fl, reason, c.J.terminate_on_capture);
+				}
 			}
 			/* per-lane histogram rows of the finished trajectories: the wave adds
 			 * each row's touched bins to its job's sums, 64 bins at a time */
@@ -217,7 +477,7 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 					const int s = __ffsll((long long) fm) - 1;
 					fm &= fm - 1;
 					const uint32_t row_s = (uint32_t) __builtin_amdgcn_readlane((int) L.row, s);
-					const uint32_t job_s = (uint32_t) __builtin_amdgcn_readlane((int) L.job, s);
+					const uint32_t job_s = (uint32_t) __builtin_amdgcn_readlane((int) (SV ? sv_round : L.job), s);
 					const uint32_t n_s = (uint32_t) __builtin_amdgcn_readlane((int) L.n_touched, s);
 					const int cls_s = __builtin_amdgcn_readlane(flush_cls, s);
 					dms_flush_rows(c.R, c.J.nbins, row_s, job_s, n_s, cls_s, (uint32_t) lane, 64u);
@@ -227,6 +487,15 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 				/* the cleared rows are reused by their lanes' next trajectories */
 				__threadfence_block();
 			}
+			if constexpr(SV)
+			{
+				dms_serve_complete(a, L, sv_round, sv_sum, finishing, lane);
+				bool handed_back = false;
+				dms_serve_refill<MX>(a, c, L, sv_round, sv_sum, handed_back, sv_poll, lane);
+				dms_serve_complete(a, L, sv_round, sv_sum, handed_back, lane);
+			}
+			else
+			{
 			/* lane refill: one atomic per wave */
 			const unsigned long long nmask = __ballot(L.phase == PH_NEED);
 			if(__builtin_expect(nmask != 0, 0))
@@ -282,6 +551,7 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 					dms_count_finished(a.job_done + DMS_PROGRESS_WORDS * L.job, DMS_F_SHIFT_FAILED, DMS_NUMERICAL_FAILURE,
 					                   c.J.terminate_on_capture);
 			}
+			} /* not a serving launch */
 		}
 		dms_lane_advance_b<MX>(L, c);
 		if(__ballot(L.phase != PH_DONE) == 0)
@@ -296,6 +566,10 @@ __global__ void __launch_bounds__(DMS_TRAJ_BLOCK) dms_traj_kernel_mixed(DmsTrajA
 __global__ void __launch_bounds__(DMS_TRAJ_BLOCK) dms_traj_kernel_multi(DmsTrajArgs a) { dms_traj_body<false, true>(a); }
 
 __global__ void __launch_bounds__(DMS_TRAJ_BLOCK) dms_traj_kernel_mixed_multi(DmsTrajArgs a) { dms_traj_body<true, true>(a); }
+
+__global__ void __launch_bounds__(DMS_TRAJ_BLOCK) dms_traj_kernel_serve(DmsTrajArgs a) { dms_traj_body<false, true, true>(a); }
+
+__global__ void __launch_bounds__(DMS_TRAJ_BLOCK) dms_traj_kernel_mixed_serve(DmsTrajArgs a) { dms_traj_body<true, true, true>(a); }
 
 /* FP32 copy of the rate table in 1/s for the mixed-precision kernel */
 __global__ void dms_grid_f32_kernel(const double* grid, size_t n, float* out)

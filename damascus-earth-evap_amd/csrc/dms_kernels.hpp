@@ -4,6 +4,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
+
 #include "dms_engine.h"
 #include "dms_halo.h"
 
@@ -14,6 +16,62 @@ __host__ __device__ inline size_t dms_lds_mass_offset(int n) { return ((size_t) 
 __host__ __device__ inline size_t dms_lds_bytes(int n) { return dms_lds_mass_offset(n) + 2 * (size_t) (n - 1) * sizeof(DmsCubic); }
 
 #define DMS_PROGRESS_WORDS 4
+
+/* Serving launch: one persistent launch that runs while a host thread appends
+ * trajectories of admitted requests as slots.  Everything the host and the
+ * running launch exchange lives in host memory mapped for the device
+ * (fine-grained, read and written with system-scope accesses), so that no copy
+ * or kernel has to run beside the launch.
+ *
+ * Slot idx (handed out in order from `next`, offset by `base`) sits at ring
+ * position idx & ring_mask; the host publishes slots by writing their
+ * descriptors and then raising ctl[0] (slots published).  A finished slot's
+ * summary goes to results[position], then position + 1 to the completion
+ * queue cq.  Histogram rows are summed per sync round: round buffer b holds the
+ * sums of one round of one request in device memory (R.hist/R.hcounts with job
+ * index b); the slot that completes the round copies them to the host mailbox
+ * b, clears them and stamps round_stamp[b] with round_gen[b]. */
+#define DMS_SERVE_NO_ROUND 0xffffffffu
+#define DMS_SERVE_CANCELLED (-2)
+
+struct DmsServeReq
+{
+	DmsPointView P;
+	DmsHaloView H;
+	uint32_t seed, point_id;
+	uint32_t cancel; /* 1: slots not yet started end at once (summary reason DMS_SERVE_CANCELLED) */
+	uint32_t pad;
+};
+
+static_assert(sizeof(DmsPointView) % 8 == 0 && sizeof(DmsHaloView) % 8 == 0, "serving loads read 8-byte words");
+static_assert(offsetof(DmsServeReq, seed) % 8 == 0 && sizeof(DmsServeReq) % 8 == 0, "serving loads read 8-byte words");
+
+struct DmsServeSlot
+{
+	uint32_t req;   /* request table index */
+	uint32_t round; /* round buffer, or DMS_SERVE_NO_ROUND */
+	uint64_t id;    /* trajectory id (Philox counter) */
+};
+
+struct DmsServeView
+{
+	const unsigned long long* ctl; /* host: [0] slots published, [8] stop */
+	unsigned long long base;       /* slot index of this launch's first handed-out slot */
+	uint32_t ring_mask;
+	const DmsServeSlot* slots;     /* host, ring */
+	const DmsServeReq* reqs;       /* host, request table */
+	DmsSummary* results;           /* host, ring */
+	uint32_t* cq;                  /* host, ring: ring position + 1 of finished slots */
+	unsigned long long* counters;  /* device: [0] slots handed out, [1] completion tickets,
+	                                * [2] published count and [3] stop flag as last read from
+	                                * the host, [4] constant-clock time of that read */
+	const uint32_t* round_size;    /* host */
+	const unsigned long long* round_gen; /* host */
+	unsigned long long* round_stamp;     /* host */
+	double* mb_hist;               /* host [round][8][nbins] */
+	unsigned long long* mb_counts; /* host [round][2][nbins] */
+	uint32_t* round_done;          /* device: finished slots per round buffer */
+};
 
 struct DmsTrajArgs
 {
@@ -46,12 +104,16 @@ struct DmsTrajArgs
 	 * fill_extra, and no slot at or past it is started */
 	unsigned int* fill_ctl;
 	uint32_t cap_stop, fill_extra;
+	/* serving launch (dms_traj_kernel_serve*) */
+	DmsServeView V;
 };
 
 __global__ void dms_traj_kernel(DmsTrajArgs a);
 __global__ void dms_traj_kernel_mixed(DmsTrajArgs a);
 __global__ void dms_traj_kernel_multi(DmsTrajArgs a);
 __global__ void dms_traj_kernel_mixed_multi(DmsTrajArgs a);
+__global__ void dms_traj_kernel_serve(DmsTrajArgs a);
+__global__ void dms_traj_kernel_mixed_serve(DmsTrajArgs a);
 __global__ void dms_grid_f32_kernel(const double* grid, size_t n, float* out);
 __global__ void dms_ic_kernel(DmsHaloView H, uint32_t seed, uint32_t point_id, uint64_t first_id, uint32_t n, double R_final,
                               dms_event* out, int32_t* ok);

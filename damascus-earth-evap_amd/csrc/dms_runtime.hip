@@ -20,7 +20,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <condition_variable>
+#include <deque>
 #include <functional>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -30,15 +32,17 @@
 #include "dms_host.hpp"
 #include "dms_kernels.hip"
 
-/* jobs (or batches) that may run on the device at the same time */
-#define DMS_CONCURRENT_JOBS 8
+/* jobs (or batches, or a resident serving launch) that may run on the device
+ * at the same time: with the context stream and the shared progress stream,
+ * the process uses four streams, one per hardware queue of the device */
+#define DMS_CONCURRENT_JOBS 2
 
 struct dms_ctx
 {
 	int device = -1;
 	hipStream_t stream = nullptr;
-	hipStream_t progress_stream = nullptr; /* progress reads while a launch runs */
-	/* streams of concurrently running jobs (pairs: work, progress reads) */
+	hipStream_t progress_stream = nullptr; /* progress reads while launches run */
+	/* work streams of concurrently running jobs */
 	std::vector<hipStream_t> pool;
 	std::vector<int> pool_free;
 	std::mutex pool_mu;
@@ -93,6 +97,11 @@ struct dms_job
 	double batch_size = 1.0;
 	std::atomic<uint64_t> done_traj{0}, done_captured{0}, done_scat{0}, done_escaped{0}, done_failed{0};
 	std::atomic<int> finished{0};
+	/* jobs of a dms_server: completion is signalled here (the server may be
+	 * gone by the time the caller waits) */
+	struct dms_server* server = nullptr;
+	std::mutex fmu;
+	std::condition_variable fcv;
 	int status = DMS_OK;
 	std::string err;
 	dms_counters C;
@@ -1046,7 +1055,7 @@ void run_batch_worker(std::shared_ptr<dms_batch> batch)
 		slot = ctx->pool_free.back();
 		ctx->pool_free.pop_back();
 	}
-	const hipStream_t st = ctx->pool[2 * slot], pst = ctx->pool[2 * slot + 1];
+	const hipStream_t st = ctx->pool[slot], pst = ctx->progress_stream;
 	hipSetDevice(ctx->device);
 	if(batch->jobs.size() == 1 && !batchable(batch->jobs[0]->desc))
 		run_job_chunked(batch->jobs[0], st, pst);
@@ -1059,6 +1068,665 @@ void run_batch_worker(std::shared_ptr<dms_batch> batch)
 	ctx->pool_cv.notify_one();
 	for(dms_job* j : batch->jobs)
 		j->finished = 1;
+}
+
+} // namespace
+
+/* ---- serving: requests join one resident launch as slots -------------------
+ *
+ * One persistent trajectory launch (dms_traj_kernel_serve) stays on the device
+ * while requests arrive; a host thread (the driver) appends each admitted
+ * request's trajectories to the launch's work queue as slots, exactly as the
+ * reference's ranks take trajectories one by one inside Generate_Data's sync
+ * rounds (src/Data_Generation.cpp:697-728), and folds every handed-back
+ * summary into its request strictly in trajectory-id order with the
+ * reference's round logic (RoundReplay, account_trajectory).  Histogram rows
+ * are summed on the device per sync round of a request (the run-level stops
+ * fall on round boundaries only, so a request's counted trajectories are
+ * always whole rounds); the driver adds the counted rounds' sums in round
+ * order.  The host and the running launch talk only through host memory the
+ * device maps (slot descriptors, request table, summaries, completion queue,
+ * round mailboxes): no copy or kernel has to run beside the launch, so it
+ * never waits on a hardware queue or compute units it holds itself.  With no
+ * request left the driver parks the launch (stop flag, launch ends) and
+ * relaunches it for the next request. */
+#define DMS_SERVE_COUNTERS 8
+
+struct ServeRoundSums
+{
+	std::vector<double> h;
+	std::vector<unsigned long long> c;
+};
+
+struct ServeRequest
+{
+	dms_job* job = nullptr;
+	uint32_t slot = 0; /* request table index */
+	RoundReplay R;
+	uint64_t interval = 1, limit = UINT64_MAX;
+	uint64_t n_pub = 0;    /* trajectories published (offsets 0 .. n_pub-1) */
+	uint64_t n_acc = 0;    /* replay cursor: offsets accounted */
+	uint64_t in_flight = 0; /* published, not yet handed back */
+	uint64_t rounds_out = 0; /* round buffers not yet returned */
+	uint64_t simulated = 0;
+	uint32_t cur_round = DMS_SERVE_NO_ROUND; /* buffer of the round being published */
+	uint64_t cur_round_end = 0;              /* offset past the round being published */
+	std::deque<DmsSummary> sums; /* offsets n_acc.. */
+	std::deque<uint8_t> have;
+	uint64_t counted_rounds = 0; /* rounds [0, counted_rounds) count */
+	uint64_t next_add = 0;       /* next counted round to add to the histogram */
+	std::map<uint64_t, ServeRoundSums> mb;
+	bool final_count = false; /* replay stopped: the counted set is final */
+	bool cancelled = false;
+	double t_start = 0.0;
+};
+
+struct ServeRoundBuf
+{
+	ServeRequest* req = nullptr;
+	uint64_t k = 0, gen = 0;
+};
+
+template <typename T>
+hipError_t host_alloc(T** p, size_t count)
+{
+	*p = nullptr;
+	hipError_t e = hipHostMalloc((void**) p, sizeof(T) * (count ? count : 1), hipHostMallocMapped | hipHostMallocCoherent);
+	if(e == hipSuccess)
+		memset((void*) *p, 0, sizeof(T) * (count ? count : 1));
+	return e;
+}
+
+template <typename T>
+T* device_view(T* host)
+{
+	void* d = nullptr;
+	if(host == nullptr || hipHostGetDevicePointer(&d, (void*) host, 0) != hipSuccess)
+		return nullptr;
+	return (T*) d;
+}
+
+struct dms_server
+{
+	dms_ctx* ctx = nullptr;
+	dms_job_desc cfg;
+	DmsJobView J;
+	bool want_rows = false;
+	int nb = 0;
+	uint32_t ring = 0, mask = 0, max_reqs = 0, max_rounds = 0;
+	int grid = 0;
+	size_t lanes = 0;
+	/* host memory the device maps */
+	unsigned long long* h_ctl = nullptr;
+	DmsServeSlot* h_slots = nullptr;
+	DmsServeReq* h_reqs = nullptr;
+	DmsSummary* h_results = nullptr;
+	uint32_t* h_cq = nullptr;
+	uint32_t* h_round_size = nullptr;
+	unsigned long long *h_round_gen = nullptr, *h_round_stamp = nullptr;
+	double* h_mb_hist = nullptr;
+	unsigned long long* h_mb_counts = nullptr;
+	/* device memory */
+	unsigned long long* d_counters = nullptr;
+	uint32_t* d_round_done = nullptr;
+	double* d_round_hist = nullptr;
+	unsigned long long* d_round_counts = nullptr;
+	double* d_rows = nullptr;
+	uint32_t* d_counts = nullptr;
+	uint16_t* d_touched = nullptr;
+	/* driver state (driver thread only) */
+	std::vector<ServeRequest*> pos_req;
+	std::vector<uint64_t> pos_off;
+	uint64_t pub = 0, cq_cursor = 0, gen = 0;
+	std::vector<ServeRoundBuf> rounds;
+	std::vector<uint32_t> free_rounds;
+	std::vector<uint32_t> free_reqs;
+	std::vector<ServeRequest*> active;
+	size_t rr = 0;
+	bool running = false;
+	int pool_slot = -1;
+	hipStream_t st = nullptr;
+	hipEvent_t ended = nullptr;
+	double idle_since = -1.0, last_check = 0.0;
+	/* shared with submitting threads */
+	std::mutex mu;
+	std::condition_variable cv;
+	std::deque<dms_job*> pending;
+	bool closing = false;
+	std::thread driver;
+	std::atomic<uint64_t> n_launches{0}, n_published{0}, n_cancelled{0}, n_done{0}, n_active{0};
+	std::atomic<int> is_running{0};
+};
+
+namespace
+{
+
+void serve_release(dms_server* S)
+{
+	hipHostFree(S->h_ctl);
+	hipHostFree(S->h_slots);
+	hipHostFree(S->h_reqs);
+	hipHostFree(S->h_results);
+	hipHostFree(S->h_cq);
+	hipHostFree(S->h_round_size);
+	hipHostFree(S->h_round_gen);
+	hipHostFree(S->h_round_stamp);
+	hipHostFree(S->h_mb_hist);
+	hipHostFree(S->h_mb_counts);
+	const hipStream_t cs = S->ctx->stream;
+	std::lock_guard<std::mutex> lock(S->ctx->mu);
+	dfree(S->d_counters, cs);
+	dfree(S->d_round_done, cs);
+	dfree(S->d_round_hist, cs);
+	dfree(S->d_round_counts, cs);
+	dfree(S->d_rows, cs);
+	dfree(S->d_counts, cs);
+	dfree(S->d_touched, cs);
+	hipStreamSynchronize(cs);
+	if(S->ended)
+		hipEventDestroy(S->ended);
+}
+
+void serve_finish_job(dms_server* S, dms_job* job, int status, const std::string& err)
+{
+	job->status = status;
+	job->err = err;
+	S->n_done++;
+	std::lock_guard<std::mutex> lock(job->fmu);
+	job->finished = 1;
+	job->fcv.notify_all();
+}
+
+/* (Re)start the resident launch: a stream from the context's pool, cleared
+ * counters, the stop flag down, slots handed out from the published count. */
+bool serve_launch(dms_server* S, std::string& err)
+{
+	dms_ctx* ctx = S->ctx;
+	{
+		std::unique_lock<std::mutex> lock(ctx->pool_mu);
+		ctx->pool_cv.wait(lock, [ctx] { return !ctx->pool_free.empty(); });
+		S->pool_slot = ctx->pool_free.back();
+		ctx->pool_free.pop_back();
+	}
+	S->st = ctx->pool[S->pool_slot];
+	hipSetDevice(ctx->device);
+	DmsTrajArgs a;
+	memset(&a, 0, sizeof(a));
+	a.S = ctx->dev;
+	a.J = S->J;
+	a.R.per_lane = S->want_rows ? 1 : 0;
+	a.R.rows = S->d_rows;
+	a.R.counts = S->d_counts;
+	a.R.touched = S->d_touched;
+	a.R.hist = S->d_round_hist;
+	a.R.hcounts = S->d_round_counts;
+	a.V.ctl = device_view(S->h_ctl);
+	a.V.base = S->pub;
+	a.V.ring_mask = S->mask;
+	a.V.slots = device_view(S->h_slots);
+	a.V.reqs = device_view(S->h_reqs);
+	a.V.results = device_view(S->h_results);
+	a.V.cq = device_view(S->h_cq);
+	a.V.counters = S->d_counters;
+	a.V.round_size = device_view(S->h_round_size);
+	a.V.round_gen = device_view(S->h_round_gen);
+	a.V.round_stamp = device_view(S->h_round_stamp);
+	a.V.mb_hist = device_view(S->h_mb_hist);
+	a.V.mb_counts = device_view(S->h_mb_counts);
+	a.V.round_done = S->d_round_done;
+	if(!a.V.ctl || !a.V.slots || !a.V.reqs || !a.V.results || !a.V.cq || !a.V.round_size || !a.V.round_gen || !a.V.round_stamp
+	   || (S->want_rows && (!a.V.mb_hist || !a.V.mb_counts)))
+	{
+		err = "hipHostGetDevicePointer failed for the serving launch's host memory";
+		{
+			std::lock_guard<std::mutex> lock(ctx->pool_mu);
+			ctx->pool_free.push_back(S->pool_slot);
+		}
+		ctx->pool_cv.notify_one();
+		S->pool_slot = -1;
+		return false;
+	}
+	S->cq_cursor = 0;
+	__atomic_store_n(S->h_ctl + 8, 0ull, __ATOMIC_RELEASE);
+	(void) hipGetLastError();
+	hipError_t e = hipMemsetAsync(S->d_counters, 0, DMS_SERVE_COUNTERS * sizeof(unsigned long long), S->st);
+	if(e == hipSuccess)
+	{
+		void (*kernel)(DmsTrajArgs) = S->J.mixed ? dms_traj_kernel_mixed_serve : dms_traj_kernel_serve;
+		hipLaunchKernelGGL(kernel, dim3(S->grid), dim3(DMS_TRAJ_BLOCK), ctx->traj_lds, S->st, a);
+		e = hipGetLastError();
+	}
+	if(e == hipSuccess)
+		e = hipEventRecord(S->ended, S->st);
+	if(e != hipSuccess)
+	{
+		err = std::string("serving launch: ") + hipGetErrorString(e);
+		hipStreamSynchronize(S->st);
+		{
+			std::lock_guard<std::mutex> lock(ctx->pool_mu);
+			ctx->pool_free.push_back(S->pool_slot);
+		}
+		ctx->pool_cv.notify_one();
+		S->pool_slot = -1;
+		return false;
+	}
+	S->running = true;
+	S->is_running = 1;
+	S->n_launches++;
+	return true;
+}
+
+/* Park the launch: every published slot has been handed back, so the lanes
+ * hold unpublished slots only and end at the stop flag. */
+bool serve_park(dms_server* S, std::string& err)
+{
+	__atomic_store_n(S->h_ctl + 8, 1ull, __ATOMIC_RELEASE);
+	const hipError_t e = hipEventSynchronize(S->ended);
+	S->running = false;
+	S->is_running = 0;
+	{
+		std::lock_guard<std::mutex> lock(S->ctx->pool_mu);
+		S->ctx->pool_free.push_back(S->pool_slot);
+	}
+	S->ctx->pool_cv.notify_one();
+	S->pool_slot = -1;
+	if(e != hipSuccess)
+	{
+		err = std::string("serving launch: ") + hipGetErrorString(e);
+		return false;
+	}
+	return true;
+}
+
+bool serve_admit(dms_server* S, dms_job* job)
+{
+	if(S->free_reqs.empty())
+		return false;
+	ServeRequest* q = new ServeRequest();
+	q->job = job;
+	q->slot = S->free_reqs.back();
+	S->free_reqs.pop_back();
+	const dms_job_desc& d = job->desc;
+	memset(&job->C, 0, sizeof(job->C));
+	job->events.clear();
+	job->records.clear();
+	job->hist.assign((size_t) 8 * S->nb, 0.0);
+	job->counts.assign((size_t) 2 * S->nb, 0ull);
+	q->R.init(d, job->pt, &job->C);
+	q->interval = q->R.interval > 0 ? q->R.interval : 1;
+	q->limit = q->R.unlimited ? UINT64_MAX : q->R.max_traj;
+	q->t_start = now_ms();
+	DmsServeReq r;
+	memset(&r, 0, sizeof(r));
+	r.P = job->pt->dev;
+	r.H = halo_view(S->ctx, d.halo);
+	r.seed = d.seed;
+	r.point_id = d.point_id;
+	r.cancel = 0;
+	/* written before any slot of the request is published (release below) */
+	S->h_reqs[q->slot] = r;
+	S->active.push_back(q);
+	S->n_active = S->active.size();
+	return true;
+}
+
+/* Trajectories the request should have published by now: all of a fixed
+ * count; for a capture target, the expected remainder with margin, as the
+ * chunked path sizes its chunks. */
+uint64_t serve_want(const ServeRequest* q)
+{
+	if(q->R.target == UINT64_MAX)
+		return q->limit;
+	const dms_counters& C = q->job->C;
+	const double rate = (C.trajectories > 0 && C.captured > 0) ? (double) C.captured / (double) C.trajectories : 0.0;
+	const uint64_t need = q->R.target > q->R.captured ? q->R.target - q->R.captured : 1;
+	uint64_t ahead = rate > 0.0 ? (uint64_t) (1.5 * (double) need / rate) + 4096 : 16384;
+	uint64_t want = q->n_acc + ahead;
+	if(want > q->limit)
+		want = q->limit;
+	return want;
+}
+
+/* Append slots, round-robin over the requests, a quantum each per pass; a
+ * cancelled request only completes the round it had started (as handed-back
+ * slots).  Returns the number of slots published. */
+uint64_t serve_publish(dms_server* S)
+{
+	const uint64_t first = S->pub;
+	const size_t n = S->active.size();
+	const uint64_t quantum = std::max<uint64_t>(1024, S->ring / 16);
+	bool ring_full = false;
+	for(size_t i = 0; i < n && !ring_full; i++)
+	{
+		ServeRequest* q = S->active[(S->rr + i) % n];
+		uint64_t want;
+		if(q->cancelled)
+			want = (q->cur_round != DMS_SERVE_NO_ROUND) ? q->cur_round_end : q->n_pub;
+		else
+			want = serve_want(q);
+		/* one request holds at most half the ring */
+		const uint64_t cap = q->in_flight < S->ring / 2 ? S->ring / 2 - q->in_flight : 0;
+		uint64_t budget = std::min<uint64_t>(quantum, cap);
+		while(budget > 0 && q->n_pub < want)
+		{
+			if(S->want_rows && q->cur_round == DMS_SERVE_NO_ROUND)
+			{
+				if(S->free_rounds.empty())
+					break;
+				const uint32_t b = S->free_rounds.back();
+				S->free_rounds.pop_back();
+				const uint64_t k = q->n_pub / q->interval;
+				const uint64_t end = std::min<uint64_t>((k + 1) * q->interval, q->limit);
+				ServeRoundBuf& rb = S->rounds[b];
+				rb.req = q;
+				rb.k = k;
+				rb.gen = ++S->gen;
+				S->h_round_size[b] = (uint32_t) (end - q->n_pub);
+				S->h_round_gen[b] = rb.gen;
+				q->cur_round = b;
+				q->cur_round_end = end;
+				q->rounds_out++;
+			}
+			const uint64_t p = S->pub & S->mask;
+			if(S->pos_req[p] != nullptr)
+			{
+				ring_full = true;
+				break;
+			}
+			DmsServeSlot sl;
+			sl.req = q->slot;
+			sl.round = S->want_rows ? q->cur_round : DMS_SERVE_NO_ROUND;
+			sl.id = q->job->desc.first_trajectory_id + q->n_pub;
+			S->h_slots[p] = sl;
+			S->pos_req[p] = q;
+			S->pos_off[p] = q->n_pub;
+			S->pub++;
+			q->n_pub++;
+			q->in_flight++;
+			budget--;
+			if(S->want_rows && q->n_pub == q->cur_round_end)
+				q->cur_round = DMS_SERVE_NO_ROUND;
+		}
+	}
+	if(n > 0)
+		S->rr = (S->rr + 1) % n;
+	if(S->pub != first)
+	{
+		__atomic_store_n(S->h_ctl, (unsigned long long) S->pub, __ATOMIC_RELEASE);
+		S->n_published += S->pub - first;
+	}
+	return S->pub - first;
+}
+
+/* Hand-backs from the completion queue: the summary moves to its request,
+ * the ring position is free again. */
+uint64_t serve_drain_cq(dms_server* S)
+{
+	uint64_t k = 0;
+	for(;;)
+	{
+		uint32_t* e = S->h_cq + (S->cq_cursor & S->mask);
+		const uint32_t v = __atomic_load_n(e, __ATOMIC_ACQUIRE);
+		if(v == 0)
+			break;
+		__atomic_store_n(e, 0u, __ATOMIC_RELAXED);
+		S->cq_cursor++;
+		k++;
+		const uint32_t p = v - 1;
+		ServeRequest* q = S->pos_req[p];
+		const uint64_t off = S->pos_off[p];
+		DmsSummary sm;
+		memcpy(&sm, (const void*) (S->h_results + p), sizeof(sm));
+		S->pos_req[p] = nullptr;
+		if(q == nullptr)
+			continue;
+		q->in_flight--;
+		if(sm.reason == DMS_SERVE_CANCELLED)
+		{
+			S->n_cancelled++;
+			continue;
+		}
+		q->simulated++;
+		if(q->final_count || off < q->n_acc)
+			continue;
+		const uint64_t i = off - q->n_acc;
+		if(q->sums.size() <= i)
+		{
+			q->sums.resize(i + 1);
+			q->have.resize(i + 1, 0);
+		}
+		q->sums[i] = sm;
+		q->have[i] = 1;
+	}
+	return k;
+}
+
+/* Round mailboxes stamped by the device: kept while the round may count. */
+uint64_t serve_drain_rounds(dms_server* S)
+{
+	uint64_t k = 0;
+	const size_t nb = (size_t) S->nb;
+	for(uint32_t b = 0; b < S->max_rounds; b++)
+	{
+		ServeRoundBuf& rb = S->rounds[b];
+		if(rb.req == nullptr || __atomic_load_n(S->h_round_stamp + b, __ATOMIC_ACQUIRE) != rb.gen)
+			continue;
+		ServeRequest* q = rb.req;
+		if(!q->final_count || rb.k < q->counted_rounds)
+		{
+			ServeRoundSums& m = q->mb[rb.k];
+			m.h.assign(S->h_mb_hist + (size_t) b * 8 * nb, S->h_mb_hist + (size_t) (b + 1) * 8 * nb);
+			m.c.assign(S->h_mb_counts + (size_t) b * 2 * nb, S->h_mb_counts + (size_t) (b + 1) * 2 * nb);
+		}
+		q->rounds_out--;
+		rb.req = nullptr;
+		S->free_rounds.push_back(b);
+		k++;
+	}
+	return k;
+}
+
+/* Replay of the request's summaries in id order, the counted rounds' sums
+ * added in round order. */
+void serve_replay(dms_server* S, ServeRequest* q)
+{
+	dms_job* job = q->job;
+	const double elapsed_sec = (now_ms() - q->t_start) * 1e-3;
+	while(!q->final_count)
+	{
+		if(!q->R.open_round())
+		{
+			q->final_count = true;
+			break;
+		}
+		if(S->want_rows)
+			q->counted_rounds = std::max<uint64_t>(q->counted_rounds, q->n_acc / q->interval + 1);
+		if(q->have.empty() || !q->have.front())
+			break;
+		account_trajectory(job, q->R, q->sums.front(), elapsed_sec);
+		q->R.close_one();
+		q->sums.pop_front();
+		q->have.pop_front();
+		q->n_acc++;
+	}
+	if(q->final_count && !q->cancelled)
+	{
+		q->cancelled = true;
+		__atomic_store_n(&S->h_reqs[q->slot].cancel, 1u, __ATOMIC_RELAXED);
+		q->sums.clear();
+		q->have.clear();
+		/* sums of rounds past the counted ones are not needed */
+		for(auto it = q->mb.begin(); it != q->mb.end();)
+			it = (it->first >= q->counted_rounds) ? q->mb.erase(it) : std::next(it);
+	}
+	const size_t nb = (size_t) S->nb;
+	while(q->next_add < q->counted_rounds)
+	{
+		auto it = q->mb.find(q->next_add);
+		if(it == q->mb.end())
+			break;
+		for(size_t i = 0; i < 8 * nb; i++)
+			job->hist[i] += it->second.h[i];
+		for(size_t i = 0; i < 2 * nb; i++)
+			job->counts[i] += it->second.c[i];
+		q->mb.erase(it);
+		q->next_add++;
+	}
+	job->done_traj = job->C.trajectories;
+	job->done_captured = job->C.captured;
+	job->done_scat = job->C.total_scatterings;
+	job->done_escaped = job->C.completed_outward_escapes;
+	job->done_failed = job->C.numerical_failures;
+}
+
+bool serve_request_done(const dms_server* S, const ServeRequest* q)
+{
+	return q->final_count && q->in_flight == 0 && q->rounds_out == 0 && (!S->want_rows || q->next_add >= q->counted_rounds);
+}
+
+void serve_fail_all(dms_server* S, const std::string& err)
+{
+	for(ServeRequest* q : S->active)
+	{
+		serve_finish_job(S, q->job, DMS_ERR_DEVICE, err);
+		delete q;
+	}
+	S->active.clear();
+	S->n_active = 0;
+	std::deque<dms_job*> rest;
+	{
+		std::lock_guard<std::mutex> lock(S->mu);
+		rest.swap(S->pending);
+	}
+	for(dms_job* j : rest)
+		serve_finish_job(S, j, DMS_ERR_DEVICE, err);
+}
+
+void serve_driver(dms_server* S)
+{
+	hipSetDevice(S->ctx->device);
+	bool broken = false;
+	std::string broken_err;
+	for(;;)
+	{
+		/* new requests */
+		std::deque<dms_job*> arrivals;
+		bool closing;
+		{
+			std::unique_lock<std::mutex> lock(S->mu);
+			if(S->pending.empty() && S->active.empty() && !S->running)
+				S->cv.wait(lock, [S] { return S->closing || !S->pending.empty(); });
+			closing = S->closing;
+			while(!S->pending.empty() && !S->free_reqs.empty() && arrivals.size() < S->free_reqs.size())
+			{
+				arrivals.push_back(S->pending.front());
+				S->pending.pop_front();
+			}
+		}
+		if(broken)
+		{
+			for(dms_job* j : arrivals)
+				serve_finish_job(S, j, DMS_ERR_DEVICE, broken_err);
+			if(closing)
+				break;
+			continue;
+		}
+		for(dms_job* j : arrivals)
+			serve_admit(S, j);
+		if(closing && S->active.empty() && arrivals.empty())
+		{
+			bool more;
+			{
+				std::lock_guard<std::mutex> lock(S->mu);
+				more = !S->pending.empty();
+			}
+			if(!more)
+				break;
+		}
+		std::string err;
+		if(!S->active.empty() && !S->running && !serve_launch(S, err))
+		{
+			broken = true;
+			broken_err = err;
+			serve_fail_all(S, err);
+			continue;
+		}
+		uint64_t work = 0;
+		work += serve_drain_cq(S);
+		if(S->want_rows)
+			work += serve_drain_rounds(S);
+		for(size_t i = 0; i < S->active.size();)
+		{
+			ServeRequest* q = S->active[i];
+			serve_replay(S, q);
+			if(serve_request_done(S, q))
+			{
+				dms_job* job = q->job;
+				q->R.finish();
+				job->C.simulated_trajectories = q->simulated;
+				job->C.computing_time_sec = (now_ms() - q->t_start) * 1e-3;
+				job->kernel_ms = job->C.computing_time_sec * 1e3;
+				job->launches = 0.0;
+				job->other_ms = 0.0;
+				S->free_reqs.push_back(q->slot);
+				S->active.erase(S->active.begin() + (long) i);
+				S->n_active = S->active.size();
+				delete q;
+				serve_finish_job(S, job, DMS_OK, "");
+				work++;
+				continue;
+			}
+			i++;
+		}
+		work += serve_publish(S);
+		const double t = now_ms();
+		if(S->running)
+		{
+			if(!S->active.empty())
+				S->idle_since = -1.0;
+			else if(S->idle_since < 0.0)
+				S->idle_since = t;
+			bool pending_empty;
+			{
+				std::lock_guard<std::mutex> lock(S->mu);
+				pending_empty = S->pending.empty();
+			}
+			/* park after 2 ms without requests (at once when closing) */
+			if(S->active.empty() && pending_empty && (closing || t - S->idle_since > 2.0))
+			{
+				if(!serve_park(S, err))
+				{
+					broken = true;
+					broken_err = err;
+					serve_fail_all(S, err);
+				}
+				continue;
+			}
+			/* a launch that ended by itself, or faulted */
+			if(t - S->last_check > 10.0)
+			{
+				S->last_check = t;
+				const hipError_t e = hipEventQuery(S->ended);
+				if(e != hipErrorNotReady)
+				{
+					broken = true;
+					broken_err = e == hipSuccess ? std::string("serving launch ended unexpectedly")
+					                             : std::string("serving launch: ") + hipGetErrorString(e);
+					S->running = false;
+					S->is_running = 0;
+					serve_fail_all(S, broken_err);
+					continue;
+				}
+			}
+		}
+		if(work == 0)
+			std::this_thread::sleep_for(std::chrono::microseconds(100));
+	}
+	if(S->running)
+	{
+		std::string err;
+		serve_park(S, err);
+	}
 }
 
 } // namespace
@@ -1219,7 +1887,7 @@ int dms_create(dms_ctx** out, int device, const double* raw_rows, size_t n_rows)
 		if(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess
 		   || hipStreamCreateWithFlags(&ctx->progress_stream, hipStreamNonBlocking) != hipSuccess)
 			return cleanup("hipStreamCreate failed");
-		for(int k = 0; k < 2 * DMS_CONCURRENT_JOBS; k++)
+		for(int k = 0; k < DMS_CONCURRENT_JOBS; k++)
 		{
 			hipStream_t sk = nullptr;
 			if(hipStreamCreateWithFlags(&sk, hipStreamNonBlocking) != hipSuccess)
@@ -1240,6 +1908,9 @@ int dms_create(dms_ctx** out, int device, const double* raw_rows, size_t n_rows)
 		   || hipFuncSetAttribute((const void*) dms_traj_kernel_mixed, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds) != hipSuccess
 		   || hipFuncSetAttribute((const void*) dms_traj_kernel_multi, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds) != hipSuccess
 		   || hipFuncSetAttribute((const void*) dms_traj_kernel_mixed_multi, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds)
+		          != hipSuccess
+		   || hipFuncSetAttribute((const void*) dms_traj_kernel_serve, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds) != hipSuccess
+		   || hipFuncSetAttribute((const void*) dms_traj_kernel_mixed_serve, hipFuncAttributeMaxDynamicSharedMemorySize, (int) lds)
 		          != hipSuccess)
 			return cleanup("hipFuncSetAttribute(MaxDynamicSharedMemorySize) failed");
 		int blocks = 0;
@@ -1401,25 +2072,34 @@ int dms_point_create(dms_ctx* ctx, const dms_point_desc* desc, dms_point** out)
 	pt->desc.speed_points = pt->Nv;
 	std::lock_guard<std::mutex> lock(ctx->mu);
 	hipSetDevice(ctx->device); /* callers may be on any host thread */
+	/* Everything here is ordered on the context stream (stream-ordered
+	 * allocations, asynchronous copies): no call waits for the null stream or
+	 * the whole device, so points can be made while a resident serving launch
+	 * runs on another stream. */
+	const hipStream_t st = ctx->stream;
 	auto bail = [&](const std::string& what) {
-		hipFree(pt->d_sigmaN);
-		hipFree(pt->d_grid);
-		hipFree(pt->d_grid32);
+		dfree(pt->d_sigmaN, st);
+		dfree(pt->d_grid, st);
+		dfree(pt->d_grid32, st);
+		hipStreamSynchronize(st);
 		delete pt;
 		return fail(ctx, DMS_ERR_DEVICE, "dms_point_create: " + what);
 	};
-	if(dev_alloc_copy(&pt->d_sigmaN, pt->P.sigma_N, DMS_MAX_TARGETS) != hipSuccess)
+	if(dalloc(&pt->d_sigmaN, DMS_MAX_TARGETS, st) != hipSuccess
+	   || hipMemcpyAsync(pt->d_sigmaN, pt->P.sigma_N, sizeof(double) * DMS_MAX_TARGETS, hipMemcpyHostToDevice, st) != hipSuccess
+	   || hipStreamSynchronize(st) != hipSuccess)
 		return bail("sigma upload failed");
 	pt->dev = host_point_view(pt);
 	pt->dev.sigma_N = pt->d_sigmaN;
 	if(grid)
 	{
 		const size_t cells = (size_t) pt->Nr * (size_t) pt->Nv;
-		if(hipMalloc((void**) &pt->d_grid, sizeof(double) * cells) != hipSuccess)
+		if(dalloc(&pt->d_grid, cells, st) != hipSuccess)
 			return bail("grid allocation failed");
 		if(desc->host_rate_grid != nullptr)
 		{
-			if(hipMemcpy(pt->d_grid, desc->host_rate_grid, sizeof(double) * cells, hipMemcpyHostToDevice) != hipSuccess)
+			if(hipMemcpyAsync(pt->d_grid, desc->host_rate_grid, sizeof(double) * cells, hipMemcpyHostToDevice, st) != hipSuccess
+			   || hipStreamSynchronize(st) != hipSuccess)
 				return bail("grid upload failed");
 		}
 		else
@@ -1432,7 +2112,7 @@ int dms_point_create(dms_ctx* ctx, const dms_point_desc* desc, dms_point** out)
 		}
 		pt->dev.grid = pt->d_grid;
 		/* FP32 copy (1/s) for the mixed-precision kernel */
-		if(hipMalloc((void**) &pt->d_grid32, sizeof(float) * cells) != hipSuccess)
+		if(dalloc(&pt->d_grid32, cells, st) != hipSuccess)
 			return bail("FP32 grid allocation failed");
 		(void) hipGetLastError();
 		hipLaunchKernelGGL(dms_grid_f32_kernel, dim3((unsigned) ((cells + 255) / 256)), dim3(256), 0, ctx->stream, pt->d_grid, cells,
@@ -1449,10 +2129,13 @@ void dms_point_destroy(dms_point* pt)
 {
 	if(pt == nullptr)
 		return;
+	/* stream-ordered frees: no wait for launches on other streams */
 	hipSetDevice(pt->ctx->device);
-	hipFree(pt->d_sigmaN);
-	hipFree(pt->d_grid);
-	hipFree(pt->d_grid32);
+	std::lock_guard<std::mutex> lock(pt->ctx->mu);
+	dfree(pt->d_sigmaN, pt->ctx->stream);
+	dfree(pt->d_grid, pt->ctx->stream);
+	dfree(pt->d_grid32, pt->ctx->stream);
+	hipStreamSynchronize(pt->ctx->stream);
 	delete pt;
 }
 
@@ -1465,8 +2148,10 @@ int dms_point_rate_grid(const dms_point* pt, double* out, size_t n)
 	const size_t cells = (size_t) pt->Nr * (size_t) pt->Nv;
 	if(n < cells)
 		return fail(pt->ctx, DMS_ERR_ARG, "dms_point_rate_grid: output too small");
+	std::lock_guard<std::mutex> lock(pt->ctx->mu);
 	hipSetDevice(pt->ctx->device);
-	DMS_HIP(pt->ctx, hipMemcpy(out, pt->d_grid, sizeof(double) * cells, hipMemcpyDeviceToHost));
+	DMS_HIP(pt->ctx, hipMemcpyAsync(out, pt->d_grid, sizeof(double) * cells, hipMemcpyDeviceToHost, pt->ctx->stream));
+	DMS_HIP(pt->ctx, hipStreamSynchronize(pt->ctx->stream));
 	return DMS_OK;
 }
 
@@ -1503,23 +2188,32 @@ int dms_point_rates(const dms_point* pt, int which, const double* r, const doubl
 	dms_ctx* ctx = pt->ctx;
 	std::lock_guard<std::mutex> lock(ctx->mu);
 	hipSetDevice(ctx->device); /* callers may be on any host thread */
+	const hipStream_t st = ctx->stream;
 	double *dr = nullptr, *dv = nullptr, *dout = nullptr;
-	DMS_HIP(ctx, hipMalloc((void**) &dr, sizeof(double) * n));
-	DMS_HIP(ctx, hipMalloc((void**) &dv, sizeof(double) * n));
-	DMS_HIP(ctx, hipMalloc((void**) &dout, sizeof(double) * n));
-	hipMemcpy(dr, r, sizeof(double) * n, hipMemcpyHostToDevice);
-	hipMemcpy(dv, v, sizeof(double) * n, hipMemcpyHostToDevice);
-	(void) hipGetLastError();
-	hipLaunchKernelGGL(dms_rates_kernel, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, ctx->stream, pt->dev, ctx->dev, which, dr, dv,
-	                   dout, n);
-	hipError_t e = hipGetLastError();
+	hipError_t e = dalloc(&dr, n, st);
 	if(e == hipSuccess)
-		e = hipStreamSynchronize(ctx->stream);
+		e = dalloc(&dv, n, st);
 	if(e == hipSuccess)
-		e = hipMemcpy(out, dout, sizeof(double) * n, hipMemcpyDeviceToHost);
-	hipFree(dr);
-	hipFree(dv);
-	hipFree(dout);
+		e = dalloc(&dout, n, st);
+	if(e == hipSuccess)
+		e = hipMemcpyAsync(dr, r, sizeof(double) * n, hipMemcpyHostToDevice, st);
+	if(e == hipSuccess)
+		e = hipMemcpyAsync(dv, v, sizeof(double) * n, hipMemcpyHostToDevice, st);
+	if(e == hipSuccess)
+	{
+		(void) hipGetLastError();
+		hipLaunchKernelGGL(dms_rates_kernel, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, st, pt->dev, ctx->dev, which, dr, dv,
+		                   dout, n);
+		e = hipGetLastError();
+	}
+	if(e == hipSuccess)
+		e = hipMemcpyAsync(out, dout, sizeof(double) * n, hipMemcpyDeviceToHost, st);
+	dfree(dr, st);
+	dfree(dv, st);
+	dfree(dout, st);
+	const hipError_t es = hipStreamSynchronize(st);
+	if(e == hipSuccess)
+		e = es;
 	if(e != hipSuccess)
 		return fail(ctx, DMS_ERR_DEVICE, std::string("dms_point_rates: ") + hipGetErrorString(e));
 	return DMS_OK;
@@ -1542,23 +2236,29 @@ int dms_initial_conditions(dms_ctx* ctx, const dms_halo* halo, uint32_t seed, ui
 	std::lock_guard<std::mutex> lock(ctx->mu);
 	hipSetDevice(ctx->device); /* callers may be on any host thread */
 	const DmsHaloView H = halo_view(ctx, *halo);
+	const hipStream_t st = ctx->stream;
 	dms_event* d_ev = nullptr;
 	int32_t* d_ok = nullptr;
-	DMS_HIP(ctx, hipMalloc((void**) &d_ev, sizeof(dms_event) * n));
-	DMS_HIP(ctx, hipMalloc((void**) &d_ok, sizeof(int32_t) * n));
+	hipError_t e = dalloc(&d_ev, n, st);
+	if(e == hipSuccess)
+		e = dalloc(&d_ok, n, st);
 	const double R = (r_final_rsun > 0.0) ? r_final_rsun * DMS_rSun : 0.0;
-	(void) hipGetLastError();
-	hipLaunchKernelGGL(dms_ic_kernel, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, ctx->stream, H, seed, point_id, first_id,
-	                   (uint32_t) n, R, d_ev, d_ok);
-	hipError_t e = hipGetLastError();
 	if(e == hipSuccess)
-		e = hipStreamSynchronize(ctx->stream);
+	{
+		(void) hipGetLastError();
+		hipLaunchKernelGGL(dms_ic_kernel, dim3((unsigned) ((n + 255) / 256)), dim3(256), 0, st, H, seed, point_id, first_id,
+		                   (uint32_t) n, R, d_ev, d_ok);
+		e = hipGetLastError();
+	}
 	if(e == hipSuccess)
-		e = hipMemcpy(out, d_ev, sizeof(dms_event) * n, hipMemcpyDeviceToHost);
+		e = hipMemcpyAsync(out, d_ev, sizeof(dms_event) * n, hipMemcpyDeviceToHost, st);
 	if(e == hipSuccess && shift_ok)
-		e = hipMemcpy(shift_ok, d_ok, sizeof(int32_t) * n, hipMemcpyDeviceToHost);
-	hipFree(d_ev);
-	hipFree(d_ok);
+		e = hipMemcpyAsync(shift_ok, d_ok, sizeof(int32_t) * n, hipMemcpyDeviceToHost, st);
+	dfree(d_ev, st);
+	dfree(d_ok, st);
+	const hipError_t es = hipStreamSynchronize(st);
+	if(e == hipSuccess)
+		e = es;
 	if(e != hipSuccess)
 		return fail(ctx, DMS_ERR_DEVICE, std::string("dms_initial_conditions: ") + hipGetErrorString(e));
 	return DMS_OK;
@@ -1711,6 +2411,12 @@ void start_batch(dms_point* const* pts, const dms_job_desc* descs, size_t n, dms
 
 void join_batch(dms_job* job)
 {
+	if(job->server != nullptr)
+	{
+		std::unique_lock<std::mutex> lock(job->fmu);
+		job->fcv.wait(lock, [job] { return job->finished.load() != 0; });
+		return;
+	}
 	dms_batch* b = job->batch.get();
 	if(b == nullptr)
 		return;
@@ -1849,6 +2555,160 @@ int dms_multi_autotune(dms_point* const* points, const dms_job_desc* descs, size
 			cost_per_trajectory[k] = cost[k];
 	}
 	return DMS_OK;
+}
+
+int dms_server_create(dms_ctx* ctx, const dms_server_desc* desc, dms_server** out)
+{
+	if(ctx == nullptr || desc == nullptr || out == nullptr)
+		return fail(ctx, DMS_ERR_ARG, "dms_server_create: null argument");
+	*out = nullptr;
+	if(ctx->device < 0)
+		return fail(ctx, DMS_ERR_STATE, "dms_server_create: context has no device");
+	const std::string m = check_job(desc->launch);
+	if(!m.empty())
+		return fail(ctx, DMS_ERR_ARG, "dms_server_create: " + m);
+	const uint32_t ring = desc->slot_ring ? desc->slot_ring : (1u << 20);
+	if(ring < 1024 || ring > (1u << 26) || (ring & (ring - 1)) != 0)
+		return fail(ctx, DMS_ERR_ARG, "dms_server_create: slot_ring must be a power of two in [1024, 2^26]");
+	const uint32_t max_reqs = desc->max_requests ? desc->max_requests : 1024;
+	if(max_reqs > 65536)
+		return fail(ctx, DMS_ERR_ARG, "dms_server_create: max_requests must be at most 65536");
+	dms_server* S = new dms_server();
+	S->ctx = ctx;
+	S->cfg = desc->launch;
+	S->J = job_view(desc->launch);
+	S->want_rows = desc->launch.mode != DMS_MODE_CAPTURE;
+	S->nb = S->J.nbins;
+	if(S->want_rows && S->nb > 65536)
+	{
+		delete S;
+		return fail(ctx, DMS_ERR_ARG, "dms_server_create: at most 65536 bins");
+	}
+	S->ring = ring;
+	S->mask = ring - 1;
+	S->max_reqs = max_reqs;
+	uint32_t max_rounds = 0;
+	if(S->want_rows)
+	{
+		const size_t per_round = (size_t) 80 * (size_t) S->nb;
+		max_rounds = desc->max_rounds ? desc->max_rounds
+		                              : (uint32_t) std::min<size_t>(4096, std::max<size_t>(64, ((size_t) 256 << 20) / per_round));
+		if(max_rounds > 65536)
+		{
+			delete S;
+			return fail(ctx, DMS_ERR_ARG, "dms_server_create: max_rounds must be at most 65536");
+		}
+	}
+	S->max_rounds = max_rounds;
+	S->grid = traj_grid(ctx, (size_t) 1 << 40);
+	S->lanes = (size_t) S->grid * DMS_TRAJ_BLOCK;
+	hipSetDevice(ctx->device);
+	const size_t nb = (size_t) S->nb;
+	const size_t R = max_rounds;
+	bool ok = host_alloc(&S->h_ctl, 16) == hipSuccess && host_alloc(&S->h_slots, ring) == hipSuccess
+	          && host_alloc(&S->h_reqs, max_reqs) == hipSuccess && host_alloc(&S->h_results, ring) == hipSuccess
+	          && host_alloc(&S->h_cq, ring) == hipSuccess && host_alloc(&S->h_round_size, R) == hipSuccess
+	          && host_alloc(&S->h_round_gen, R) == hipSuccess && host_alloc(&S->h_round_stamp, R) == hipSuccess;
+	if(ok && S->want_rows)
+		ok = host_alloc(&S->h_mb_hist, R * 8 * nb) == hipSuccess && host_alloc(&S->h_mb_counts, R * 2 * nb) == hipSuccess;
+	/* device buffers: stream-ordered on the context stream, so that making a
+	 * server never waits for launches on other streams */
+	const hipStream_t cs = ctx->stream;
+	std::unique_lock<std::mutex> ctx_lock(ctx->mu);
+	if(ok)
+		ok = dalloc(&S->d_counters, DMS_SERVE_COUNTERS, cs) == hipSuccess && dalloc(&S->d_round_done, R, cs) == hipSuccess
+		     && hipMemsetAsync(S->d_round_done, 0, sizeof(uint32_t) * (R ? R : 1), cs) == hipSuccess;
+	if(ok && S->want_rows)
+		ok = dalloc(&S->d_round_hist, R * 8 * nb, cs) == hipSuccess && dalloc(&S->d_round_counts, R * 2 * nb, cs) == hipSuccess
+		     && dalloc(&S->d_rows, S->lanes * 2 * nb, cs) == hipSuccess && dalloc(&S->d_counts, S->lanes * nb, cs) == hipSuccess
+		     && dalloc(&S->d_touched, S->lanes * nb, cs) == hipSuccess
+		     && hipMemsetAsync(S->d_round_hist, 0, sizeof(double) * R * 8 * nb, cs) == hipSuccess
+		     && hipMemsetAsync(S->d_round_counts, 0, sizeof(unsigned long long) * R * 2 * nb, cs) == hipSuccess
+		     && hipMemsetAsync(S->d_rows, 0, sizeof(double) * S->lanes * 2 * nb, cs) == hipSuccess
+		     && hipMemsetAsync(S->d_counts, 0, sizeof(uint32_t) * S->lanes * nb, cs) == hipSuccess;
+	if(ok)
+		ok = hipEventCreateWithFlags(&S->ended, hipEventDisableTiming) == hipSuccess && hipStreamSynchronize(cs) == hipSuccess;
+	ctx_lock.unlock();
+	if(!ok)
+	{
+		serve_release(S);
+		delete S;
+		return fail(ctx, DMS_ERR_NOMEM, "dms_server_create: host or device allocation failed");
+	}
+	S->pos_req.assign(ring, nullptr);
+	S->pos_off.assign(ring, 0);
+	S->rounds.assign(R, ServeRoundBuf());
+	for(uint32_t b = 0; b < max_rounds; b++)
+		S->free_rounds.push_back(max_rounds - 1 - b);
+	for(uint32_t q = 0; q < max_reqs; q++)
+		S->free_reqs.push_back(max_reqs - 1 - q);
+	S->driver = std::thread(serve_driver, S);
+	*out = S;
+	return DMS_OK;
+}
+
+int dms_server_submit(dms_server* S, dms_point* pt, const dms_job_desc* desc, dms_job** out)
+{
+	if(S == nullptr || pt == nullptr || desc == nullptr || out == nullptr)
+		return fail(S ? S->ctx : nullptr, DMS_ERR_ARG, "dms_server_submit: null argument");
+	*out = nullptr;
+	if(pt->ctx != S->ctx)
+		return fail(S->ctx, DMS_ERR_ARG, "dms_server_submit: point of another context");
+	const std::string m = check_submit(*desc);
+	if(!m.empty())
+		return fail(S->ctx, DMS_ERR_ARG, "dms_server_submit: " + m);
+	if(!same_launch_config(S->cfg, *desc))
+		return fail(S->ctx, DMS_ERR_ARG,
+		            "dms_server_submit: mode, precision, limits, r_max or n_bins differ from the server's launch descriptor");
+	if(desc->record_trajectories)
+		return fail(S->ctx, DMS_ERR_UNSUPPORTED, "dms_server_submit: served requests do not keep trajectory records");
+	dms_job* job = new dms_job();
+	job->pt = pt;
+	job->desc = *desc;
+	job->server = S;
+	memset(&job->C, 0, sizeof(job->C));
+	{
+		std::lock_guard<std::mutex> lock(S->mu);
+		if(S->closing)
+		{
+			delete job;
+			return fail(S->ctx, DMS_ERR_STATE, "dms_server_submit: server is being destroyed");
+		}
+		S->pending.push_back(job);
+	}
+	S->cv.notify_one();
+	*out = job;
+	return DMS_OK;
+}
+
+int dms_server_get_stats(const dms_server* S, dms_server_stats* st)
+{
+	if(S == nullptr || st == nullptr)
+		return fail(nullptr, DMS_ERR_ARG, "dms_server_get_stats: null argument");
+	memset(st, 0, sizeof(*st));
+	st->launches = S->n_launches.load();
+	st->slots_published = S->n_published.load();
+	st->slots_cancelled = S->n_cancelled.load();
+	st->requests_done = S->n_done.load();
+	st->requests_active = (uint32_t) S->n_active.load();
+	st->running = (uint32_t) S->is_running.load();
+	return DMS_OK;
+}
+
+void dms_server_destroy(dms_server* S)
+{
+	if(S == nullptr)
+		return;
+	{
+		std::lock_guard<std::mutex> lock(S->mu);
+		S->closing = true;
+	}
+	S->cv.notify_one();
+	if(S->driver.joinable())
+		S->driver.join();
+	hipSetDevice(S->ctx->device);
+	serve_release(S);
+	delete S;
 }
 
 int dms_job_poll(dms_job* job, dms_job_status* st)

@@ -100,6 +100,16 @@ class EvapEvent(C.Structure):
                 ("completion_wall_time_sec", C.c_double), ("lifetime_unbinding_sec", C.c_double)]
 
 
+class ServerDesc(C.Structure):
+    _fields_ = [("launch", JobDesc), ("slot_ring", C.c_uint32), ("max_requests", C.c_uint32),
+                ("max_rounds", C.c_uint32), ("pad", C.c_uint32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("launches", C.c_uint64), ("slots_published", C.c_uint64), ("slots_cancelled", C.c_uint64),
+                ("requests_done", C.c_uint64), ("requests_active", C.c_uint32), ("running", C.c_uint32)]
+
+
 class JobStatus(C.Structure):
     _fields_ = [("trajectories_done", C.c_uint64), ("captured", C.c_uint64), ("scatterings", C.c_uint64),
                 ("done", C.c_int32), ("pad", C.c_int32), ("completed_outward_escapes", C.c_uint64),
@@ -214,6 +224,10 @@ def _declare(lib):
         "dms_job_submit_multi": (C.c_int, [_P(vp), _P(JobDesc), C.c_size_t, _P(vp)]),
         "dms_multi_autotune": (C.c_int, [_P(vp), _P(JobDesc), C.c_size_t, C.c_uint64, C.c_uint64, _P(C.c_uint32),
                                          _P(C.c_double)]),
+        "dms_server_create": (C.c_int, [vp, _P(ServerDesc), _P(vp)]),
+        "dms_server_submit": (C.c_int, [vp, vp, _P(JobDesc), _P(vp)]),
+        "dms_server_get_stats": (C.c_int, [vp, _P(ServerStats)]),
+        "dms_server_destroy": (None, [vp]),
         "dms_job_poll": (C.c_int, [vp, _P(JobStatus)]),
         "dms_job_wait": (C.c_int, [vp, _P(JobStatus)]),
         "dms_job_read": (C.c_int, [vp, _P(Counters), _P(C.c_double), _P(C.c_uint64), C.c_size_t,
@@ -234,7 +248,7 @@ def _declare(lib):
 EXPORTED = ["dms_abi_version", "dms_create", "dms_destroy", "dms_last_error", "dms_set_job_memory_budget", "dms_load_solar_file",
             "dms_solar_eval", "dms_isotopes", "dms_constant", "dms_point_create", "dms_point_destroy",
             "dms_point_rate_grid", "dms_point_grid_shape", "dms_point_cross_sections", "dms_point_rates", "dms_initial_conditions",
-            "dms_simulate_events", "dms_simulate_events_noalloc", "dms_job_submit", "dms_job_submit_batch", "dms_job_submit_multi", "dms_multi_autotune", "dms_job_poll", "dms_job_wait", "dms_job_read",
+            "dms_simulate_events", "dms_simulate_events_noalloc", "dms_job_submit", "dms_job_submit_batch", "dms_job_submit_multi", "dms_multi_autotune", "dms_server_create", "dms_server_submit", "dms_server_get_stats", "dms_server_destroy", "dms_job_poll", "dms_job_wait", "dms_job_read",
             "dms_job_timing", "dms_job_destroy", "dms_probe_math"]
 
 _lib = None
@@ -357,6 +371,58 @@ def run_multi_tuned(points, jobs, want_records=False, **calibration):
     for pos, k in enumerate(order):
         out[k] = res[pos]
     return out
+
+
+class Server:
+    """dms_server: one resident trajectory launch that requests join as slots
+    while it runs.  launch: a JobDesc whose mode, precision, limits, r_max and
+    n_bins every request shares."""
+
+    def __init__(self, ctx, launch, slot_ring=0, max_requests=0, max_rounds=0):
+        self.ctx = ctx
+        d = ServerDesc()
+        d.launch = launch
+        d.slot_ring = slot_ring
+        d.max_requests = max_requests
+        d.max_rounds = max_rounds
+        h = C.c_void_p()
+        ctx.check(lib().dms_server_create(ctx.h, C.byref(d), C.byref(h)), "dms_server_create")
+        self.h = h
+
+    def submit(self, point, job):
+        """Queue one request; returns a handle for collect()."""
+        h = C.c_void_p()
+        self.ctx.check(lib().dms_server_submit(self.h, point.h, C.byref(job), C.byref(h)), "dms_server_submit")
+        return (h, job, point)
+
+    def collect(self, handle, want_records=False):
+        h, job, point = handle
+        try:
+            return point._collect(h, job, False)
+        finally:
+            lib().dms_job_destroy(h)
+
+    def poll(self, handle):
+        st = JobStatus()
+        self.ctx.check(lib().dms_job_poll(handle[0], C.byref(st)), "dms_job_poll")
+        return st
+
+    def stats(self):
+        st = ServerStats()
+        self.ctx.check(lib().dms_server_get_stats(self.h, C.byref(st)), "dms_server_get_stats")
+        return dict(launches=st.launches, slots_published=st.slots_published, slots_cancelled=st.slots_cancelled,
+                    requests_done=st.requests_done, requests_active=st.requests_active, running=st.running)
+
+    def close(self):
+        if self.h:
+            lib().dms_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Point:

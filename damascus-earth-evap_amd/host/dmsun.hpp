@@ -359,10 +359,13 @@ struct Request_Result
 class Server
 {
   public:
-	/* memory_budget: device bytes the running requests may hold together;
-	 * per_job_budget: bytes one request's trajectory buffers may use */
+	/* Requests join one resident trajectory launch per launch configuration
+	 * (dms_server): max_concurrent requests at once; memory_budget: device
+	 * bytes the resident parameter points' rate tables may hold together
+	 * (unused points are evicted); per_job_budget: the context's memory budget
+	 * for jobs run outside the server (dms_set_job_memory_budget). */
 	Server(Solar_Model& model, size_t memory_budget = (size_t) 128 << 30, size_t per_job_budget = (size_t) 8 << 30,
-	       int max_concurrent = 8);
+	       int max_concurrent = 64);
 	~Server();
 	Server(const Server&) = delete;
 	Server& operator=(const Server&) = delete;
@@ -371,6 +374,8 @@ class Server
 	void Wait_All();
 	std::vector<Request_Result> Results() const; /* finished requests, by id */
 	double Elapsed() const;
+	/* counters of the resident launches, summed over launch configurations */
+	dms_server_stats Stats() const;
 
   private:
 	struct Impl;

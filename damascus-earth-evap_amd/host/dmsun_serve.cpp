@@ -1,6 +1,7 @@
 /*
  * dmsun_serve: a request stream of parameter points served with continuous
- * batching on one MI355X.
+ * batching on one MI355X: every request's trajectories join one resident
+ * trajectory launch as slots (dmsun::Server over dms_server).
  *
  *   dmsun_serve [--requests N] [--rate R] [--sample-size S] [--sample-size-max S2]
  *               [--max-trajectories M] [--max-scatterings K] [--rk-budget B]
@@ -16,7 +17,9 @@
  * trajectory (heavy captured particles do not evaporate: K bounds them).
  * Latency is completion minus arrival.  --check-alone K reruns the first K
  * requests one at a time after the mix and compares their results with the
- * ones they got while sharing the GPU.  One JSON line summarises the run;
+ * ones they got while sharing the GPU.  One JSON line summarises the run
+ * (with the resident launch's counters: launches started, slots published,
+ * slots handed back unstarted after their request had stopped);
  * --results writes one JSON line per request: its point and job descriptor,
  * counters, visit counts, histogram sums and evaporation events.
  */
@@ -43,7 +46,7 @@ struct Options
 	uint64_t sample_size = 20, sample_size_max = 0, max_trajectories = 200000;
 	uint64_t max_scatterings = 100000, rk_budget = 0;
 	int precision = DMS_FP64;
-	int concurrency = 8;
+	int concurrency = 64;
 	double memory_budget_gib = 128.0, per_job_budget_gib = 8.0;
 	unsigned int seed = 20260710;
 	unsigned int interp = 1000;
@@ -241,6 +244,7 @@ int main(int argc, char** argv)
 
 		std::vector<dmsun::Request_Result> mix;
 		double elapsed = 0.0;
+		dms_server_stats sst{};
 		{
 			dmsun::Server server(SSM, (size_t) (o.memory_budget_gib * gib), (size_t) (o.per_job_budget_gib * gib), o.concurrency);
 			const auto t0 = std::chrono::steady_clock::now();
@@ -267,10 +271,11 @@ int main(int argc, char** argv)
 			server.Wait_All();
 			elapsed = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 			mix = server.Results();
+			sst = server.Stats();
 		}
 
 		int failed = 0;
-		uint64_t traj = 0, captured = 0;
+		uint64_t traj = 0, captured = 0, simulated = 0;
 		std::vector<double> lat, wait;
 		for(const auto& r : mix)
 		{
@@ -284,6 +289,7 @@ int main(int argc, char** argv)
 			wait.push_back(r.start_sec - r.arrival_sec);
 			traj += r.counters.trajectories;
 			captured += r.counters.captured;
+			simulated += r.counters.simulated_trajectories;
 		}
 
 		int mismatches = 0;
@@ -317,7 +323,9 @@ int main(int argc, char** argv)
 		   << ", \"latency_p50_sec\": " << dmsun::Percentile(lat, 50.0) << ", \"latency_p99_sec\": " << dmsun::Percentile(lat, 99.0)
 		   << ", \"latency_max_sec\": " << dmsun::Percentile(lat, 100.0) << ", \"queue_wait_p50_sec\": " << dmsun::Percentile(wait, 50.0)
 		   << ", \"queue_wait_p99_sec\": " << dmsun::Percentile(wait, 99.0) << ", \"checked_alone\": " << o.check_alone
-		   << ", \"alone_mismatches\": " << mismatches << "}";
+		   << ", \"alone_mismatches\": " << mismatches << ", \"simulated_trajectories\": " << simulated
+		   << ", \"launches\": " << sst.launches << ", \"slots_published\": " << sst.slots_published
+		   << ", \"slots_cancelled\": " << sst.slots_cancelled << "}";
 		std::cout << js.str() << std::endl;
 		if(!o.json.empty())
 		{

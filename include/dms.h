@@ -364,6 +364,58 @@ int dms_job_read(dms_job* job, dms_counters* counters, double* hist, uint64_t* c
 int dms_job_timing(const dms_job* job, double* trajectory_kernel_ms, double* launches, double* other_ms);
 void dms_job_destroy(dms_job* job);
 
+/* ---- serving: requests join one resident launch ---------------------------- */
+
+/* A server keeps one persistent trajectory launch on the device and appends
+ * the trajectories of each submitted request to its work queue as slots while
+ * it runs (continuous batching at slot level): a request starts on the lanes
+ * that free up, without waiting for the others and without a launch of its
+ * own.  Every request's results equal those of dms_job_submit on the same
+ * point and descriptor (counters, counts and events exactly; the histogram
+ * sums to the rounding of their summation order): its trajectories are
+ * replayed in id order with the reference's round logic, as the ranks of
+ * Simulation_Data::Generate_Data take trajectories inside their sync rounds
+ * (src/Data_Generation.cpp:697-728).  With no request left the launch ends
+ * (after about 2 ms) and the next request starts it again.  While it runs the
+ * launch occupies the device's compute units, so jobs submitted outside the
+ * server meanwhile run on what is left.  No reference counterpart (one
+ * Generate_Data call per parameter point). */
+typedef struct dms_server dms_server;
+
+typedef struct
+{
+	/* mode, precision, max_scatterings, max_free_steps, rk_attempt_budget,
+	 * r_max_rsun and n_bins of every request (the other fields are ignored) */
+	dms_job_desc launch;
+	uint32_t slot_ring;    /* slots in flight at most: power of two in [1024, 2^26] (0: 2^20) */
+	uint32_t max_requests; /* requests admitted at once, others queue (0: 1024; at most 65536) */
+	uint32_t max_rounds;   /* per-round histogram buffers (0: about 256 MiB of them; at most 65536) */
+	uint32_t pad;
+} dms_server_desc;
+
+typedef struct
+{
+	uint64_t launches;        /* resident launches started: 1 + restarts after idle periods */
+	uint64_t slots_published; /* trajectory slots appended to the launch */
+	uint64_t slots_cancelled; /* slots handed back unstarted after their request had stopped */
+	uint64_t requests_done;
+	uint32_t requests_active; /* admitted and not finished */
+	uint32_t running;         /* 1 while the launch is on the device */
+} dms_server_stats;
+
+int dms_server_create(dms_ctx* ctx, const dms_server_desc* desc, dms_server** server);
+/* Queue a request; the returned job is waited on, polled, read and destroyed
+ * like any other (dms_job_wait / dms_job_poll / dms_job_read /
+ * dms_job_destroy) and stays valid after the server is destroyed.  The
+ * descriptor must agree with the server's launch descriptor in mode,
+ * precision, limits, r_max and n_bins; record_trajectories is not supported
+ * (DMS_ERR_UNSUPPORTED). */
+int dms_server_submit(dms_server* server, dms_point* point, const dms_job_desc* desc, dms_job** job);
+int dms_server_get_stats(const dms_server* server, dms_server_stats* stats);
+/* Waits for every submitted request to finish, ends the launch and frees the
+ * server. */
+void dms_server_destroy(dms_server* server);
+
 /* ---- diagnostics ----------------------------------------------------------- */
 
 /* Evaluate the deterministic elementary functions on the device for n inputs.
