@@ -76,7 +76,9 @@ static void run_one(or_batch* B, size_t i)
 		R->dE_first_negative_from_prev_eV = dm_nan();
 		return;
 	}
+	OR_OPS_PUSH(OR_OPS_TRAJECTORY);
 	const int st = or_simulate_one(cfg, &ic, B->job->seed, B->job->point_id, id, &B->rec[i], dt, v2, cn);
+	OR_OPS_POP();
 	if(st != DMS_OK)
 		B->status = st;
 }

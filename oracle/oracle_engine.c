@@ -782,6 +782,7 @@ static int propagate_freely(or_traj* T, dms_event* ev)
 
 	while(time_steps < cfg->max_time_steps && outcome == DMS_MAX_FREE_STEPS)
 	{
+		OR_OPS_STEP_BEGIN();
 		if(WALL_EXCEEDED())
 		{
 			dms_event e3;
@@ -824,7 +825,9 @@ static int propagate_freely(or_traj* T, dms_event* ev)
 		}
 		const struct or_prop prop_before = prop;
 		const double t_before = prop.time;
+		OR_OPS_PUSH(OR_OPS_RK);
 		const int ok = prop_rk45(&prop, S, 0.0);
+		OR_OPS_POP();
 		const double actual_dt = prop.time - t_before;
 		const double r_after = prop_radius(&prop);
 		const double v_after = prop_speed(&prop);
@@ -859,6 +862,7 @@ static int propagate_freely(or_traj* T, dms_event* ev)
 		if(interior_interval(&eb, &ea, r_before, r_after, &is, &ie))
 		{
 			T->rec->n_steps_interior++;
+			OR_OPS_STEP_CLASS(OR_OPS_INTERIOR);
 			if(r_after < OR_rSun && v_after < 0.0)
 				break;
 			od_piece pieces[MAX_OPTICAL_DEPTH_PIECES];
@@ -928,6 +932,7 @@ static int propagate_freely(or_traj* T, dms_event* ev)
 		}
 		else
 			T->rec->n_steps_exterior++;
+			OR_OPS_STEP_CLASS(OR_OPS_EXTERIOR);
 		if(!scattering)
 		{
 			dms_event be = ea;
@@ -1264,9 +1269,13 @@ int or_simulate_one(const or_sim_cfg* cfg, const dms_event* ic, uint32_t seed, u
 	while(n_scat < cfg->max_scatterings)
 	{
 		const int pr = propagate_freely(&T, &cur);
+		OR_OPS_STEP_END();
 		if(pr == DMS_SCATTER && or_norm(cur.r) < OR_rSun)
 		{
-			if(or_scatter_impl(P, &cur, &T.rng, &rec->n_tv_iterations) != 0)
+			OR_OPS_PUSH(OR_OPS_SCATTER);
+			const int sr = or_scatter_impl(P, &cur, &T.rng, &rec->n_tv_iterations);
+			OR_OPS_POP();
+			if(sr != 0)
 			{
 				reason = DMS_NUMERICAL_FAILURE;
 				break;

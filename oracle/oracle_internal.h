@@ -136,6 +136,17 @@ typedef struct
 	double bin_width_km;
 } or_sim_cfg;
 
+/* Attribution marks of the op-counting build (oracle/opcount/): which part of a
+ * trajectory the floating-point operations belong to.  They do nothing in the
+ * oracle itself. */
+#ifndef OR_OPCOUNT
+#define OR_OPS_PUSH(part) ((void) 0)
+#define OR_OPS_POP() ((void) 0)
+#define OR_OPS_STEP_BEGIN() ((void) 0)
+#define OR_OPS_STEP_CLASS(part) ((void) 0)
+#define OR_OPS_STEP_END() ((void) 0)
+#endif
+
 int or_simulate_one(const or_sim_cfg* cfg, const dms_event* ic, uint32_t seed, uint32_t point_id, uint64_t traj_id,
                     dms_traj_record* rec, double* dt_hist, double* v2dt_hist, uint32_t* counts);
 int or_scatter_impl(const or_point* P, dms_event* e, dm_rng* rng, uint64_t* tv_iterations);
