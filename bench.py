@@ -44,15 +44,16 @@ import time
 ROOT = pathlib.Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "damascus-earth-evap_amd"))
 
-# Algorithmic FP64 operations per event (hand count of the reference formulas:
-# +, -, *, /, sqrt = 1; each transcendental = 1).
+# Algorithmic FP64 operations per event (+, -, *, /, sqrt = 1; each
+# transcendental = 1), measured with the op-counting build of the oracle on
+# 2000 trajectories of each workload and frozen here: oracle/opcount/weights.txt
+# (the RKF45 attempt costs less outside the Sun, where Mass() is constant, so
+# the two modes' weights differ with their mix of interior and exterior steps).
 FLOP_WEIGHTS = {
-    "rk_attempt": 266.0,        # one RKF45 attempt: 6 stages, 6 Mass() splines, step control
-    "step_interior": 412.0,     # step prep + optical depth with 4 pieces (Parameter point)
-    "step_interior_capture": 281.0,  # step prep + optical depth with 2 pieces (Capture)
-    "step_exterior": 194.0,     # step prep + boundary handling outside the Sun
-    "scatter": 3000.0,          # Sample_Target (64 thermal-averaged rates) + kinematics
-    "trajectory": 150.0,        # trajectory start/finish bookkeeping
+    "parameter": {"rk_attempt": 246.2, "step_interior": 400.6, "step_exterior": 191.6, "scatter": 2518.5,
+                  "trajectory": 66.8, "segment": 136.8},
+    "capture": {"rk_attempt": 233.8, "step_interior": 308.4, "step_exterior": 178.2, "scatter": 2404.4,
+                "trajectory": 12.0, "segment": 131.1},
 }
 PEAK_FP64_TFLOPS = 78.6
 # The default timed launch (20 jobs of 10^6 trajectories, the bench's default
@@ -66,13 +67,11 @@ PROFILED_LAUNCH = {"steps": 20, "trajectories": 1_000_000, "hbm_bytes": 1413.330
 
 
 def algorithmic_flops(c, capture_mode):
-    w = FLOP_WEIGHTS
+    w = FLOP_WEIGHTS["capture" if capture_mode else "parameter"]
     segs = c.total_scatterings + c.trajectories
-    return (w["rk_attempt"] * c.n_rk_attempts
-            + (w["step_interior_capture"] if capture_mode else w["step_interior"]) * c.n_steps_interior
-            + w["step_exterior"] * c.n_steps_exterior
-            + w["scatter"] * c.total_scatterings
-            + w["trajectory"] * c.trajectories + 110.0 * segs)
+    return (w["rk_attempt"] * c.n_rk_attempts + w["step_interior"] * c.n_steps_interior
+            + w["step_exterior"] * c.n_steps_exterior + w["scatter"] * c.total_scatterings
+            + w["trajectory"] * c.trajectories + w["segment"] * segs)
 
 
 def make_workload(dmsun, n_traj, first_id, precision=0):

@@ -931,8 +931,10 @@ static int propagate_freely(or_traj* T, dms_event* ev)
 				minus_log_xi -= dtau;
 		}
 		else
+		{
 			T->rec->n_steps_exterior++;
 			OR_OPS_STEP_CLASS(OR_OPS_EXTERIOR);
+		}
 		if(!scattering)
 		{
 			dms_event be = ea;
