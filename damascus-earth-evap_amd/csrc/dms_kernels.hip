@@ -327,6 +327,7 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 	uint32_t sv_round = DMS_SERVE_NO_ROUND;
 	DmsSummary sv_sum;
 	DmsServePoll sv_poll = {0u, 0u};
+	uint32_t cancel_tick = 0;
 	for(;;)
 	{
 		dms_lane_advance_a<MX>(L, c);
@@ -554,6 +555,19 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 			} /* not a serving launch */
 		}
 		dms_lane_advance_b<MX>(L, c);
+		if constexpr(!SV)
+		{
+			/* cancellation: one read of the host's flag per wave every 1024
+			 * iterations; every lane of the launch then ends where it is */
+			if(__builtin_expect(a.cancel != nullptr, 0) && ((++cancel_tick) & 1023u) == 0u)
+			{
+				unsigned int cv = 0;
+				if(lane == 0)
+					cv = __hip_atomic_load(a.cancel, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+				if(__builtin_amdgcn_readlane((int) cv, 0) != 0)
+					L.phase = PH_DONE;
+			}
+		}
 		if(__ballot(L.phase != PH_DONE) == 0)
 			break;
 	}

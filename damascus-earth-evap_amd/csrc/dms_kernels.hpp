@@ -104,6 +104,9 @@ struct DmsTrajArgs
 	 * fill_extra, and no slot at or past it is started */
 	unsigned int* fill_ctl;
 	uint32_t cap_stop, fill_extra;
+	/* host memory mapped for the device (may be null): nonzero ends the launch;
+	 * each wave reads it every 1024 loop iterations */
+	const unsigned int* cancel;
 	/* serving launch (dms_traj_kernel_serve*) */
 	DmsServeView V;
 };

@@ -36,6 +36,8 @@
  * at the same time: with the context stream and the shared progress stream,
  * the process uses four streams, one per hardware queue of the device */
 #define DMS_CONCURRENT_JOBS 2
+/* cancel words of the context: one per running job or batch launch */
+#define DMS_CANCEL_WORDS 64
 
 struct dms_ctx
 {
@@ -64,6 +66,11 @@ struct dms_ctx
 	/* device address of this GPU's copy of the allocation-free entry point's
 	 * scratch array (hipGetSymbolAddress at set-up) */
 	unsigned char* scratch_ptr = nullptr;
+	/* cancel words of running launches: host memory mapped for the device,
+	 * allocated once (freeing pinned memory would wait for the device) */
+	unsigned int* h_cancel = nullptr;
+	unsigned int* d_cancel = nullptr;
+	std::vector<int> cancel_free;
 };
 
 struct dms_point
@@ -86,6 +93,9 @@ struct dms_batch
 	std::thread worker;
 	std::mutex mu;
 	bool joined = false;
+	/* dms_job_cancel: the request, and the launch's cancel word once it has one */
+	std::atomic<int> cancelled{0};
+	std::atomic<int> cancel_word{-1};
 };
 
 struct dms_job
@@ -102,6 +112,7 @@ struct dms_job
 	struct dms_server* server = nullptr;
 	std::mutex fmu;
 	std::condition_variable fcv;
+	std::atomic<int> cancel_requested{0};
 	int status = DMS_OK;
 	std::string err;
 	dms_counters C;
@@ -370,7 +381,8 @@ struct FillStop
 
 hipError_t run_trajectories(dms_ctx* ctx, hipStream_t st, hipStream_t pst, const dms_point* pt, const DmsJobView& J, ChunkBuffers& B,
                             size_t n, bool rows, bool recs, const LaunchJobs& LJ, const DmsRows* lane_rows, float* ms,
-                            const ProgressFn& progress = nullptr, const FillStop* stop = nullptr)
+                            const ProgressFn& progress = nullptr, const FillStop* stop = nullptr,
+                            const unsigned int* cancel = nullptr)
 {
 	DmsTrajArgs a;
 	a.S = ctx->dev;
@@ -402,6 +414,7 @@ hipError_t run_trajectories(dms_ctx* ctx, hipStream_t st, hipStream_t pst, const
 	a.fill_ctl = nullptr;
 	a.cap_stop = 0;
 	a.fill_extra = 0;
+	a.cancel = cancel;
 	hipError_t e;
 	if((e = hipMemsetAsync(B.next, 0, sizeof(unsigned int), st)) != hipSuccess)
 		return e;
@@ -458,14 +471,15 @@ hipError_t run_trajectories(dms_ctx* ctx, hipStream_t st, hipStream_t pst, const
 /* single-job launch over slots [0, n) with ids first_id .. */
 hipError_t run_trajectories_single(dms_ctx* ctx, hipStream_t st, hipStream_t pst, const dms_point* pt, const DmsJobView& J,
                                    ChunkBuffers& B, size_t n, bool rows, bool recs, uint64_t first_id, float* ms,
-                                   const ProgressFn& progress = nullptr, const FillStop* stop = nullptr)
+                                   const ProgressFn& progress = nullptr, const FillStop* stop = nullptr,
+                                   const unsigned int* cancel = nullptr)
 {
 	LaunchJobs LJ;
 	LJ.start.push_back(0);
 	LJ.first_id.push_back(first_id);
 	hipError_t e = LJ.upload(st);
 	if(e == hipSuccess)
-		e = run_trajectories(ctx, st, pst, pt, J, B, n, rows, recs, LJ, nullptr, ms, progress, stop);
+		e = run_trajectories(ctx, st, pst, pt, J, B, n, rows, recs, LJ, nullptr, ms, progress, stop, cancel);
 	LJ.release(st);
 	return e;
 }
@@ -645,7 +659,8 @@ int account_trajectory(dms_job* job, RoundReplay& R, const DmsSummary& s, double
 
 /* Chunked job: speculative id-ordered chunks with per-trajectory rows, so that
  * capture-target runs count exactly the trajectories the reference would. */
-void run_job_chunked(dms_job* job, hipStream_t st, hipStream_t pst)
+void run_job_chunked(dms_job* job, hipStream_t st, hipStream_t pst, const unsigned int* cancel = nullptr,
+                     const std::atomic<int>* cancelled = nullptr)
 {
 	dms_point* pt = job->pt;
 	dms_ctx* ctx = pt->ctx;
@@ -704,6 +719,8 @@ void run_job_chunked(dms_job* job, hipStream_t st, hipStream_t pst)
 	std::vector<dms_traj_record> hrec;
 	while(R.open_round())
 	{
+		if(cancelled && cancelled->load())
+			break;
 		uint64_t want;
 		if(R.target == UINT64_MAX)
 			want = R.max_traj - R.local_total;
@@ -757,7 +774,7 @@ void run_job_chunked(dms_job* job, hipStream_t st, hipStream_t pst)
 			                                       job->done_escaped = base_esc + d[2];
 			                                       job->done_failed = base_fail + d[3];
 		                                       },
-		                                       fill_stop ? &fs : nullptr);
+		                                       fill_stop ? &fs : nullptr, cancel);
 		if(e != hipSuccess)
 		{
 			set_err(std::string("trajectory kernel: ") + hipGetErrorString(e));
@@ -853,7 +870,8 @@ void run_job_chunked(dms_job* job, hipStream_t st, hipStream_t pst)
  * streamed through a single persistent launch: lanes refill across job
  * boundaries, so the long tail of one job overlaps the bulk of the others.
  * Histograms use per-lane scratch rows flushed into per-job sums. */
-void run_batch_fixed(std::vector<dms_job*>& jobs, hipStream_t st, hipStream_t pst)
+void run_batch_fixed(std::vector<dms_job*>& jobs, hipStream_t st, hipStream_t pst, const unsigned int* cancel = nullptr,
+                     const std::atomic<int>* cancelled = nullptr)
 {
 	dms_point* pt = jobs[0]->pt;
 	dms_ctx* ctx = pt->ctx;
@@ -937,7 +955,8 @@ void run_batch_fixed(std::vector<dms_job*>& jobs, hipStream_t st, hipStream_t ps
 				                                jobs[k]->done_escaped = d[DMS_PROGRESS_WORDS * k + 2];
 				                                jobs[k]->done_failed = d[DMS_PROGRESS_WORDS * k + 3];
 			                                }
-		                                });
+		                                },
+		                                nullptr, cancel);
 		if(e != hipSuccess)
 		{
 			set_err(std::string("trajectory kernel: ") + hipGetErrorString(e));
@@ -1011,8 +1030,8 @@ void run_batch_fixed(std::vector<dms_job*>& jobs, hipStream_t st, hipStream_t ps
 	/* A run-level fuse stopped a job before its last trajectory: its histogram
 	 * sums hold trajectories past the stop, so recount it exactly (chunked). */
 	for(dms_job* job : jobs)
-		if(job->status == DMS_OK && job->needs_rerun)
-			run_job_chunked(job, st, pst);
+		if(job->status == DMS_OK && job->needs_rerun && !(cancelled && cancelled->load()))
+			run_job_chunked(job, st, pst, cancel, cancelled);
 }
 
 /* Largest slot count of one batched launch (initial conditions and summaries
@@ -1057,13 +1076,46 @@ void run_batch_worker(std::shared_ptr<dms_batch> batch)
 	}
 	const hipStream_t st = ctx->pool[slot], pst = ctx->progress_stream;
 	hipSetDevice(ctx->device);
-	if(batch->jobs.size() == 1 && !batchable(batch->jobs[0]->desc))
-		run_job_chunked(batch->jobs[0], st, pst);
-	else
-		run_batch_fixed(batch->jobs, st, pst);
+	/* the launch's cancel word (dms_job_cancel); none left: not cancellable */
+	int word = -1;
+	{
+		std::lock_guard<std::mutex> lock(ctx->pool_mu);
+		if(!ctx->cancel_free.empty())
+		{
+			word = ctx->cancel_free.back();
+			ctx->cancel_free.pop_back();
+		}
+	}
+	const unsigned int* cancel = nullptr;
+	if(word >= 0)
+	{
+		__atomic_store_n(ctx->h_cancel + word, 0u, __ATOMIC_SEQ_CST);
+		batch->cancel_word.store(word);
+		if(batch->cancelled.load())
+			__atomic_store_n(ctx->h_cancel + word, 1u, __ATOMIC_SEQ_CST);
+		cancel = ctx->d_cancel + word;
+	}
+	if(!batch->cancelled.load())
+	{
+		if(batch->jobs.size() == 1 && !batchable(batch->jobs[0]->desc))
+			run_job_chunked(batch->jobs[0], st, pst, cancel, &batch->cancelled);
+		else
+			run_batch_fixed(batch->jobs, st, pst, cancel, &batch->cancelled);
+	}
+	if(batch->cancelled.load())
+		for(dms_job* j : batch->jobs)
+		{
+			j->status = DMS_ERR_CANCELLED;
+			j->err = "cancelled (dms_job_cancel)";
+		}
 	{
 		std::lock_guard<std::mutex> lock(ctx->pool_mu);
 		ctx->pool_free.push_back(slot);
+		if(word >= 0)
+		{
+			batch->cancel_word.store(-1);
+			ctx->cancel_free.push_back(word);
+		}
 	}
 	ctx->pool_cv.notify_one();
 	for(dms_job* j : batch->jobs)
@@ -1118,6 +1170,7 @@ struct ServeRequest
 	std::map<uint64_t, ServeRoundSums> mb;
 	bool final_count = false; /* replay stopped: the counted set is final */
 	bool cancelled = false;
+	bool by_request = false; /* stopped by dms_job_cancel */
 	double t_start = 0.0;
 };
 
@@ -1532,6 +1585,11 @@ void serve_replay(dms_server* S, ServeRequest* q)
 {
 	dms_job* job = q->job;
 	const double elapsed_sec = (now_ms() - q->t_start) * 1e-3;
+	if(!q->final_count && job->cancel_requested.load())
+	{
+		q->final_count = true;
+		q->by_request = true;
+	}
 	while(!q->final_count)
 	{
 		if(!q->R.open_round())
@@ -1671,8 +1729,12 @@ void serve_driver(dms_server* S)
 				S->free_reqs.push_back(q->slot);
 				S->active.erase(S->active.begin() + (long) i);
 				S->n_active = S->active.size();
+				const bool by_request = q->by_request;
 				delete q;
-				serve_finish_job(S, job, DMS_OK, "");
+				if(by_request)
+					serve_finish_job(S, job, DMS_ERR_CANCELLED, "cancelled (dms_job_cancel)");
+				else
+					serve_finish_job(S, job, DMS_OK, "");
 				work++;
 				continue;
 			}
@@ -1939,6 +2001,13 @@ int dms_create(dms_ctx** out, int device, const double* raw_rows, size_t n_rows)
 		ctx->dev.iso_Z = ctx->d_Z;
 		if(scratch_address(device, &ctx->scratch_ptr) != hipSuccess)
 			return cleanup("hipGetSymbolAddress of the scratch array failed");
+		if(hipHostMalloc((void**) &ctx->h_cancel, sizeof(unsigned int) * DMS_CANCEL_WORDS, hipHostMallocMapped | hipHostMallocCoherent)
+		       != hipSuccess
+		   || hipHostGetDevicePointer((void**) &ctx->d_cancel, ctx->h_cancel, 0) != hipSuccess)
+			return cleanup("pinned allocation of the cancel words failed");
+		memset(ctx->h_cancel, 0, sizeof(unsigned int) * DMS_CANCEL_WORDS);
+		for(int k = DMS_CANCEL_WORDS - 1; k >= 0; k--)
+			ctx->cancel_free.push_back(k);
 	}
 	*out = ctx;
 	return DMS_OK;
@@ -1964,6 +2033,8 @@ void dms_destroy(dms_ctx* ctx)
 			hipStreamDestroy(ctx->progress_stream);
 		for(hipStream_t sk : ctx->pool)
 			hipStreamDestroy(sk);
+		if(ctx->h_cancel)
+			hipHostFree(ctx->h_cancel);
 	}
 	delete ctx;
 }
@@ -2371,6 +2442,7 @@ int dms_simulate_events_noalloc(dms_point* pt, const dms_job_desc* job, const dm
 	a.fill_ctl = nullptr;
 	a.cap_stop = 0;
 	a.fill_extra = 0;
+	a.cancel = nullptr;
 	hipLaunchKernelGGL(J.mixed ? dms_traj_kernel_mixed : dms_traj_kernel, dim3(traj_grid(ctx, n)), dim3(DMS_TRAJ_BLOCK), ctx->traj_lds,
 	                   st, a);
 	hipError_t e = hipGetLastError();
@@ -2709,6 +2781,25 @@ void dms_server_destroy(dms_server* S)
 	hipSetDevice(S->ctx->device);
 	serve_release(S);
 	delete S;
+}
+
+int dms_job_cancel(dms_job* job)
+{
+	if(job == nullptr)
+		return fail(nullptr, DMS_ERR_ARG, "dms_job_cancel: null job");
+	if(job->finished.load())
+		return DMS_OK;
+	job->cancel_requested.store(1);
+	if(job->server != nullptr)
+		return DMS_OK; /* the serving driver stops the request */
+	dms_batch* b = job->batch.get();
+	if(b == nullptr)
+		return DMS_OK;
+	b->cancelled.store(1);
+	const int w = b->cancel_word.load();
+	if(w >= 0)
+		__atomic_store_n(job->pt->ctx->h_cancel + w, 1u, __ATOMIC_SEQ_CST);
+	return DMS_OK;
 }
 
 int dms_job_poll(dms_job* job, dms_job_status* st)

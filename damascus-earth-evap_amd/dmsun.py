@@ -18,6 +18,7 @@ DATA_FILE = REPO / "data" / "model_agss09.dat"
 
 # dms_status
 DMS_OK = 0
+DMS_ERR_CANCELLED = -6
 
 # dms_reason (TrajectoryTerminationReason)
 REASONS = ["Unknown", "OutwardEscape", "Scatter", "WallTimeLimit", "MaxFreeSteps", "MaxScatterings",
@@ -228,6 +229,7 @@ def _declare(lib):
         "dms_server_submit": (C.c_int, [vp, vp, _P(JobDesc), _P(vp)]),
         "dms_server_get_stats": (C.c_int, [vp, _P(ServerStats)]),
         "dms_server_destroy": (None, [vp]),
+        "dms_job_cancel": (C.c_int, [vp]),
         "dms_job_poll": (C.c_int, [vp, _P(JobStatus)]),
         "dms_job_wait": (C.c_int, [vp, _P(JobStatus)]),
         "dms_job_read": (C.c_int, [vp, _P(Counters), _P(C.c_double), _P(C.c_uint64), C.c_size_t,
@@ -248,7 +250,7 @@ def _declare(lib):
 EXPORTED = ["dms_abi_version", "dms_create", "dms_destroy", "dms_last_error", "dms_set_job_memory_budget", "dms_load_solar_file",
             "dms_solar_eval", "dms_isotopes", "dms_constant", "dms_point_create", "dms_point_destroy",
             "dms_point_rate_grid", "dms_point_grid_shape", "dms_point_cross_sections", "dms_point_rates", "dms_initial_conditions",
-            "dms_simulate_events", "dms_simulate_events_noalloc", "dms_job_submit", "dms_job_submit_batch", "dms_job_submit_multi", "dms_multi_autotune", "dms_server_create", "dms_server_submit", "dms_server_get_stats", "dms_server_destroy", "dms_job_poll", "dms_job_wait", "dms_job_read",
+            "dms_simulate_events", "dms_simulate_events_noalloc", "dms_job_submit", "dms_job_submit_batch", "dms_job_submit_multi", "dms_multi_autotune", "dms_server_create", "dms_server_submit", "dms_server_get_stats", "dms_server_destroy", "dms_job_cancel", "dms_job_poll", "dms_job_wait", "dms_job_read",
             "dms_job_timing", "dms_job_destroy", "dms_probe_math"]
 
 _lib = None
@@ -407,6 +409,14 @@ class Server:
         self.ctx.check(lib().dms_job_poll(handle[0], C.byref(st)), "dms_job_poll")
         return st
 
+    def cancel(self, handle):
+        """dms_job_cancel on a request: it stops where it is."""
+        self.ctx.check(lib().dms_job_cancel(handle[0]), "dms_job_cancel")
+
+    def wait_status(self, handle):
+        st = JobStatus()
+        return lib().dms_job_wait(handle[0], C.byref(st))
+
     def stats(self):
         st = ServerStats()
         self.ctx.check(lib().dms_server_get_stats(self.h, C.byref(st)), "dms_server_get_stats")
@@ -523,6 +533,15 @@ class Point:
         h = C.c_void_p()
         self.ctx.check(lib().dms_job_submit(self.h, C.byref(job), C.byref(h)), "dms_job_submit")
         return (h, job)
+
+    def cancel(self, handle):
+        """dms_job_cancel on a submitted job (its launch ends within a few ms)."""
+        self.ctx.check(lib().dms_job_cancel(handle[0]), "dms_job_cancel")
+
+    def wait_status(self, handle):
+        """dms_job_wait's return code (DMS_OK, DMS_ERR_CANCELLED, ...) without raising."""
+        st = JobStatus()
+        return lib().dms_job_wait(handle[0], C.byref(st))
 
     def collect(self, handle, want_records=False):
         """Wait for a submitted job, read its results and release it."""

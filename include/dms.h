@@ -40,7 +40,8 @@ enum dms_status
 	DMS_ERR_DEVICE = -2,      /* HIP runtime error */
 	DMS_ERR_NOMEM = -3,       /* device or host allocation failed */
 	DMS_ERR_UNSUPPORTED = -4, /* option outside this build (e.g. SD couplings) */
-	DMS_ERR_STATE = -5        /* call order violated (e.g. read before wait) */
+	DMS_ERR_STATE = -5,       /* call order violated (e.g. read before wait) */
+	DMS_ERR_CANCELLED = -6    /* the job was cancelled (dms_job_cancel) */
 };
 
 /* TrajectoryTerminationReason, include/Simulation_Trajectory.hpp:35-48 */
@@ -352,6 +353,16 @@ int dms_job_submit_multi(dms_point* const* points, const dms_job_desc* descs, si
 int dms_multi_autotune(dms_point* const* points, const dms_job_desc* descs, size_t n, uint64_t calibration_trajectories,
                        uint64_t calibration_rk_attempts, uint32_t* order, double* cost_per_trajectory);
 int dms_job_poll(dms_job* job, dms_job_status* status);
+/* Stop a running job: every lane of the launch it runs in ends within about
+ * 1024 loop iterations (a few ms for a lone trajectory), so that a runaway
+ * trajectory (no RKF45 budget, unbounded scatterings) cannot hold the device;
+ * jobs sharing that launch (a batch, a multi-point launch) end with it.  Each
+ * of them then returns DMS_ERR_CANCELLED from dms_job_wait and dms_job_read.
+ * A request of a dms_server stops at its current point instead: its slots not
+ * yet started are handed back and the trajectories already running finish.
+ * Cancelling a finished job does nothing.  No reference counterpart (the
+ * reference's runs end by their wall-time guard). */
+int dms_job_cancel(dms_job* job);
 int dms_job_wait(dms_job* job, dms_job_status* status);
 /* hist: 8 x n_bins doubles in the order of bincount.txt columns
  *   cap_dt, cap_v2dt, cap_dt_sq, cap_v2dt_sq, nc_dt, nc_v2dt, nc_dt_sq, nc_v2dt_sq
