@@ -1034,6 +1034,29 @@ void run_batch_fixed(std::vector<dms_job*>& jobs, hipStream_t st, hipStream_t ps
 			run_job_chunked(job, st, pst, cancel, cancelled);
 }
 
+/* Device bytes of a batched launch: per-lane histogram scratch (rows, counts,
+ * touched list), per-slot initial event, shift flag, summary and record, and
+ * the per-job class sums. */
+size_t batch_device_bytes(const dms_ctx* ctx, const std::vector<dms_job*>& jobs)
+{
+	const dms_job_desc& d0 = jobs[0]->desc;
+	const size_t nb = (size_t) job_view(d0).nbins;
+	const bool rows = d0.mode != DMS_MODE_CAPTURE;
+	size_t n = 0;
+	bool recs = false;
+	for(const dms_job* j : jobs)
+	{
+		n += (size_t) j->desc.max_trajectories;
+		recs = recs || j->desc.record_trajectories != 0;
+	}
+	const size_t lanes = (size_t) traj_grid(ctx, n) * DMS_TRAJ_BLOCK;
+	size_t bytes = n * (sizeof(dms_event) + sizeof(int32_t) + sizeof(DmsSummary) + (recs ? sizeof(dms_traj_record) : 0));
+	if(rows)
+		bytes += lanes * nb * (2 * sizeof(double) + sizeof(uint32_t) + sizeof(uint16_t))
+		         + jobs.size() * nb * (8 * sizeof(double) + 2 * sizeof(unsigned long long));
+	return bytes;
+}
+
 /* Largest slot count of one batched launch (initial conditions and summaries
  * of every trajectory stay resident: about 150 B each). */
 #define DMS_BATCH_MAX_SLOTS ((uint64_t) 1 << 27)
@@ -1099,6 +1122,14 @@ void run_batch_worker(std::shared_ptr<dms_batch> batch)
 	{
 		if(batch->jobs.size() == 1 && !batchable(batch->jobs[0]->desc))
 			run_job_chunked(batch->jobs[0], st, pst, cancel, &batch->cancelled);
+		else if(batch_device_bytes(ctx, batch->jobs) > ctx->job_memory_budget)
+		{
+			/* a batch whose buffers exceed the job memory budget: each job as
+			 * chunked launches within the budget (the same results) */
+			for(dms_job* j : batch->jobs)
+				if(!batch->cancelled.load())
+					run_job_chunked(j, st, pst, cancel, &batch->cancelled);
+		}
 		else
 			run_batch_fixed(batch->jobs, st, pst, cancel, &batch->cancelled);
 	}

@@ -251,10 +251,12 @@ int dms_create(dms_ctx** ctx, int device, const double* raw_rows, size_t n_rows)
 void dms_destroy(dms_ctx* ctx);
 const char* dms_last_error(const dms_ctx* ctx);
 
-/* Device memory a capture-target job may use for its trajectory buffers
- * (chunk size = budget / per-trajectory bytes; default 32 GiB).  Up to 8 jobs
- * run on the device at once, each on its own streams; a serving loop sizes
- * this to its admission budget.  No reference counterpart. */
+/* Device memory one job may use for its trajectory buffers (default 32 GiB):
+ * a capture-target job sizes its chunks by it (budget / per-trajectory bytes),
+ * and a fixed-count job or batch whose launch would need more (per-lane
+ * histogram scratch, per-slot events and summaries) runs as chunked launches
+ * within it instead, with the same results.  Up to 2 jobs run on the device
+ * at once, each on its own stream.  No reference counterpart. */
 int dms_set_job_memory_budget(dms_ctx* ctx, size_t bytes);
 
 /* Parse the data rows of a model_agss09.dat-format file (20 header lines). */

@@ -115,6 +115,24 @@ def test_job_memory_budget_chunks_give_same_results(gpu_ctx, multi_points):
         gpu_ctx.set_job_memory_budget(1 << 20)
 
 
+def test_fixed_count_job_over_the_memory_budget_runs_in_chunks(gpu_ctx, multi_points):
+    """A fixed-count job whose batched launch needs more device memory than the
+    job budget (2000 bins: 2.9 GB of per-lane scratch against 64 MiB) runs as
+    chunked launches within it, with the results of the unrestricted run."""
+    pt = multi_points[0]
+    job = dmsun.make_job(mode=dmsun.MODE_PARAMETER_POINT, max_trajectories=4000, capture_target=0, seed=6,
+                         max_scatterings=10**14, n_bins=2000, record=0, rk_attempt_budget=RK_BUDGET)
+    full = pt.run_job(job, want_records=False)
+    gpu_ctx.set_job_memory_budget(64 << 20)
+    try:
+        small = pt.run_job(job, want_records=False)
+    finally:
+        gpu_ctx.set_job_memory_budget(32 << 30)
+    print(f"launches: unrestricted {full['launches']}, 64 MiB budget {small['launches']}", flush=True)
+    assert_job_equal(small, full)
+    assert full["launches"] == 1 and small["launches"] > 1
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_serving_results_same_alone_and_in_the_mix(tmp_path, precision):
     out = tmp_path / "serve.json"
