@@ -173,6 +173,7 @@ def measure_uneven(dmsun, ctx, args):
                                halo=dmsun.default_halo((0.0, 248.0, 0.0))) for k in range(len(pts))]
         t = time.perf_counter()
         order, cost = dmsun.autotune_multi(pts, jobs)
+        refill, refill_secs = dmsun.autotune_knobs(pts, jobs, refill_candidates=(1, 4, 16), repeats=2)
         t_cal = time.perf_counter() - t
         t = time.perf_counter()
         res = dmsun.run_multi([pts[k] for k in order], [jobs[k] for k in order])
@@ -182,7 +183,8 @@ def measure_uneven(dmsun, ctx, args):
         trunc = sum(r["counters"].computational_truncations for r in res)
         leg = {"trajectories_per_s": traj / wall, "scatterings_per_s": scat / wall, "seconds": wall,
                "calibration_s": t_cal, "trajectories": traj, "scatterings": scat, "truncated": trunc,
-               "kernel_ms": res[0]["kernel_ms"], "cost_per_trajectory_range": [float(cost.min()), float(cost.max())]}
+               "kernel_ms": res[0]["kernel_ms"], "cost_per_trajectory_range": [float(cost.min()), float(cost.max())],
+               "refill_batch": refill, "refill_candidates_s": {str(k): float(v) for k, v in zip((1, 4, 16), refill_secs)}}
         if name == "fp64":
             t = time.perf_counter()
             dmsun.run_multi(pts, jobs)
@@ -190,6 +192,7 @@ def measure_uneven(dmsun, ctx, args):
         out[name] = leg
         print(f"uneven {name}: {traj} trajectories, {scat} scatterings in {wall:.2f} s (calibration {t_cal:.2f} s)",
               file=sys.stderr, flush=True)
+    ctx.set_refill_batch(1)
     for p in pts:
         p.close()
     return out

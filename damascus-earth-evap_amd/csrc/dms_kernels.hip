@@ -497,9 +497,21 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 			}
 			else
 			{
-			/* lane refill: one atomic per wave */
+			/* lane refill: one atomic per wave, once refill_batch lanes are free
+			 * (or no lane of the wave is left running), so that trajectory starts
+			 * run together instead of one lane at a time */
 			const unsigned long long nmask = __ballot(L.phase == PH_NEED);
-			if(__builtin_expect(nmask != 0, 0))
+			bool refill = nmask != 0;
+			if(__builtin_expect(refill, 0) && (uint32_t) __popcll(nmask) < a.refill_batch
+			   && __ballot((L.phase != PH_NEED) & (L.phase != PH_DONE)) != 0)
+			{
+				/* wait for more free lanes unless every slot is handed out */
+				unsigned int handed = 0;
+				if(lane == __ffsll((long long) nmask) - 1)
+					handed = __hip_atomic_load(a.next, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+				refill = (unsigned int) __builtin_amdgcn_readlane((int) handed, __ffsll((long long) nmask) - 1) >= a.n;
+			}
+			if(__builtin_expect(refill, 0))
 			{
 				const int leader = __ffsll((long long) nmask) - 1;
 				unsigned int base = 0, limit = a.n;

@@ -85,6 +85,9 @@ struct DmsTrajArgs
 	DmsSummary* summ;
 	DmsRows R;
 	unsigned int* next;
+	/* lanes a wave lets go free before it takes new slots (1: at once); a wave
+	 * with no lane left running refills whatever it has free */
+	uint32_t refill_batch;
 	/* jobs sharing the launch: slots [job_start[j], job_start[j+1]) belong to job j,
 	 * whose trajectory ids start at job_first_id[j] */
 	uint32_t n_jobs;

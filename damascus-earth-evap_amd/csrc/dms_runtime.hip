@@ -22,6 +22,7 @@
 #include <condition_variable>
 #include <deque>
 #include <functional>
+#include <limits>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -71,6 +72,8 @@ struct dms_ctx
 	unsigned int* h_cancel = nullptr;
 	unsigned int* d_cancel = nullptr;
 	std::vector<int> cancel_free;
+	/* launch knobs (dms_set_launch_knobs, chosen by dms_autotune_knobs) */
+	dms_launch_knobs knobs = {1u, 0u};
 };
 
 struct dms_point
@@ -405,6 +408,7 @@ hipError_t run_trajectories(dms_ctx* ctx, hipStream_t st, hipStream_t pst, const
 		a.R.hcounts = nullptr;
 	}
 	a.next = B.next;
+	a.refill_batch = ctx->knobs.refill_batch;
 	a.n_jobs = (uint32_t) LJ.start.size();
 	a.job_start = LJ.d_start;
 	a.job_first_id = LJ.d_first;
@@ -2464,6 +2468,7 @@ int dms_simulate_events_noalloc(dms_point* pt, const dms_job_desc* job, const dm
 	a.R.hist = nullptr;
 	a.R.hcounts = nullptr;
 	a.next = reinterpret_cast<unsigned int*>(s + kScratchNext);
+	a.refill_batch = 1;
 	a.n_jobs = 1;
 	a.job_start = reinterpret_cast<const uint32_t*>(s + kScratchStart);
 	a.job_first_id = reinterpret_cast<const uint64_t*>(s + kScratchFirst);
@@ -2605,16 +2610,18 @@ int dms_job_submit_multi(dms_point* const* points, const dms_job_desc* descs, si
 	return DMS_OK;
 }
 
-int dms_multi_autotune(dms_point* const* points, const dms_job_desc* descs, size_t n, uint64_t calibration_trajectories,
-                       uint64_t calibration_rk_attempts, uint32_t* order, double* cost_per_trajectory)
+namespace
 {
-	if(points == nullptr || descs == nullptr || order == nullptr || n == 0 || points[0] == nullptr)
-		return fail(nullptr, DMS_ERR_ARG, "dms_multi_autotune: null argument or empty batch");
+/* The autotuners' calibration launch: the same points, keys and limits,
+ * trajectory ids past each job's own range, an RKF45-attempt cap that bounds
+ * the launch.  Returns each job's lane iterations per trajectory and the
+ * launch's kernel time. */
+int calibration_launch(dms_point* const* points, const dms_job_desc* descs, size_t n, uint64_t calibration_trajectories,
+                       uint64_t calibration_rk_attempts, const char* who, std::vector<double>* cost, double* kernel_ms)
+{
 	dms_ctx* ctx = points[0]->ctx;
 	if(calibration_trajectories == 0 || calibration_trajectories * n > DMS_BATCH_MAX_SLOTS || calibration_rk_attempts == 0)
-		return fail(ctx, DMS_ERR_ARG, "dms_multi_autotune: calibration size must be positive and fit one launch");
-	/* the calibration jobs: the same points, keys and limits, trajectory ids
-	 * past each job's own range, an RKF45-attempt cap that bounds the launch */
+		return fail(ctx, DMS_ERR_ARG, std::string(who) + ": calibration size must be positive and fit one launch");
 	std::vector<dms_job_desc> cal(descs, descs + n);
 	for(size_t k = 0; k < n; k++)
 	{
@@ -2629,7 +2636,8 @@ int dms_multi_autotune(dms_point* const* points, const dms_job_desc* descs, size
 	const int st = dms_job_submit_multi(points, cal.data(), n, jobs.data());
 	if(st != DMS_OK)
 		return st;
-	std::vector<double> cost(n, 0.0);
+	if(cost)
+		cost->assign(n, 0.0);
 	int status = DMS_OK;
 	for(size_t k = 0; k < n; k++)
 	{
@@ -2639,12 +2647,30 @@ int dms_multi_autotune(dms_point* const* points, const dms_job_desc* descs, size
 		{
 			const dms_counters& c = jobs[k]->C;
 			/* lane iterations: one per RKF45 attempt, about four per scattering */
-			cost[k] = c.trajectories ? ((double) c.n_rk_attempts + 4.0 * (double) c.total_scatterings) / (double) c.trajectories : 0.0;
+			if(cost)
+				(*cost)[k] =
+				    c.trajectories ? ((double) c.n_rk_attempts + 4.0 * (double) c.total_scatterings) / (double) c.trajectories : 0.0;
+			if(kernel_ms && k == 0)
+				*kernel_ms = jobs[k]->kernel_ms;
 		}
 		dms_job_destroy(jobs[k]);
 	}
 	if(status != DMS_OK)
-		return fail(ctx, status, "dms_multi_autotune: calibration launch failed");
+		return fail(ctx, status, std::string(who) + ": calibration launch failed");
+	return DMS_OK;
+}
+} // namespace
+
+int dms_multi_autotune(dms_point* const* points, const dms_job_desc* descs, size_t n, uint64_t calibration_trajectories,
+                       uint64_t calibration_rk_attempts, uint32_t* order, double* cost_per_trajectory)
+{
+	if(points == nullptr || descs == nullptr || order == nullptr || n == 0 || points[0] == nullptr)
+		return fail(nullptr, DMS_ERR_ARG, "dms_multi_autotune: null argument or empty batch");
+	std::vector<double> cost;
+	const int st = calibration_launch(points, descs, n, calibration_trajectories, calibration_rk_attempts, "dms_multi_autotune",
+	                                  &cost, nullptr);
+	if(st != DMS_OK)
+		return st;
 	/* cheapest first: the long trajectories of the costly points start last and
 	 * finish in waves the short ones have left, instead of sharing full waves */
 	std::vector<uint32_t> idx(n);
@@ -2657,6 +2683,70 @@ int dms_multi_autotune(dms_point* const* points, const dms_job_desc* descs, size
 		if(cost_per_trajectory)
 			cost_per_trajectory[k] = cost[k];
 	}
+	return DMS_OK;
+}
+
+int dms_set_launch_knobs(dms_ctx* ctx, const dms_launch_knobs* knobs)
+{
+	if(ctx == nullptr || knobs == nullptr)
+		return fail(ctx, DMS_ERR_ARG, "dms_set_launch_knobs: null argument");
+	if(knobs->refill_batch < 1 || knobs->refill_batch > 64)
+		return fail(ctx, DMS_ERR_ARG, "dms_set_launch_knobs: refill_batch must be in [1, 64]");
+	ctx->knobs = *knobs;
+	ctx->knobs.pad = 0;
+	return DMS_OK;
+}
+
+int dms_get_launch_knobs(const dms_ctx* ctx, dms_launch_knobs* knobs)
+{
+	if(ctx == nullptr || knobs == nullptr)
+		return fail(nullptr, DMS_ERR_ARG, "dms_get_launch_knobs: null argument");
+	*knobs = ctx->knobs;
+	return DMS_OK;
+}
+
+int dms_autotune_knobs(dms_point* const* points, const dms_job_desc* descs, size_t n, uint64_t calibration_trajectories,
+                       uint64_t calibration_rk_attempts, const uint32_t* refill_candidates, size_t n_candidates, int repeats,
+                       double* seconds, dms_launch_knobs* best, int apply)
+{
+	if(points == nullptr || descs == nullptr || n == 0 || points[0] == nullptr || refill_candidates == nullptr || n_candidates == 0
+	   || best == nullptr || repeats < 1)
+		return fail(nullptr, DMS_ERR_ARG, "dms_autotune_knobs: null argument, no candidates or repeats < 1");
+	dms_ctx* ctx = points[0]->ctx;
+	for(size_t i = 0; i < n_candidates; i++)
+		if(refill_candidates[i] < 1 || refill_candidates[i] > 64)
+			return fail(ctx, DMS_ERR_ARG, "dms_autotune_knobs: refill_batch candidates must be in [1, 64]");
+	const dms_launch_knobs saved = ctx->knobs;
+	std::vector<double> t(n_candidates, std::numeric_limits<double>::infinity());
+	/* candidates in turn, repeated: drifts of the device hit them alike */
+	for(int r = 0; r < repeats; r++)
+		for(size_t i = 0; i < n_candidates; i++)
+		{
+			ctx->knobs.refill_batch = refill_candidates[i];
+			double ms = 0.0;
+			const int st = calibration_launch(points, descs, n, calibration_trajectories, calibration_rk_attempts,
+			                                  "dms_autotune_knobs", nullptr, &ms);
+			if(st != DMS_OK)
+			{
+				ctx->knobs = saved;
+				return st;
+			}
+			t[i] = std::min(t[i], ms * 1e-3);
+		}
+	/* the first candidate is the reference; another replaces it only when it is
+	 * more than 3% faster (the run-to-run spread of the calibration launch:
+	 * on the 65-point workload its ranking of 1, 4 and 16 was within 1%, while
+	 * the full launch took 2.14, 2.16 and 2.34 s) */
+	size_t b = 0;
+	for(size_t i = 1; i < n_candidates; i++)
+		if(t[i] < 0.97 * t[0] && t[i] < t[b])
+			b = i;
+	best->refill_batch = refill_candidates[b];
+	best->pad = 0;
+	if(seconds)
+		for(size_t i = 0; i < n_candidates; i++)
+			seconds[i] = t[i];
+	ctx->knobs = apply ? *best : saved;
 	return DMS_OK;
 }
 

@@ -101,6 +101,10 @@ class EvapEvent(C.Structure):
                 ("completion_wall_time_sec", C.c_double), ("lifetime_unbinding_sec", C.c_double)]
 
 
+class LaunchKnobs(C.Structure):
+    _fields_ = [("refill_batch", C.c_uint32), ("pad", C.c_uint32)]
+
+
 class ServerDesc(C.Structure):
     _fields_ = [("launch", JobDesc), ("slot_ring", C.c_uint32), ("max_requests", C.c_uint32),
                 ("max_rounds", C.c_uint32), ("pad", C.c_uint32)]
@@ -230,6 +234,10 @@ def _declare(lib):
         "dms_server_get_stats": (C.c_int, [vp, _P(ServerStats)]),
         "dms_server_destroy": (None, [vp]),
         "dms_job_cancel": (C.c_int, [vp]),
+        "dms_set_launch_knobs": (C.c_int, [vp, _P(LaunchKnobs)]),
+        "dms_get_launch_knobs": (C.c_int, [vp, _P(LaunchKnobs)]),
+        "dms_autotune_knobs": (C.c_int, [_P(vp), _P(JobDesc), C.c_size_t, C.c_uint64, C.c_uint64, _P(C.c_uint32),
+                                         C.c_size_t, C.c_int, _P(C.c_double), _P(LaunchKnobs), C.c_int]),
         "dms_job_poll": (C.c_int, [vp, _P(JobStatus)]),
         "dms_job_wait": (C.c_int, [vp, _P(JobStatus)]),
         "dms_job_read": (C.c_int, [vp, _P(Counters), _P(C.c_double), _P(C.c_uint64), C.c_size_t,
@@ -250,7 +258,8 @@ def _declare(lib):
 EXPORTED = ["dms_abi_version", "dms_create", "dms_destroy", "dms_last_error", "dms_set_job_memory_budget", "dms_load_solar_file",
             "dms_solar_eval", "dms_isotopes", "dms_constant", "dms_point_create", "dms_point_destroy",
             "dms_point_rate_grid", "dms_point_grid_shape", "dms_point_cross_sections", "dms_point_rates", "dms_initial_conditions",
-            "dms_simulate_events", "dms_simulate_events_noalloc", "dms_job_submit", "dms_job_submit_batch", "dms_job_submit_multi", "dms_multi_autotune", "dms_server_create", "dms_server_submit", "dms_server_get_stats", "dms_server_destroy", "dms_job_cancel", "dms_job_poll", "dms_job_wait", "dms_job_read",
+            "dms_simulate_events", "dms_simulate_events_noalloc", "dms_job_submit", "dms_job_submit_batch", "dms_job_submit_multi", "dms_multi_autotune", "dms_server_create", "dms_server_submit", "dms_server_get_stats", "dms_server_destroy", "dms_job_cancel", "dms_set_launch_knobs", "dms_get_launch_knobs",
+            "dms_autotune_knobs", "dms_job_poll", "dms_job_wait", "dms_job_read",
             "dms_job_timing", "dms_job_destroy", "dms_probe_math"]
 
 _lib = None
@@ -312,6 +321,15 @@ class Context:
     def point(self, desc):
         return Point(self, desc)
 
+    def set_refill_batch(self, k):
+        """dms_set_launch_knobs: free lanes a wave waits for before it refills."""
+        self.check(lib().dms_set_launch_knobs(self.h, C.byref(LaunchKnobs(k, 0))), "dms_set_launch_knobs")
+
+    def refill_batch(self):
+        k = LaunchKnobs()
+        self.check(lib().dms_get_launch_knobs(self.h, C.byref(k)), "dms_get_launch_knobs")
+        return k.refill_batch
+
     def set_job_memory_budget(self, nbytes):
         """Device bytes one job's trajectory buffers may hold (default 32 GiB);
         a job larger than that runs as several launches."""
@@ -362,6 +380,24 @@ def autotune_multi(points, jobs, calibration_trajectories=128, calibration_rk_at
     points[0].ctx.check(L.dms_multi_autotune(hp, descs, n, calibration_trajectories, calibration_rk_attempts, order, dptr(cost)),
                         "dms_multi_autotune")
     return list(order), cost
+
+
+def autotune_knobs(points, jobs, refill_candidates=(1, 4, 16), repeats=2, apply=True, calibration_trajectories=128,
+                   calibration_rk_attempts=5000):
+    """dms_autotune_knobs: the calibration launch once per candidate
+    refill_batch (repeated); returns (best refill_batch, seconds per
+    candidate) and, with apply, sets it on the context."""
+    L = lib()
+    n = len(jobs)
+    descs = (JobDesc * n)(*jobs)
+    hp = (C.c_void_p * n)(*[p.h.value for p in points])
+    cand = (C.c_uint32 * len(refill_candidates))(*refill_candidates)
+    secs = np.zeros(len(refill_candidates), dtype=np.float64)
+    best = LaunchKnobs()
+    points[0].ctx.check(L.dms_autotune_knobs(hp, descs, n, calibration_trajectories, calibration_rk_attempts, cand,
+                                             len(refill_candidates), repeats, dptr(secs), C.byref(best), 1 if apply else 0),
+                        "dms_autotune_knobs")
+    return best.refill_batch, secs
 
 
 def run_multi_tuned(points, jobs, want_records=False, **calibration):

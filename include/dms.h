@@ -354,6 +354,29 @@ int dms_job_submit_multi(dms_point* const* points, const dms_job_desc* descs, si
  * never a job's results.  No reference counterpart. */
 int dms_multi_autotune(dms_point* const* points, const dms_job_desc* descs, size_t n, uint64_t calibration_trajectories,
                        uint64_t calibration_rk_attempts, uint32_t* order, double* cost_per_trajectory);
+/* Launch knobs of a context: results never depend on them, only speed.
+ * refill_batch: free lanes a wave waits for before it takes new slots (1 = at
+ * once, up to 64), so that trajectory starts run together instead of one lane
+ * at a time; a wave with no lane left running, or a launch whose slots are all
+ * handed out, refills at once.  No reference counterpart. */
+typedef struct
+{
+	uint32_t refill_batch;
+	uint32_t pad;
+} dms_launch_knobs;
+
+int dms_set_launch_knobs(dms_ctx* ctx, const dms_launch_knobs* knobs);
+int dms_get_launch_knobs(const dms_ctx* ctx, dms_launch_knobs* knobs);
+/* Autotuner of the launch knobs: the calibration launch of dms_multi_autotune
+ * (same jobs, same caps) run once per candidate refill_batch, `repeats` times
+ * each in turn; seconds[i] receives candidate i's fastest kernel time.  The
+ * first candidate is the reference: another is chosen (*best) only when it is
+ * more than 3% faster.  With apply != 0 the context takes *best.  Submit no
+ * other jobs on the context meanwhile (the knobs are context-wide).  No
+ * reference counterpart. */
+int dms_autotune_knobs(dms_point* const* points, const dms_job_desc* descs, size_t n, uint64_t calibration_trajectories,
+                       uint64_t calibration_rk_attempts, const uint32_t* refill_candidates, size_t n_candidates,
+                       int repeats, double* seconds, dms_launch_knobs* best, int apply);
 int dms_job_poll(dms_job* job, dms_job_status* status);
 /* Stop a running job: every lane of the launch it runs in ends within about
  * 1024 loop iterations (a few ms for a lone trajectory), so that a runaway

@@ -59,6 +59,55 @@ def test_multi_point_launch_matches_oracle_and_alone(gpu_ctx, oracle_solar, mult
         assert_job_equal(pt.run_job(job, want_records=False), r)
 
 
+def test_batched_uneven_all_65_points_match_the_oracle(gpu_ctx, oracle_solar):
+    """The batched-uneven workload's 65 points (64 SI on m = logspace(0.1, 5
+    GeV, 8) x sigma_p = logspace(1e-39, 1e-35 cm^2, 8), plus the dark-photon
+    point) in one multi-point launch, with its limits (1e5 scatterings and 1e5
+    RKF45 attempts per trajectory): each job equals the oracle's run of its own
+    point and descriptor.  150 trajectories per point and 300 x 300 rate tables
+    keep the oracle's share to seconds."""
+    masses = np.logspace(np.log10(0.1), np.log10(5.0), 8)
+    sigmas = np.logspace(-39.0, -35.0, 8)
+    descs = [dmsun.make_point(float(m), float(s), interpolation_points=300) for m in masses for s in sigmas]
+    descs.append(dmsun.make_point(0.1, 0.0, sigma_e_cm2=1e-37, interaction=dmsun.DARK_PHOTON,
+                                  form_factor=dmsun.FF_GENERAL, mediator_mass_MeV=0.01, interpolation_points=300))
+    jobs = [dmsun.make_job(mode=dmsun.MODE_PARAMETER_POINT, max_trajectories=150, capture_target=0, seed=20260710,
+                           point_id=k, max_scatterings=100_000, rk_attempt_budget=100_000, n_bins=400, record=0)
+            for k in range(len(descs))]
+    pts = [gpu_ctx.point(d) for d in descs]
+    print(f"65-point launch: {sum(j.max_trajectories for j in jobs)} trajectories", flush=True)
+    res = dmsun.run_multi(pts, jobs)
+    for k, (d, job, r) in enumerate(zip(descs, jobs, res)):
+        o = oracle_lib.Point(oracle_solar, d, threads=16).run_job(job, threads=16, want_records=False)
+        assert_job_equal(r, o), k
+    for p in pts:
+        p.close()
+
+
+def test_refill_batch_knob_keeps_results_and_autotunes(gpu_ctx, multi_points):
+    """Waves that wait for 16 free lanes before refilling give every job the
+    results of immediate refills (bit-identical records); the knob autotuner
+    times the calibration launch per candidate and keeps the fastest."""
+    jobs = multi_jobs(dmsun.MODE_PARAMETER_POINT)
+    base = dmsun.run_multi(multi_points, jobs, want_records=True)
+    try:
+        gpu_ctx.set_refill_batch(16)
+        batched = dmsun.run_multi(multi_points, jobs, want_records=True)
+        best, secs = dmsun.autotune_knobs(multi_points, jobs, refill_candidates=(1, 8, 32), repeats=2)
+        print(f"refill_batch candidates (1, 8, 32): {np.round(secs, 4).tolist()} s, best {best}", flush=True)
+        assert best in (1, 8, 32) and np.all(secs > 0) and gpu_ctx.refill_batch() == best
+        # the first candidate stays unless another is more than 3% faster
+        k = [1, 8, 32].index(best)
+        assert (k == 0 and np.all(secs[1:] >= 0.97 * secs[0])) or (secs[k] < 0.97 * secs[0] and secs[k] == secs.min())
+    finally:
+        gpu_ctx.set_refill_batch(1)
+    for a, b in zip(base, batched):
+        assert_job_equal(b, a)
+        assert np.array_equal(rec_bytes(a["records"]), rec_bytes(b["records"]))
+    with pytest.raises(dmsun.DmsError):
+        gpu_ctx.set_refill_batch(0)
+
+
 def test_multi_point_mixed_equals_alone(multi_points):
     jobs = multi_jobs(dmsun.MODE_PARAMETER_POINT, precision=dmsun.MIXED)
     print("multi-point launch, mixed precision", flush=True)
