@@ -154,12 +154,12 @@ class Point:
         assert st == 0, st
         return recs, rows, counts
 
-    def run_job(self, job, threads=8, want_records=True, record_capacity=None):
+    def run_job(self, job, threads=8, want_records=True, record_capacity=None, event_capacity=1 << 16):
         nb = job.n_bins
         hist = np.zeros((8, nb))
         counts = np.zeros((2, nb), dtype=np.uint64)
         cnt = dmsun.Counters()
-        cap = 1 << 16
+        cap = event_capacity
         evs = (dmsun.EvapEvent * cap)()
         n_ev = C.c_size_t(0)
         rc = record_capacity if record_capacity is not None else (job.max_trajectories or 100000)

@@ -272,6 +272,42 @@ def test_batch_rejects_mixed_descriptors(gpu_ctx):
         gpu_ctx.point(desc).run_batch([a, c])
 
 
+def test_parameter_point_workload_full_size(gpu_ctx, oracle_solar):
+    """The parameter-point workload at its full size, exactly as the bench's
+    single job runs it: SI contact m = 0.3 GeV, sigma_p = 1e-36 cm^2, 10^6
+    trajectories in one launch, unlimited scatterings, no RKF45 budget, 1000
+    bins, 1000 x 1000 rate grid.  The job's counters, visit counts and
+    evaporation events equal the oracle's Generate_Data run of the same
+    descriptor (16 threads).  Its histogram sums are sums of up to 10^6
+    non-negative per-trajectory terms, added by device atomics in a varying
+    order against the oracle's id order: the recursive-summation bound puts
+    their relative difference below n u = 10^6 x 2^-53 ~ 1.1e-10 (measured:
+    ~2e-12), so that is the tolerance here (1e-12 holds for the smaller jobs
+    of the other tests)."""
+    desc = dmsun.make_point(0.3, 1e-36, interpolation_points=1000)
+    job = dmsun.make_job(mode=dmsun.MODE_PARAMETER_POINT, max_trajectories=1_000_000, capture_target=0,
+                         max_scatterings=10**14, n_bins=1000, record=0, halo=dmsun.default_halo((0.0, 248.0, 0.0)))
+    print("parameter-point workload: 1e6 trajectories", flush=True)
+    res = gpu_ctx.point(desc).run_job(job, want_records=False)
+    print(f"GPU done: {res['counters'].trajectories} trajectories, {res['counters'].captured} captured", flush=True)
+    ores = oracle_lib.Point(oracle_solar, desc, threads=16).run_job(job, threads=16, want_records=False,
+                                                                    event_capacity=1 << 19)
+    assert len(ores["events"]) < 1 << 19
+    c, oc = res["counters"], ores["counters"]
+    for f, _ in dmsun.Counters._fields_:
+        if f in ("computing_time_sec", "simulated_trajectories"):
+            continue
+        a, b = getattr(c, f), getattr(oc, f)
+        assert (list(a) if f == "reason_counts" else a) == (list(b) if f == "reason_counts" else b), f
+    assert np.array_equal(res["counts"], ores["counts"])
+    assert [(e.trajectory_id, e.lifetime_unbinding_sec) for e in res["events"]] == \
+        [(e.trajectory_id, e.lifetime_unbinding_sec) for e in ores["events"]]
+    rel = np.abs(res["hist"] - ores["hist"]) / np.maximum(np.abs(ores["hist"]), 1e-300)
+    print(f"histogram: largest relative difference {rel.max():.3e}", flush=True)
+    assert np.all(rel <= 1.0e6 * 2.0 ** -53)
+    assert c.trajectories == 1_000_000 and len(res["events"]) > 1000
+
+
 def test_capture_workload_full_size(gpu_ctx, oracle_solar):
     """The capture-mode workload at its full size: SI contact m = 1 GeV,
     sigma_p = 1e-35 cm^2, 10^4 trajectories, at most 1000 scatterings, 1000 x
