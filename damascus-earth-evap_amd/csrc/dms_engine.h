@@ -1137,9 +1137,12 @@ DMS_HD int dms_step_prep(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 	return -1;
 }
 
-/* Loop body after the RKF45 step (:921-1115).  Returns -2 to run PREP again,
- * -3 when the segment ended in a scattering (Scatter follows), or the
- * Propagate_Freely return value for every other exit. */
+/* Loop body after the RKF45 step (:921-1115).  Returns DMS_POST_NEXT when the
+ * next step starts from the propagator state the attempt left (the common
+ * exit), -2 to run PREP again from another state (optical-depth rejection,
+ * Kepler return), -3 when the segment ended in a scattering (Scatter
+ * follows), or the Propagate_Freely return value for every other exit. */
+#define DMS_POST_NEXT (-4)
 template <bool MX = false>
 DMS_HD int dms_step_post(DmsLane& L, const DmsPointView& P, const DmsSolarView& S, const DmsJobView& J, const DmsRows& R)
 {
@@ -1515,7 +1518,7 @@ DMS_HD int dms_step_post(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 	L.rb_cached = want_next ? 1 : 0;
 	L.rate_before = rate_next;
 	L.rate_before_f = rate_next_f;
-	return -2;
+	return DMS_POST_NEXT;
 #undef DMS_RETURN_AT
 }
 
@@ -1948,10 +1951,9 @@ struct DmsLaneCtx
 	DmsRows R;
 };
 
-/* RK attempt, step post-processing and the scatter hand-off (r, v and the
- * Sample_Target uniform are drawn here, before the wave-level target choice). */
+/* One RKF45 attempt (PH_RK; PH_POST once accepted). */
 template <bool MX = false>
-DMS_HD void dms_lane_advance_a(DmsLane& L, const DmsLaneCtx& c)
+DMS_HD void dms_lane_advance_rk(DmsLane& L, const DmsLaneCtx& c)
 {
 	if(L.phase == PH_RK)
 	{
@@ -1961,20 +1963,39 @@ DMS_HD void dms_lane_advance_a(DmsLane& L, const DmsLaneCtx& c)
 		L.rk_ok = (r >= 0) ? r : L.rk_ok;
 		L.phase = (r >= 0) ? (int) PH_POST : L.phase;
 	}
+}
+
+/* Step post-processing and the scatter hand-off (r, v and the Sample_Target
+ * uniform are drawn here, before the wave-level target choice).  *post_ret
+ * receives dms_step_post's value (0 when no post ran). */
+template <bool MX = false>
+DMS_HD void dms_lane_advance_post(DmsLane& L, const DmsLaneCtx& c, int* post_ret = nullptr)
+{
+	int pr = 0;
 	if(L.phase == PH_POST)
 	{
-		const int pr = dms_step_post<MX>(L, c.P, c.S, c.J, c.R);
-		if(pr == -2)
+		pr = dms_step_post<MX>(L, c.P, c.S, c.J, c.R);
+		if((pr == -2) | (pr == DMS_POST_NEXT))
 			L.phase = PH_PREP;
 		else
 			L.phase = dms_after_propagation(L, (pr == -3) ? (int) DMS_SCATTER : pr);
 	}
+	if(post_ret)
+		*post_ret = pr;
 	if(L.phase == PH_SCAT)
 	{
 		L.sc_r = dms_norm(L.ev_r);
 		L.sc_v = dms_norm(L.ev_v);
 		L.sc_xi = dm_uniform(&L.rng);
 	}
+}
+
+/* RK attempt, step post-processing and the scatter hand-off. */
+template <bool MX = false>
+DMS_HD void dms_lane_advance_a(DmsLane& L, const DmsLaneCtx& c)
+{
+	dms_lane_advance_rk<MX>(L, c);
+	dms_lane_advance_post<MX>(L, c);
 }
 
 /* Rest of Scatter once sc_target holds the Sample_Target result. */
@@ -2059,6 +2080,169 @@ DMS_HD void dms_lane_advance_b(DmsLane& L, const DmsLaneCtx& c)
 		const int pr = dms_step_prep<MX>(L, c.P, c.S, c.J);
 		L.phase = (pr < 0) ? (int) PH_RK : dms_after_propagation(L, pr);
 	}
+}
+
+
+/* ---- a lone trajectory split over two waves --------------------------------
+ * A trajectory that runs alone in its wave executes one instruction stream:
+ * each step is the step preparation, the RKF45 attempts, then the step's
+ * post-processing (optical depth, bincount, capture state).  The preparation
+ * of the next step depends on the post only through the state the post leaves
+ * when it continues (DMS_POST_NEXT): the committed event is the propagator's,
+ * r_before and v_before are its radius and speed, and the next step's rate is
+ * Total_DM_Scattering_Rate at that point.  A second wave of the block (the
+ * helper) therefore computes the next step's preparation and RKF45 attempts
+ * from the propagator state alone, assuming the post continues, while the
+ * trajectory's own wave runs the post; the post then takes the helper's step
+ * (dms_pipe_take) instead of computing it.  Every other post exit (an
+ * optical-depth rejection, a Kepler return, a scattering, an end) discards the
+ * helper's steps and the trajectory's wave goes on from its own state.  The
+ * helper runs the same functions on the same values, so the trajectory is the
+ * serial one bit for bit. */
+#define DMS_PIPE_STEP 0     /* an entry holds a prepared, accepted step */
+#define DMS_PIPE_PREP_END 1 /* the step preparation ended the segment */
+
+/* Session start: the propagator after the accepted RKF45 attempt of the step
+ * whose post the trajectory's wave is about to run. */
+struct DmsPipeSeed
+{
+	DmsProp prop;
+	uint64_t time_steps; /* before that post */
+	uint64_t rk_all;
+	double time_origin;
+	uint32_t job;
+	uint32_t pad;
+};
+
+/* One step prepared by the helper: what dms_step_prep and the RKF45 attempts
+ * change in the lane state. */
+struct DmsPipeEntry
+{
+	double time, radius, phi, vr, ts;                /* propagator after the accepted attempt */
+	double pb_time, pb_radius, pb_phi, pb_vr, pb_ts; /* the step's start, as the preparation left it */
+	double rate_before;
+	uint64_t rk_all;
+	float rate_before_f;
+	int32_t rk_ok, inner_iter;
+	uint32_t session;
+	int32_t kind;
+	int32_t pad;
+};
+
+DMS_HD void dms_pipe_make_seed(const DmsLane& L, uint32_t job, DmsPipeSeed& s)
+{
+	s.prop = L.prop;
+	s.time_steps = L.time_steps;
+	s.rk_all = L.rk_all;
+	s.time_origin = L.time_origin;
+	s.job = job;
+	s.pad = 0;
+}
+
+DMS_HD void dms_pipe_from_seed(DmsLane& A, const DmsPipeSeed& s)
+{
+	A.prop = s.prop;
+	A.time_steps = s.time_steps;
+	A.rk_all = s.rk_all;
+	A.time_origin = s.time_origin;
+}
+
+/* The speed dms_prop_event_speed returns, without the event. */
+DMS_HD double dms_prop_speed_as_event(const DmsProp& p)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+	if(__builtin_expect(!p.radial & dms_in_binades(p.radius, 63, 132) & dms_in_binades(dm_fabs(p.J), -99, 165), 1))
+	{
+		const DmRecip R = dm_recip_shared(p.radius);
+		return dm_sqrt(p.vr * p.vr + dm_div_shared(dm_div_shared(p.J * p.J, R), R));
+	}
+#endif
+	return dms_prop_speed(p);
+}
+
+/* Helper side: the updates of dms_step_post's DMS_POST_NEXT exit that the next
+ * dms_step_prep reads (the event is needed only outside the Sun; the rate at
+ * the new point is recomputed by the preparation: Total_DM_Scattering_Rate
+ * there is the value the post's end-point query caches). */
+template <bool MX = false>
+DMS_HD void dms_pipe_assume_next(DmsLane& A)
+{
+	const double r_after = dms_prop_radius(A.prop);
+	double v_after;
+	if(r_after >= DMS_rSun)
+	{
+		double t;
+		v_after = MX ? dms_prop_event_mx(A.prop, t, A.eb_r, A.eb_v) : dms_prop_event_speed(A.prop, t, A.eb_r, A.eb_v);
+		A.eb_t = t;
+	}
+	else
+		v_after = MX ? dms_prop_speed_mx(A.prop) : dms_prop_speed_as_event(A.prop);
+	A.r_before = r_after;
+	A.v_before = v_after;
+	A.eb_valid = 1;
+	A.rb_cached = 0;
+	A.time_steps++;
+	A.od_retries = 0;
+}
+
+/* Helper side: the next step's preparation and RKF45 attempts (as
+ * dms_lane_advance_b and dms_lane_advance_a run them) into e.  Returns false
+ * when the preparation ends the segment (e.kind DMS_PIPE_PREP_END). */
+template <bool MX = false>
+DMS_HD bool dms_pipe_prepare_step(DmsLane& A, const DmsPointView& P, const DmsSolarView& S, const DmsJobView& J, DmsPipeEntry& e)
+{
+	if(dms_step_prep<MX>(A, P, S, J) >= 0)
+	{
+		e.kind = DMS_PIPE_PREP_END;
+		return false;
+	}
+	int r;
+	do
+	{
+		A.inner_iter++;
+		A.rk_all++;
+		r = MX ? dms_rk45_attempt_mx(A.prop, S, A.inner_iter) : dms_rk45_attempt(A.prop, S, A.inner_iter);
+	} while(r < 0);
+	e.time = A.prop.time;
+	e.radius = A.prop.radius;
+	e.phi = A.prop.phi;
+	e.vr = A.prop.vr;
+	e.ts = A.prop.ts;
+	e.pb_time = A.pb_time;
+	e.pb_radius = A.pb_radius;
+	e.pb_phi = A.pb_phi;
+	e.pb_vr = A.pb_vr;
+	e.pb_ts = A.pb_ts;
+	e.rate_before = A.rate_before;
+	e.rate_before_f = A.rate_before_f;
+	e.rk_all = A.rk_all;
+	e.rk_ok = r;
+	e.inner_iter = A.inner_iter;
+	e.kind = DMS_PIPE_STEP;
+	return true;
+}
+
+/* Trajectory side, after its post returned DMS_POST_NEXT: take the helper's
+ * step as if dms_step_prep and the RKF45 attempts had run here.  The fields the
+ * preparation would also write (eb, r_before, v_before, rb_cached) already hold
+ * the values it would write, from the post. */
+DMS_HD void dms_pipe_take(DmsLane& L, const DmsPipeEntry& e)
+{
+	L.prop.time = e.time;
+	L.prop.radius = e.radius;
+	L.prop.phi = e.phi;
+	L.prop.vr = e.vr;
+	L.prop.ts = e.ts;
+	L.pb_time = e.pb_time;
+	L.pb_radius = e.pb_radius;
+	L.pb_phi = e.pb_phi;
+	L.pb_vr = e.pb_vr;
+	L.pb_ts = e.pb_ts;
+	L.rate_before = e.rate_before;
+	L.rate_before_f = e.rate_before_f;
+	L.rk_all = e.rk_all;
+	L.rk_ok = e.rk_ok;
+	L.inner_iter = e.inner_iter;
 }
 
 #endif

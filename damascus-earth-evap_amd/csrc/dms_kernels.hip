@@ -283,6 +283,120 @@ __device__ __forceinline__ void dms_serve_refill(const DmsTrajArgs& a, DmsLaneCt
 		poll.idle = 0;
 }
 
+/* ---- two-wave split of a lone trajectory ----------------------------------- */
+
+__device__ __forceinline__ unsigned int dms_lds_load(const unsigned int* p)
+{
+	return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ void dms_lds_store(unsigned int* p, unsigned int v)
+{
+	__hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+/* Spin limits (polls of at least 64 cycles each): a trajectory's wave that
+ * waits this long for a helper's step gives the split up and goes on alone. */
+#define DMS_PIPE_WAIT_POLLS (1u << 22)
+
+/* A helper wave serving slot sl (its lane 0 computes): for each session the
+ * trajectory's wave opens, it prepares the following steps, assuming each
+ * post continues, and writes them to the ring.  Returns when the trajectory's
+ * wave has finished all its lanes (want = 0). */
+template <bool MX, bool MP>
+__device__ __forceinline__ void dms_pipe_serve(const DmsTrajArgs& a, const DmsSolarView& S, DmsPipeSlot* sl, int lane)
+{
+	if(lane != 0)
+		return;
+	DmsLane A;
+	DmsPointView P = a.P;
+	unsigned int cur = 0, head = dms_lds_load(&sl->head);
+	bool ended = true;
+	for(;;)
+	{
+		if(dms_lds_load(&sl->want) == 0u)
+			return;
+		const unsigned int s = dms_lds_load(&sl->session);
+		if(s != cur)
+		{
+			cur = s;
+			ended = (s == 0u);
+			if(!ended)
+			{
+				DmsPipeSeed seed = sl->seed;
+				dms_pipe_from_seed(A, seed);
+				if constexpr(MP)
+					P = a.job_points[seed.job];
+				/* the seed was read after the session word: if the session
+				 * moved meanwhile, the next pass starts over */
+				if(dms_lds_load(&sl->session) != s)
+				{
+					cur = 0u;
+					continue;
+				}
+			}
+		}
+		if(ended)
+		{
+			__builtin_amdgcn_s_sleep(2);
+			continue;
+		}
+		DmsPipeEntry e;
+		dms_pipe_assume_next<MX>(A);
+		ended = !dms_pipe_prepare_step<MX>(A, P, S, a.J, e);
+		e.session = cur;
+		/* wait for a free ring entry */
+		bool stale = false;
+		while(head - dms_lds_load(&sl->tail) >= (unsigned int) DMS_PIPE_DEPTH)
+		{
+			if((dms_lds_load(&sl->session) != cur) | (dms_lds_load(&sl->want) == 0u))
+			{
+				stale = true;
+				break;
+			}
+			__builtin_amdgcn_s_sleep(1);
+		}
+		if(stale || dms_lds_load(&sl->session) != cur)
+			continue;
+		sl->e[head % DMS_PIPE_DEPTH] = e;
+		head++;
+		dms_lds_store(&sl->head, head);
+	}
+}
+
+/* A wave whose lanes have all finished: serve lone trajectories of the other
+ * waves of the block until every wave of the block has finished. */
+template <bool MX, bool MP>
+__device__ __forceinline__ void dms_pipe_idle(const DmsTrajArgs& a, const DmsSolarView& S, DmsPipeBlock* pb, int wave, int lane)
+{
+	const int nwaves = (int) (blockDim.x >> 6);
+	for(;;)
+	{
+		if(dms_lds_load(&pb->waves_done) >= (unsigned int) nwaves)
+			return;
+		int claimed = -1;
+		if(lane == 0)
+			for(int w = 0; w < nwaves; w++)
+			{
+				DmsPipeSlot* sl = &pb->slot[w];
+				if(w == wave || dms_lds_load(&sl->want) == 0u || dms_lds_load(&sl->helper) != 0u)
+					continue;
+				unsigned int expect = 0u;
+				if(__hip_atomic_compare_exchange_strong(&sl->helper, &expect, (unsigned int) (wave + 1), __ATOMIC_ACQ_REL,
+				                                        __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP))
+				{
+					claimed = w;
+					break;
+				}
+			}
+		claimed = __builtin_amdgcn_readfirstlane(claimed);
+		if(claimed >= 0)
+			dms_pipe_serve<MX, MP>(a, S, &pb->slot[claimed], lane);
+		else
+			__builtin_amdgcn_s_sleep(8);
+	}
+}
+
 template <bool MX, bool MP, bool SV = false>
 __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 {
@@ -301,6 +415,19 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 	{
 		lmass[i] = a.S.mass[i];
 		lvesc[i] = a.S.vesc2[i];
+	}
+	/* the two-wave split: a lone trajectory's wave and a finished wave of the
+	 * block prepare its steps together (not in a serving launch, whose lanes
+	 * wait for published slots instead of finishing) */
+	constexpr bool PIPE = !SV;
+	DmsPipeBlock* pb = reinterpret_cast<DmsPipeBlock*>(dms_lds + dms_lds_pipe_offset(nn));
+	const int wave = (int) (threadIdx.x >> 6);
+	DmsPipeSlot* my = &pb->slot[wave];
+	if constexpr(PIPE)
+	{
+		unsigned int* w = reinterpret_cast<unsigned int*>(pb);
+		for(int i = threadIdx.x; i < (int) (sizeof(DmsPipeBlock) / sizeof(unsigned int)); i += blockDim.x)
+			w[i] = 0u;
 	}
 	__syncthreads();
 	DmsLaneCtx c;
@@ -328,9 +455,43 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 	DmsSummary sv_sum;
 	DmsServePoll sv_poll = {0u, 0u};
 	uint32_t cancel_tick = 0;
+	/* two-wave split state (wave-uniform): lone = one live lane, the others
+	 * finished; attached = a helper serves this wave; session = the open
+	 * session (0: none); tail = ring entries read */
+	bool lone = false, attached = false;
+	bool split_ok = PIPE && a.lone_serial == 0u;
+	unsigned int session = 0u, sessions = 0u, tail = 0u;
 	for(;;)
 	{
-		dms_lane_advance_a<MX>(L, c);
+		dms_lane_advance_rk<MX>(L, c);
+		if constexpr(PIPE)
+		{
+			if(__builtin_expect(lone & !attached & split_ok, 0))
+				attached = __builtin_amdgcn_readfirstlane(dms_lds_load(&my->helper)) != 0u;
+			/* the lone lane accepted an attempt with no session open: the helper
+			 * prepares the steps after this one from here */
+			if(__builtin_expect(attached & (session == 0u), 0) && __ballot(L.phase == PH_POST) != 0)
+			{
+				session = ++sessions;
+				if(L.phase == PH_POST)
+				{
+					dms_pipe_make_seed(L, L.job, my->seed);
+					dms_lds_store(&my->session, session);
+				}
+			}
+		}
+		int post_ret;
+		dms_lane_advance_post<MX>(L, c, &post_ret);
+		if constexpr(PIPE)
+		{
+			/* the session lasts while the posts continue to the next step */
+			if(__builtin_expect(session != 0u, 0)
+			   && __ballot((L.phase == PH_PREP) & (post_ret == DMS_POST_NEXT)) == 0)
+			{
+				session = 0u;
+				dms_lds_store(&my->session, 0u);
+			}
+		}
 		/* The wave-level services below (target sampling, scatter completion,
 		 * capture counting, trajectory end, row flush, refill, start) act only
 		 * on lanes in those phases: while every live lane is between RKF45
@@ -565,6 +726,62 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 					                   c.J.terminate_on_capture);
 			}
 			} /* not a serving launch */
+			if constexpr(PIPE)
+			{
+				/* one live lane left and every other lane finished for good (no
+				 * slot left): ask for a helper wave */
+				if(split_ok & !lone && __popcll(__ballot(L.phase != PH_DONE)) == 1)
+				{
+					lone = true;
+					if(lane == 0)
+						dms_lds_store(&my->want, 1u);
+				}
+			}
+		}
+		if constexpr(PIPE)
+		{
+			/* the next step of the open session: the helper's preparation and
+			 * attempts take the place of this lane's (the post runs next) */
+			if(__builtin_expect(session != 0u, 0))
+			{
+				bool got = false, timed_out = true;
+				for(unsigned int polls = 0; polls < DMS_PIPE_WAIT_POLLS; polls++)
+				{
+					const unsigned int h = __builtin_amdgcn_readfirstlane(dms_lds_load(&my->head));
+					if(h == tail)
+					{
+						__builtin_amdgcn_s_sleep(1);
+						continue;
+					}
+					const DmsPipeEntry* ep = &my->e[tail % DMS_PIPE_DEPTH];
+					const unsigned int es = __builtin_amdgcn_readfirstlane(ep->session);
+					const int kind = __builtin_amdgcn_readfirstlane(ep->kind);
+					if((es == session) & (kind == DMS_PIPE_STEP) & (L.phase == PH_PREP))
+					{
+						dms_pipe_take(L, *ep);
+						L.phase = PH_POST;
+					}
+					tail++;
+					if(lane == 0)
+						dms_lds_store(&my->tail, tail);
+					if(es == session)
+					{
+						/* a preparation that ends the segment: this lane runs its
+						 * own, which ends it the same way */
+						got = kind == DMS_PIPE_STEP;
+						timed_out = false;
+						break;
+					}
+				}
+				if(!got)
+				{
+					session = 0u;
+					dms_lds_store(&my->session, 0u);
+					/* no answer from the helper: go on alone for good */
+					if(__builtin_expect(timed_out, 0))
+						split_ok = attached = false;
+				}
+			}
 		}
 		dms_lane_advance_b<MX>(L, c);
 		if constexpr(!SV)
@@ -582,6 +799,18 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 		}
 		if(__ballot(L.phase != PH_DONE) == 0)
 			break;
+	}
+	if constexpr(PIPE)
+	{
+		/* this wave's lanes have all finished: release a helper, then help the
+		 * block's other lone trajectories until every wave has finished */
+		if(lane == 0)
+		{
+			dms_lds_store(&my->want, 0u);
+			__hip_atomic_fetch_add(&pb->waves_done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+		}
+		if(a.lone_serial == 0u)
+			dms_pipe_idle<MX, MP>(a, c.S, pb, wave, lane);
 	}
 }
 

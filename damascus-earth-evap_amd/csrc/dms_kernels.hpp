@@ -11,9 +11,46 @@
 
 #define DMS_TRAJ_BLOCK 256
 
-/* Dynamic LDS of dms_traj_kernel: nodes x[n], then Mass and v_esc^2 cubics. */
+#define DMS_TRAJ_WAVES (DMS_TRAJ_BLOCK / 64)
+
+/* Two-wave split of a lone trajectory (dms_engine.h): one exchange area per
+ * wave of a block, in LDS.  The trajectory's wave (the only live lane of its
+ * wave) asks for a helper (want); a wave of the block whose lanes have all
+ * finished claims it (helper, by compare-and-swap) and serves it until that
+ * wave's lanes have all finished too.  Each session (a step whose post the
+ * trajectory's wave is about to run) starts from seed; the helper writes the
+ * steps it prepares to the ring e[] (head, written by the helper) and the
+ * trajectory's wave reads them (tail); entries of an older session are
+ * skipped. */
+#define DMS_PIPE_DEPTH 2
+
+struct DmsPipeSlot
+{
+	unsigned int want;    /* 1 while the wave's lone trajectory would take a helper */
+	unsigned int helper;  /* helper wave + 1 once claimed, 0 = none */
+	unsigned int session; /* current session, 0 = none */
+	unsigned int head;    /* entries written (helper) */
+	unsigned int tail;    /* entries read (trajectory's wave) */
+	unsigned int pad[3];
+	DmsPipeSeed seed;
+	DmsPipeEntry e[DMS_PIPE_DEPTH];
+};
+
+struct DmsPipeBlock
+{
+	unsigned int waves_done; /* waves whose lanes have all finished */
+	unsigned int pad[3];
+	DmsPipeSlot slot[DMS_TRAJ_WAVES];
+};
+
+/* Dynamic LDS of dms_traj_kernel: nodes x[n], then Mass and v_esc^2 cubics,
+ * then the two-wave exchange areas. */
 __host__ __device__ inline size_t dms_lds_mass_offset(int n) { return ((size_t) n * sizeof(double) + 31) & ~(size_t) 31; }
-__host__ __device__ inline size_t dms_lds_bytes(int n) { return dms_lds_mass_offset(n) + 2 * (size_t) (n - 1) * sizeof(DmsCubic); }
+__host__ __device__ inline size_t dms_lds_pipe_offset(int n)
+{
+	return (dms_lds_mass_offset(n) + 2 * (size_t) (n - 1) * sizeof(DmsCubic) + 15) & ~(size_t) 15;
+}
+__host__ __device__ inline size_t dms_lds_bytes(int n) { return dms_lds_pipe_offset(n) + sizeof(DmsPipeBlock); }
 
 #define DMS_PROGRESS_WORDS 4
 
@@ -112,6 +149,8 @@ struct DmsTrajArgs
 	const unsigned int* cancel;
 	/* serving launch (dms_traj_kernel_serve*) */
 	DmsServeView V;
+	/* 1: a lone trajectory runs on its own wave only (no helper wave) */
+	uint32_t lone_serial;
 };
 
 __global__ void dms_traj_kernel(DmsTrajArgs a);

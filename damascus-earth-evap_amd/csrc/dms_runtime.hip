@@ -409,6 +409,7 @@ hipError_t run_trajectories(dms_ctx* ctx, hipStream_t st, hipStream_t pst, const
 	}
 	a.next = B.next;
 	a.refill_batch = ctx->knobs.refill_batch;
+	a.lone_serial = ctx->knobs.lone_serial;
 	a.n_jobs = (uint32_t) LJ.start.size();
 	a.job_start = LJ.d_start;
 	a.job_first_id = LJ.d_first;
@@ -2469,6 +2470,7 @@ int dms_simulate_events_noalloc(dms_point* pt, const dms_job_desc* job, const dm
 	a.R.hcounts = nullptr;
 	a.next = reinterpret_cast<unsigned int*>(s + kScratchNext);
 	a.refill_batch = 1;
+	a.lone_serial = ctx->knobs.lone_serial;
 	a.n_jobs = 1;
 	a.job_start = reinterpret_cast<const uint32_t*>(s + kScratchStart);
 	a.job_first_id = reinterpret_cast<const uint64_t*>(s + kScratchFirst);
@@ -2692,8 +2694,9 @@ int dms_set_launch_knobs(dms_ctx* ctx, const dms_launch_knobs* knobs)
 		return fail(ctx, DMS_ERR_ARG, "dms_set_launch_knobs: null argument");
 	if(knobs->refill_batch < 1 || knobs->refill_batch > 64)
 		return fail(ctx, DMS_ERR_ARG, "dms_set_launch_knobs: refill_batch must be in [1, 64]");
+	if(knobs->lone_serial > 1)
+		return fail(ctx, DMS_ERR_ARG, "dms_set_launch_knobs: lone_serial must be 0 or 1");
 	ctx->knobs = *knobs;
-	ctx->knobs.pad = 0;
 	return DMS_OK;
 }
 
@@ -2742,7 +2745,7 @@ int dms_autotune_knobs(dms_point* const* points, const dms_job_desc* descs, size
 		if(t[i] < 0.97 * t[0] && t[i] < t[b])
 			b = i;
 	best->refill_batch = refill_candidates[b];
-	best->pad = 0;
+	best->lone_serial = saved.lone_serial;
 	if(seconds)
 		for(size_t i = 0; i < n_candidates; i++)
 			seconds[i] = t[i];

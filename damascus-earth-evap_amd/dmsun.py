@@ -102,7 +102,7 @@ class EvapEvent(C.Structure):
 
 
 class LaunchKnobs(C.Structure):
-    _fields_ = [("refill_batch", C.c_uint32), ("pad", C.c_uint32)]
+    _fields_ = [("refill_batch", C.c_uint32), ("lone_serial", C.c_uint32)]
 
 
 class ServerDesc(C.Structure):
@@ -321,14 +321,26 @@ class Context:
     def point(self, desc):
         return Point(self, desc)
 
-    def set_refill_batch(self, k):
-        """dms_set_launch_knobs: free lanes a wave waits for before it refills."""
-        self.check(lib().dms_set_launch_knobs(self.h, C.byref(LaunchKnobs(k, 0))), "dms_set_launch_knobs")
-
-    def refill_batch(self):
+    def knobs(self):
         k = LaunchKnobs()
         self.check(lib().dms_get_launch_knobs(self.h, C.byref(k)), "dms_get_launch_knobs")
-        return k.refill_batch
+        return k
+
+    def set_refill_batch(self, k):
+        """dms_set_launch_knobs: free lanes a wave waits for before it refills."""
+        kn = self.knobs()
+        kn.refill_batch = k
+        self.check(lib().dms_set_launch_knobs(self.h, C.byref(kn)), "dms_set_launch_knobs")
+
+    def refill_batch(self):
+        return self.knobs().refill_batch
+
+    def set_lone_serial(self, on):
+        """dms_set_launch_knobs: 1 runs a trajectory left alone in its wave on
+        that wave only; 0 (default) splits its steps with a finished wave."""
+        kn = self.knobs()
+        kn.lone_serial = 1 if on else 0
+        self.check(lib().dms_set_launch_knobs(self.h, C.byref(kn)), "dms_set_launch_knobs")
 
     def set_job_memory_budget(self, nbytes):
         """Device bytes one job's trajectory buffers may hold (default 32 GiB);

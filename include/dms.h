@@ -358,11 +358,16 @@ int dms_multi_autotune(dms_point* const* points, const dms_job_desc* descs, size
  * refill_batch: free lanes a wave waits for before it takes new slots (1 = at
  * once, up to 64), so that trajectory starts run together instead of one lane
  * at a time; a wave with no lane left running, or a launch whose slots are all
- * handed out, refills at once.  No reference counterpart. */
+ * handed out, refills at once.
+ * lone_serial: 0 (default) lets a trajectory left alone in its wave (every
+ * other lane finished, no slot left) share its steps with a finished wave of
+ * its block, which prepares the next step while the trajectory's wave
+ * post-processes the current one; 1 runs it on its own wave only.  No
+ * reference counterpart. */
 typedef struct
 {
 	uint32_t refill_batch;
-	uint32_t pad;
+	uint32_t lone_serial;
 } dms_launch_knobs;
 
 int dms_set_launch_knobs(dms_ctx* ctx, const dms_launch_knobs* knobs);

@@ -170,11 +170,70 @@ void rp_initial_conditions(rp_ctx* c, const dms_halo* h, uint32_t seed, uint32_t
 	}
 }
 
+extern "C++" {
+/* The wave-cooperative Sample_Target of the kernel, serially: the same
+ * per-target formulas and the same summation and comparison order (FP64:
+ * dms_choose_target; mixed: FP32 rates and reciprocal of the total). */
+template <bool MX>
+static void sample_target_serial(DmsLane& L, const DmsLaneCtx& c)
+{
+	if constexpr(!MX)
+	{
+		double rates[DMS_MAX_TARGETS];
+		for(int k = 0; k < DMS_MAX_TARGETS; k++)
+			rates[k] = 0.0;
+		for(int k = 0; k < c.S.n_iso; k++)
+			rates[k] = dms_rate_target(c.P, c.S, L.sc_r, L.sc_v, k);
+		rates[DMS_MAX_TARGETS - 1] = dms_rate_target(c.P, c.S, L.sc_r, L.sc_v, DMS_MAX_TARGETS - 1);
+		L.sc_target = dms_choose_target(rates, c.S.n_iso, L.sc_xi);
+	}
+	else
+	{
+		float rates[DMS_MAX_TARGETS];
+		int target = -2;
+		bool bad = false;
+		for(int k = 0; k < DMS_MAX_TARGETS; k++)
+		{
+			rates[k] = 0.0f;
+			if(k < c.S.n_iso || k == DMS_MAX_TARGETS - 1)
+			{
+				rates[k] = dms_rate_target_mx(c.P, c.S, L.sc_r, L.sc_v, k);
+				bad = bad || !mx_isfinite(rates[k]) || rates[k] < 0.0f;
+			}
+		}
+		if(!bad)
+		{
+			float total = rates[DMS_MAX_TARGETS - 1];
+			for(int i = 0; i < c.S.n_iso; i++)
+				total += rates[i];
+			if(mx_isfinite(total) && total > 0.0f)
+			{
+				const float xf = (float) L.sc_xi;
+				float sum = rates[DMS_MAX_TARGETS - 1] * mx_rcp(total);
+				if(sum > xf)
+					target = -1;
+				else
+					for(int i = 0; i < c.S.n_iso; i++)
+					{
+						sum += rates[i] * mx_rcp(total);
+						if(sum > xf || i == c.S.n_iso - 1)
+						{
+							target = i;
+							break;
+						}
+					}
+			}
+		}
+		L.sc_target = target;
+	}
+}
+
+} /* extern "C++" */
+
 /* One trajectory through the kernel's phase drivers, with the wave-cooperative
  * Sample_Target done serially (same per-target formulas, same sum order). */
 static void replay_lane(DmsLane& L, const DmsLaneCtx& c, const dms_event& ic, bool shift_ok, dms_traj_record* rec, DmsSummary* sum)
 {
-	double rates[DMS_MAX_TARGETS];
 	dms_lane_start(L, c, ic, shift_ok, rec, sum);
 	if(L.phase == PH_NEED)
 		return;
@@ -183,14 +242,7 @@ static void replay_lane(DmsLane& L, const DmsLaneCtx& c, const dms_event& ic, bo
 	{
 		dms_lane_advance_a(L, c);
 		if(L.phase == PH_SCAT)
-		{
-			for(int k = 0; k < DMS_MAX_TARGETS; k++)
-				rates[k] = 0.0;
-			for(int k = 0; k < c.S.n_iso; k++)
-				rates[k] = dms_rate_target(c.P, c.S, L.sc_r, L.sc_v, k);
-			rates[DMS_MAX_TARGETS - 1] = dms_rate_target(c.P, c.S, L.sc_r, L.sc_v, DMS_MAX_TARGETS - 1);
-			L.sc_target = dms_choose_target(rates, c.S.n_iso, L.sc_xi);
-		}
+			sample_target_serial<false>(L, c);
 		dms_lane_finish_scatter(L, c);
 		if(L.phase == PH_FIN)
 		{
@@ -205,7 +257,6 @@ static void replay_lane(DmsLane& L, const DmsLaneCtx& c, const dms_event& ic, bo
  * kernel's summation order). */
 static void replay_lane_mx(DmsLane& L, const DmsLaneCtx& c, const dms_event& ic, bool shift_ok, dms_traj_record* rec, DmsSummary* sum)
 {
-	float rates[DMS_MAX_TARGETS];
 	dms_lane_start(L, c, ic, shift_ok, rec, sum);
 	if(L.phase == PH_NEED)
 		return;
@@ -214,43 +265,7 @@ static void replay_lane_mx(DmsLane& L, const DmsLaneCtx& c, const dms_event& ic,
 	{
 		dms_lane_advance_a<true>(L, c);
 		if(L.phase == PH_SCAT)
-		{
-			int target = -2;
-			bool bad = false;
-			for(int k = 0; k < DMS_MAX_TARGETS; k++)
-			{
-				rates[k] = 0.0f;
-				if(k < c.S.n_iso || k == DMS_MAX_TARGETS - 1)
-				{
-					rates[k] = dms_rate_target_mx(c.P, c.S, L.sc_r, L.sc_v, k);
-					bad = bad || !mx_isfinite(rates[k]) || rates[k] < 0.0f;
-				}
-			}
-			if(!bad)
-			{
-				float total = rates[DMS_MAX_TARGETS - 1];
-				for(int i = 0; i < c.S.n_iso; i++)
-					total += rates[i];
-				if(mx_isfinite(total) && total > 0.0f)
-				{
-					const float xf = (float) L.sc_xi;
-					float sum = rates[DMS_MAX_TARGETS - 1] * mx_rcp(total);
-					if(sum > xf)
-						target = -1;
-					else
-						for(int i = 0; i < c.S.n_iso; i++)
-						{
-							sum += rates[i] * mx_rcp(total);
-							if(sum > xf || i == c.S.n_iso - 1)
-							{
-								target = i;
-								break;
-							}
-						}
-				}
-			}
-			L.sc_target = target;
-		}
+			sample_target_serial<true>(L, c);
 		dms_lane_finish_scatter<true>(L, c);
 		if(L.phase == PH_FIN)
 		{
@@ -350,6 +365,130 @@ int rp_simulate(rp_point* p, const dms_job_desc* job, const dms_event* ics, size
 			replay_lane_mx(L, c, ics[i], true, &recs[i], nullptr);
 		else
 			replay_lane(L, c, ics[i], true, &recs[i], nullptr);
+	}
+	return 0;
+}
+
+extern "C++" {
+/* The two-wave split of a lone trajectory, serially: from the step whose
+ * RKF45 attempts the trajectory's own lane has just run, a helper lane (all
+ * fields it is not given hold garbage) prepares up to `depth` further steps
+ * assuming each post continues, and the post takes them (dms_pipe_take);
+ * every other post exit discards them.  Returns the last post's value, or
+ * DMS_POST_NEXT when the helper's preparation ended the segment (the lane
+ * then runs its own preparation, which ends it the same way).  *taken counts
+ * the steps taken from the helper, *dropped the ones discarded. */
+template <bool MX>
+static int pipe_session(DmsLane& L, const DmsLaneCtx& c, int depth, uint64_t* taken, uint64_t* dropped)
+{
+	int r;
+	do
+	{
+		L.inner_iter++;
+		L.rk_all++;
+		r = MX ? dms_rk45_attempt_mx(L.prop, c.S, L.inner_iter) : dms_rk45_attempt(L.prop, c.S, L.inner_iter);
+		L.rk_ok = (r >= 0) ? r : L.rk_ok;
+	} while(r < 0);
+	DmsPipeSeed seed;
+	dms_pipe_make_seed(L, L.job, seed);
+	DmsLane A;
+	memset(&A, 0xC3, sizeof(A));
+	dms_pipe_from_seed(A, seed);
+	std::vector<DmsPipeEntry> ring;
+	size_t head = 0;
+	bool helper_ended = false;
+	for(;;)
+	{
+		while(!helper_ended && ring.size() - head < (size_t) depth)
+		{
+			DmsPipeEntry e;
+			dms_pipe_assume_next<MX>(A);
+			if(!dms_pipe_prepare_step<MX>(A, c.P, c.S, c.J, e))
+				helper_ended = true;
+			ring.push_back(e);
+		}
+		const int pr = dms_step_post<MX>(L, c.P, c.S, c.J, c.R);
+		if(pr != DMS_POST_NEXT)
+		{
+			*dropped += ring.size() - head;
+			return pr;
+		}
+		const DmsPipeEntry e = ring[head++];
+		if(e.kind == DMS_PIPE_PREP_END)
+		{
+			*dropped += ring.size() - head;
+			return DMS_POST_NEXT;
+		}
+		dms_pipe_take(L, e);
+		(*taken)++;
+	}
+}
+
+template <bool MX>
+static void replay_lane_pipe(DmsLane& L, const DmsLaneCtx& c, const dms_event& ic, bool shift_ok, dms_traj_record* rec,
+                             DmsSummary* sum, int depth, uint64_t* taken, uint64_t* dropped)
+{
+	dms_lane_start(L, c, ic, shift_ok, rec, sum);
+	if(L.phase == PH_NEED)
+		return;
+	dms_lane_advance_b<MX>(L, c);
+	while(L.phase != PH_NEED)
+	{
+		if(L.phase == PH_RK)
+		{
+			const int pr = pipe_session<MX>(L, c, depth, taken, dropped);
+			L.phase = ((pr == -2) | (pr == DMS_POST_NEXT)) ? (int) PH_PREP
+			                                                : dms_after_propagation(L, (pr == -3) ? (int) DMS_SCATTER : pr);
+		}
+		dms_lane_advance_a<MX>(L, c); /* the scatter hand-off when the session ended in one */
+		if(L.phase == PH_SCAT)
+			sample_target_serial<MX>(L, c);
+		dms_lane_finish_scatter<MX>(L, c);
+		if(L.phase == PH_FIN)
+		{
+			dms_lane_finish(L, c, ic, rec, sum);
+			break;
+		}
+		dms_lane_advance_b<MX>(L, c);
+	}
+}
+
+} /* extern "C++" */
+
+/* rp_simulate with every step loop run through the two-wave split;
+ * stats[0] steps taken from the helper, stats[1] steps discarded. */
+int rp_simulate_pipe(rp_point* p, const dms_job_desc* job, const dms_event* ics, size_t n, dms_traj_record* recs,
+                     double* rows, int depth, uint64_t* stats)
+{
+	DmsLaneCtx c;
+	c.S = p->ctx->T.view();
+	c.P = p->view;
+	DmsJobView& J = c.J;
+	fill_job_view(J, job);
+	c.R.rows = J.terminate_on_capture ? nullptr : rows;
+	c.R.counts = nullptr;
+	c.R.touched = nullptr;
+	c.R.per_lane = 0;
+	c.R.hist = nullptr;
+	c.R.hcounts = nullptr;
+	if(depth < 1)
+		return -1;
+	if(rows)
+		memset(rows, 0, sizeof(double) * 2 * (size_t) J.nbins * n);
+	stats[0] = stats[1] = 0;
+	for(size_t i = 0; i < n; i++)
+	{
+		DmsLane L;
+		memset(&L, 0, sizeof(L));
+		L.cur_bin = -1;
+		L.slot = (uint32_t) i;
+		L.row = (uint32_t) i;
+		L.job = 0;
+		L.id = J.first_id + i;
+		if(c.J.mixed)
+			replay_lane_pipe<true>(L, c, ics[i], true, &recs[i], nullptr, depth, &stats[0], &stats[1]);
+		else
+			replay_lane_pipe<false>(L, c, ics[i], true, &recs[i], nullptr, depth, &stats[0], &stats[1]);
 	}
 	return 0;
 }

@@ -47,6 +47,8 @@ def lib():
                                        C.c_size_t, _P(C.c_double), _P(C.c_uint64)]),
             "rp_simulate": (C.c_int, [vp, _P(dmsun.JobDesc), _P(dmsun.Event), C.c_size_t, _P(dmsun.TrajRecord),
                                       _P(C.c_double)]),
+            "rp_simulate_pipe": (C.c_int, [vp, _P(dmsun.JobDesc), _P(dmsun.Event), C.c_size_t, _P(dmsun.TrajRecord),
+                                           _P(C.c_double), C.c_int, _P(C.c_uint64)]),
             "rp_div_const": (None, [_P(C.c_double), C.c_size_t, C.c_double, C.c_double, _P(C.c_double)]),
         }
         for name, (res, args) in sig.items():
@@ -102,6 +104,18 @@ class ReplayPoint:
         rows = np.zeros((n, 2, job.n_bins)) if want_rows else None
         assert lib().rp_simulate(self.h, C.byref(job), events, n, recs, dmsun.dptr(rows) if want_rows else None) == 0
         return recs, rows
+
+    def simulate_events_pipe(self, job, events, depth, want_rows=False):
+        """simulate_events with every step loop split between the trajectory's
+        lane and a helper lane that prepares up to `depth` steps ahead; returns
+        (records, rows, steps taken from the helper, steps discarded)."""
+        n = len(events)
+        recs = dmsun.record_array(n)
+        rows = np.zeros((n, 2, job.n_bins)) if want_rows else None
+        stats = np.zeros(2, dtype=np.uint64)
+        assert lib().rp_simulate_pipe(self.h, C.byref(job), events, n, recs, dmsun.dptr(rows) if want_rows else None,
+                                      depth, stats.ctypes.data_as(_P(C.c_uint64))) == 0
+        return recs, rows, int(stats[0]), int(stats[1])
 
     def lane_hist(self, job, events, ok, nlanes=3):
         """Class sums through per-lane scratch rows (serial, id order)."""
