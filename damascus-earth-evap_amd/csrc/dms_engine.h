@@ -1052,6 +1052,19 @@ DMS_HD int dms_segment_start(DmsLane& L, const DmsSolarView& S, double dm_mass, 
 	return -1;
 }
 
+/* Event_In_3D of a propagator state and the speed the step post takes with it
+ * (dms_prop_event_speed, or dms_prop_event_mx in mixed precision). */
+struct DmsStepEvent
+{
+	double t, r[3], v[3], speed;
+};
+
+template <bool MX = false>
+DMS_HD void dms_step_event(const DmsProp& p, DmsStepEvent& ev)
+{
+	ev.speed = MX ? dms_prop_event_mx(p, ev.t, ev.r, ev.v) : dms_prop_event_speed(p, ev.t, ev.r, ev.v);
+}
+
 /* Top of the step loop (:876-918).  Returns -1 when an RKF45 step follows,
  * otherwise the termination reason. */
 template <bool MX = false>
@@ -1144,18 +1157,27 @@ DMS_HD int dms_step_prep(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
  * follows), or the Propagate_Freely return value for every other exit. */
 #define DMS_POST_NEXT (-4)
 template <bool MX = false>
-DMS_HD int dms_step_post(DmsLane& L, const DmsPointView& P, const DmsSolarView& S, const DmsJobView& J, const DmsRows& R)
+DMS_HD int dms_step_post(DmsLane& L, const DmsPointView& P, const DmsSolarView& S, const DmsJobView& J, const DmsRows& R,
+                         const DmsStepEvent* pre = nullptr)
 {
 	const double actual_dt = L.prop.time - L.pb_time;
 	const double r_after = dms_prop_radius(L.prop);
 	double ea_t, ea_r[3], ea_v[3];
 	double v_after;
-	if constexpr(MX)
+	if(pre) /* the event of this propagator state, computed beforehand (dms_step_event) */
+	{
+		ea_t = pre->t;
+		for(int c = 0; c < 3; c++)
+		{
+			ea_r[c] = pre->r[c];
+			ea_v[c] = pre->v[c];
+		}
+		v_after = pre->speed;
+	}
+	else if constexpr(MX)
 		v_after = dms_prop_event_mx(L.prop, ea_t, ea_r, ea_v);
 	else
-	{
 		v_after = dms_prop_event_speed(L.prop, ea_t, ea_r, ea_v);
-	}
 #define DMS_RETURN_AT(tt, rr, vv, code) \
 	do                                  \
 	{                                   \
@@ -1969,12 +1991,12 @@ DMS_HD void dms_lane_advance_rk(DmsLane& L, const DmsLaneCtx& c)
  * uniform are drawn here, before the wave-level target choice).  *post_ret
  * receives dms_step_post's value (0 when no post ran). */
 template <bool MX = false>
-DMS_HD void dms_lane_advance_post(DmsLane& L, const DmsLaneCtx& c, int* post_ret = nullptr)
+DMS_HD void dms_lane_advance_post(DmsLane& L, const DmsLaneCtx& c, int* post_ret = nullptr, const DmsStepEvent* pre = nullptr)
 {
 	int pr = 0;
 	if(L.phase == PH_POST)
 	{
-		pr = dms_step_post<MX>(L, c.P, c.S, c.J, c.R);
+		pr = dms_step_post<MX>(L, c.P, c.S, c.J, c.R, pre);
 		if((pr == -2) | (pr == DMS_POST_NEXT))
 			L.phase = PH_PREP;
 		else
@@ -2091,14 +2113,14 @@ DMS_HD void dms_lane_advance_b(DmsLane& L, const DmsLaneCtx& c)
  * when it continues (DMS_POST_NEXT): the committed event is the propagator's,
  * r_before and v_before are its radius and speed, and the next step's rate is
  * Total_DM_Scattering_Rate at that point.  A second wave of the block (the
- * helper) therefore computes the next step's preparation and RKF45 attempts
- * from the propagator state alone, assuming the post continues, while the
- * trajectory's own wave runs the post; the post then takes the helper's step
- * (dms_pipe_take) instead of computing it.  Every other post exit (an
- * optical-depth rejection, a Kepler return, a scattering, an end) discards the
- * helper's steps and the trajectory's wave goes on from its own state.  The
- * helper runs the same functions on the same values, so the trajectory is the
- * serial one bit for bit. */
+ * helper) therefore computes the next step's preparation, RKF45 attempts and
+ * end event from the propagator state alone, assuming the post continues,
+ * while the trajectory's own wave runs the post; the post then takes the
+ * helper's step (dms_pipe_take) instead of computing it, and its end event.
+ * Every other post exit (an optical-depth rejection, a Kepler return, a
+ * scattering, an end) discards the helper's steps and the trajectory's wave
+ * goes on from its own state.  The helper runs the same functions on the same
+ * values, so the trajectory is the serial one bit for bit. */
 #define DMS_PIPE_STEP 0     /* an entry holds a prepared, accepted step */
 #define DMS_PIPE_PREP_END 1 /* the step preparation ended the segment */
 
@@ -2115,7 +2137,8 @@ struct DmsPipeSeed
 };
 
 /* One step prepared by the helper: what dms_step_prep and the RKF45 attempts
- * change in the lane state. */
+ * change in the lane state, and the event of the propagator they leave (the
+ * step post's end event). */
 struct DmsPipeEntry
 {
 	double time, radius, phi, vr, ts;                /* propagator after the accepted attempt */
@@ -2127,6 +2150,7 @@ struct DmsPipeEntry
 	uint32_t session;
 	int32_t kind;
 	int32_t pad;
+	DmsStepEvent ev;
 };
 
 DMS_HD void dms_pipe_make_seed(const DmsLane& L, uint32_t job, DmsPipeSeed& s)
@@ -2147,42 +2171,47 @@ DMS_HD void dms_pipe_from_seed(DmsLane& A, const DmsPipeSeed& s)
 	A.time_origin = s.time_origin;
 }
 
-/* The speed dms_prop_event_speed returns, without the event. */
-DMS_HD double dms_prop_speed_as_event(const DmsProp& p)
-{
-#if defined(__HIP_DEVICE_COMPILE__)
-	if(__builtin_expect(!p.radial & dms_in_binades(p.radius, 63, 132) & dms_in_binades(dm_fabs(p.J), -99, 165), 1))
-	{
-		const DmRecip R = dm_recip_shared(p.radius);
-		return dm_sqrt(p.vr * p.vr + dm_div_shared(dm_div_shared(p.J * p.J, R), R));
-	}
-#endif
-	return dms_prop_speed(p);
-}
-
 /* Helper side: the updates of dms_step_post's DMS_POST_NEXT exit that the next
- * dms_step_prep reads (the event is needed only outside the Sun; the rate at
+ * dms_step_prep reads, from the event ev of the propagator state (the rate at
  * the new point is recomputed by the preparation: Total_DM_Scattering_Rate
  * there is the value the post's end-point query caches). */
-template <bool MX = false>
-DMS_HD void dms_pipe_assume_next(DmsLane& A)
+DMS_HD void dms_pipe_assume_next(DmsLane& A, const DmsStepEvent& ev)
 {
-	const double r_after = dms_prop_radius(A.prop);
-	double v_after;
-	if(r_after >= DMS_rSun)
+	A.eb_t = ev.t;
+	for(int c = 0; c < 3; c++)
 	{
-		double t;
-		v_after = MX ? dms_prop_event_mx(A.prop, t, A.eb_r, A.eb_v) : dms_prop_event_speed(A.prop, t, A.eb_r, A.eb_v);
-		A.eb_t = t;
+		A.eb_r[c] = ev.r[c];
+		A.eb_v[c] = ev.v[c];
 	}
-	else
-		v_after = MX ? dms_prop_speed_mx(A.prop) : dms_prop_speed_as_event(A.prop);
-	A.r_before = r_after;
-	A.v_before = v_after;
+	A.r_before = dms_prop_radius(A.prop);
+	A.v_before = ev.speed;
 	A.eb_valid = 1;
 	A.rb_cached = 0;
 	A.time_steps++;
 	A.od_retries = 0;
+}
+
+/* Helper side: the entry of a step whose attempts ended with r >= 0. */
+template <bool MX = false>
+DMS_HD void dms_pipe_fill_entry(const DmsLane& A, int r, DmsPipeEntry& e)
+{
+	e.time = A.prop.time;
+	e.radius = A.prop.radius;
+	e.phi = A.prop.phi;
+	e.vr = A.prop.vr;
+	e.ts = A.prop.ts;
+	e.pb_time = A.pb_time;
+	e.pb_radius = A.pb_radius;
+	e.pb_phi = A.pb_phi;
+	e.pb_vr = A.pb_vr;
+	e.pb_ts = A.pb_ts;
+	e.rate_before = A.rate_before;
+	e.rate_before_f = A.rate_before_f;
+	e.rk_all = A.rk_all;
+	e.rk_ok = r;
+	e.inner_iter = A.inner_iter;
+	e.kind = DMS_PIPE_STEP;
+	dms_step_event<MX>(A.prop, e.ev);
 }
 
 /* Helper side: the next step's preparation and RKF45 attempts (as
@@ -2203,22 +2232,7 @@ DMS_HD bool dms_pipe_prepare_step(DmsLane& A, const DmsPointView& P, const DmsSo
 		A.rk_all++;
 		r = MX ? dms_rk45_attempt_mx(A.prop, S, A.inner_iter) : dms_rk45_attempt(A.prop, S, A.inner_iter);
 	} while(r < 0);
-	e.time = A.prop.time;
-	e.radius = A.prop.radius;
-	e.phi = A.prop.phi;
-	e.vr = A.prop.vr;
-	e.ts = A.prop.ts;
-	e.pb_time = A.pb_time;
-	e.pb_radius = A.pb_radius;
-	e.pb_phi = A.pb_phi;
-	e.pb_vr = A.pb_vr;
-	e.pb_ts = A.pb_ts;
-	e.rate_before = A.rate_before;
-	e.rate_before_f = A.rate_before_f;
-	e.rk_all = A.rk_all;
-	e.rk_ok = r;
-	e.inner_iter = A.inner_iter;
-	e.kind = DMS_PIPE_STEP;
+	dms_pipe_fill_entry<MX>(A, r, e);
 	return true;
 }
 

@@ -409,16 +409,6 @@ DMS_HD double dms_prop_event_mx(const DmsProp& p, double& t, double* r, double* 
 	return dm_sqrt(p.vr * p.vr + tangential * tangential);
 }
 
-/* The speed dms_prop_event_mx returns, without the event. */
-DMS_HD double dms_prop_speed_mx(const DmsProp& p)
-{
-	const double safe_radius = (dm_isfinite(p.radius) && p.radius > 0.0) ? p.radius : 0.0;
-	if(p.radial || p.J == 0.0 || safe_radius == 0.0)
-		return dm_fabs(p.vr);
-	const double tangential = p.J * dm_recip_shared(safe_radius).y;
-	return dm_sqrt(p.vr * p.vr + tangential * tangential);
-}
-
 /* E = m (v^2 - v_esc^2) / 2 in eV from v^2, without the square root of v_esc^2
  * (outside the Sun 2 G M / r by a refined reciprocal). */
 DMS_HD double dms_capture_energy_mx(const DmsSolarView& S, double dm_mass, double radius, double speed2)

@@ -301,14 +301,16 @@ __device__ __forceinline__ void dms_lds_store(unsigned int* p, unsigned int v)
 
 /* A helper wave serving slot sl (its lane 0 computes): for each session the
  * trajectory's wave opens, it prepares the following steps, assuming each
- * post continues, and writes them to the ring.  Returns when the trajectory's
- * wave has finished all its lanes (want = 0). */
+ * post continues, and writes them to the ring.  A session that changes is
+ * noticed between the preparation and every RKF45 attempt.  Returns when the
+ * trajectory's wave has finished all its lanes (want = 0). */
 template <bool MX, bool MP>
 __device__ __forceinline__ void dms_pipe_serve(const DmsTrajArgs& a, const DmsSolarView& S, DmsPipeSlot* sl, int lane)
 {
 	if(lane != 0)
 		return;
 	DmsLane A;
+	DmsStepEvent ev;
 	DmsPointView P = a.P;
 	unsigned int cur = 0, head = dms_lds_load(&sl->head);
 	bool ended = true;
@@ -324,9 +326,6 @@ __device__ __forceinline__ void dms_pipe_serve(const DmsTrajArgs& a, const DmsSo
 			if(!ended)
 			{
 				DmsPipeSeed seed = sl->seed;
-				dms_pipe_from_seed(A, seed);
-				if constexpr(MP)
-					P = a.job_points[seed.job];
 				/* the seed was read after the session word: if the session
 				 * moved meanwhile, the next pass starts over */
 				if(dms_lds_load(&sl->session) != s)
@@ -334,6 +333,10 @@ __device__ __forceinline__ void dms_pipe_serve(const DmsTrajArgs& a, const DmsSo
 					cur = 0u;
 					continue;
 				}
+				dms_pipe_from_seed(A, seed);
+				if constexpr(MP)
+					P = a.job_points[seed.job];
+				dms_step_event<MX>(A.prop, ev);
 			}
 		}
 		if(ended)
@@ -342,8 +345,32 @@ __device__ __forceinline__ void dms_pipe_serve(const DmsTrajArgs& a, const DmsSo
 			continue;
 		}
 		DmsPipeEntry e;
-		dms_pipe_assume_next<MX>(A);
-		ended = !dms_pipe_prepare_step<MX>(A, P, S, a.J, e);
+		dms_pipe_assume_next(A, ev);
+		if(dms_step_prep<MX>(A, P, S, a.J) >= 0)
+		{
+			e.kind = DMS_PIPE_PREP_END;
+			ended = true;
+		}
+		else
+		{
+			int r = -1;
+			bool moved = false;
+			do
+			{
+				if(__hip_atomic_load(&sl->session, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != cur)
+				{
+					moved = true;
+					break;
+				}
+				A.inner_iter++;
+				A.rk_all++;
+				r = MX ? dms_rk45_attempt_mx(A.prop, S, A.inner_iter) : dms_rk45_attempt(A.prop, S, A.inner_iter);
+			} while(r < 0);
+			if(moved)
+				continue;
+			dms_pipe_fill_entry<MX>(A, r, e);
+			ev = e.ev;
+		}
 		e.session = cur;
 		/* wait for a free ring entry */
 		bool stale = false;
@@ -458,7 +485,7 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 	/* two-wave split state (wave-uniform): lone = one live lane, the others
 	 * finished; attached = a helper serves this wave; session = the open
 	 * session (0: none); tail = ring entries read */
-	bool lone = false, attached = false;
+	bool lone = false, attached = false, held = false;
 	bool split_ok = PIPE && a.lone_serial == 0u;
 	unsigned int session = 0u, sessions = 0u, tail = 0u;
 	for(;;)
@@ -481,9 +508,18 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 			}
 		}
 		int post_ret;
-		dms_lane_advance_post<MX>(L, c, &post_ret);
+		/* a step taken from the helper: its entry (with the end event the post
+		 * uses) stays in the ring until the post is done */
+		dms_lane_advance_post<MX>(L, c, &post_ret, held ? &my->e[tail % DMS_PIPE_DEPTH].ev : nullptr);
 		if constexpr(PIPE)
 		{
+			if(__builtin_expect(held, 0))
+			{
+				held = false;
+				tail++;
+				if(lane == 0)
+					dms_lds_store(&my->tail, tail);
+			}
 			/* the session lasts while the posts continue to the next step */
 			if(__builtin_expect(session != 0u, 0)
 			   && __ballot((L.phase == PH_PREP) & (post_ret == DMS_POST_NEXT)) == 0)
@@ -756,10 +792,19 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 					const DmsPipeEntry* ep = &my->e[tail % DMS_PIPE_DEPTH];
 					const unsigned int es = __builtin_amdgcn_readfirstlane(ep->session);
 					const int kind = __builtin_amdgcn_readfirstlane(ep->kind);
-					if((es == session) & (kind == DMS_PIPE_STEP) & (L.phase == PH_PREP))
+					got = (es == session) & (kind == DMS_PIPE_STEP);
+					if(got)
 					{
-						dms_pipe_take(L, *ep);
-						L.phase = PH_POST;
+						/* held until the post has run (the ring entry is not
+						 * reused before tail passes it) */
+						if(L.phase == PH_PREP)
+						{
+							dms_pipe_take(L, *ep);
+							L.phase = PH_POST;
+						}
+						held = true;
+						timed_out = false;
+						break;
 					}
 					tail++;
 					if(lane == 0)
@@ -768,7 +813,6 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 					{
 						/* a preparation that ends the segment: this lane runs its
 						 * own, which ends it the same way */
-						got = kind == DMS_PIPE_STEP;
 						timed_out = false;
 						break;
 					}

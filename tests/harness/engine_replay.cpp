@@ -394,32 +394,39 @@ static int pipe_session(DmsLane& L, const DmsLaneCtx& c, int depth, uint64_t* ta
 	DmsLane A;
 	memset(&A, 0xC3, sizeof(A));
 	dms_pipe_from_seed(A, seed);
+	DmsStepEvent ev;
+	dms_step_event<MX>(A.prop, ev);
 	std::vector<DmsPipeEntry> ring;
 	size_t head = 0;
 	bool helper_ended = false;
+	const DmsStepEvent* pre = nullptr; /* the first post computes its own event */
 	for(;;)
 	{
 		while(!helper_ended && ring.size() - head < (size_t) depth)
 		{
 			DmsPipeEntry e;
-			dms_pipe_assume_next<MX>(A);
+			memset(&e, 0xA7, sizeof(e));
+			dms_pipe_assume_next(A, ev);
 			if(!dms_pipe_prepare_step<MX>(A, c.P, c.S, c.J, e))
 				helper_ended = true;
+			else
+				ev = e.ev;
 			ring.push_back(e);
 		}
-		const int pr = dms_step_post<MX>(L, c.P, c.S, c.J, c.R);
+		const int pr = dms_step_post<MX>(L, c.P, c.S, c.J, c.R, pre);
 		if(pr != DMS_POST_NEXT)
 		{
 			*dropped += ring.size() - head;
 			return pr;
 		}
-		const DmsPipeEntry e = ring[head++];
+		const DmsPipeEntry& e = ring[head++];
 		if(e.kind == DMS_PIPE_PREP_END)
 		{
 			*dropped += ring.size() - head;
 			return DMS_POST_NEXT;
 		}
 		dms_pipe_take(L, e);
+		pre = &e.ev;
 		(*taken)++;
 	}
 }
