@@ -115,17 +115,28 @@ struct DmsLane
 	int sc_target;
 };
 
-/* Rows written per trajectory: dt[nbins], v2dt[nbins] (FP64), counts[nbins] (u32). */
+/* One bin of a per-lane scratch row: the trajectory's time, v^2-weighted time
+ * and visits in that bin, in one 32-byte record (one cache line per bin change
+ * instead of one per array). */
+struct DmsBinAcc
+{
+	double dt, v2dt;
+	uint32_t cnt, pad0;
+	uint64_t pad1;
+};
+
 /* Per-trajectory radial histogram storage.  Per-trajectory mode: one row per
- * trajectory slot, summed after the launch.  Per-lane mode: one scratch row per
- * resident lane plus the list of bins the current trajectory touched; at the end
- * of a trajectory its bins are added (FP64 atomics) to the sums of its class and
- * the scratch is cleared for the next trajectory of the lane. */
+ * trajectory slot (dt[nbins], v2dt[nbins] in FP64, counts[nbins]), summed
+ * after the launch.  Per-lane mode: one scratch row of DmsBinAcc records per
+ * resident lane plus the list of bins the current trajectory touched; at the
+ * end of a trajectory its bins are added (FP64 atomics) to the sums of its
+ * class and the scratch is cleared for the next trajectory of the lane. */
 struct DmsRows
 {
-	double* rows;       /* [row][2][nbins] or NULL */
-	uint32_t* counts;   /* [row][nbins] or NULL */
-	uint16_t* touched;  /* [row][nbins] touched-bin list (per-lane mode) */
+	double* rows;       /* per-trajectory mode: [row][2][nbins] or NULL */
+	uint32_t* counts;   /* per-trajectory mode: [row][nbins] or NULL */
+	DmsBinAcc* bins;    /* per-lane mode: [row][nbins] */
+	uint16_t* touched;  /* per-lane mode: [row][nbins] touched-bin list */
 	int per_lane;
 	double* hist;       /* per-lane mode: [job][8][nbins] class sums */
 	unsigned long long* hcounts; /* per-lane mode: [job][2][nbins] visit counts */
@@ -954,6 +965,15 @@ DMS_HD void dms_bin_flush(DmsLane& L, const DmsRows& R, int nbins)
 {
 	if(L.cur_bin < 0)
 		return;
+	if(R.per_lane)
+	{
+		DmsBinAcc* b = R.bins + (size_t) L.row * (size_t) nbins + L.cur_bin;
+		b->dt = L.acc_dt;
+		b->v2dt = L.acc_v2;
+		b->cnt = L.acc_cnt;
+		L.cur_bin = -1;
+		return;
+	}
 	if(R.rows)
 	{
 		double* row = R.rows + (size_t) L.row * 2 * (size_t) nbins;
@@ -985,7 +1005,16 @@ DMS_HD void dms_accumulate_bincount(DmsLane& L, const DmsRows& R, const DmsJobVi
 	{
 		dms_bin_flush(L, R, J.nbins);
 		L.cur_bin = bin;
-		if(R.rows)
+		if(R.per_lane)
+		{
+			const DmsBinAcc* b = R.bins + (size_t) L.row * (size_t) J.nbins + bin;
+			L.acc_dt = b->dt;
+			L.acc_v2 = b->v2dt;
+			L.acc_cnt = b->cnt;
+			if(L.acc_cnt == 0u)
+				R.touched[(size_t) L.row * (size_t) J.nbins + L.n_touched++] = (uint16_t) bin;
+		}
+		else if(R.rows)
 		{
 			const double* row = R.rows + (size_t) L.row * 2 * (size_t) J.nbins;
 			L.acc_dt = row[bin];
@@ -996,9 +1025,8 @@ DMS_HD void dms_accumulate_bincount(DmsLane& L, const DmsRows& R, const DmsJobVi
 			L.acc_dt = 0.0;
 			L.acc_v2 = 0.0;
 		}
-		L.acc_cnt = R.counts ? R.counts[(size_t) L.row * (size_t) J.nbins + bin] : 0u;
-		if(R.per_lane && L.acc_cnt == 0u)
-			R.touched[(size_t) L.row * (size_t) J.nbins + L.n_touched++] = (uint16_t) bin;
+		if(!R.per_lane)
+			L.acc_cnt = R.counts ? R.counts[(size_t) L.row * (size_t) J.nbins + bin] : 0u;
 	}
 	const double v2dt = v2 * dt_sec;
 	L.acc_dt += dt_sec;
@@ -1925,17 +1953,16 @@ DMS_HD void dms_shift_failure_record(uint64_t id, const dms_event& ic, dms_traj_
 DMS_HD void dms_flush_rows(const DmsRows& R, int nbins, uint32_t row_index, uint32_t job, uint32_t n, int cls, uint32_t first,
                            uint32_t stride)
 {
-	double* row = R.rows + (size_t) row_index * 2 * (size_t) nbins;
-	uint32_t* cnt = R.counts + (size_t) row_index * (size_t) nbins;
+	DmsBinAcc* row = R.bins + (size_t) row_index * (size_t) nbins;
 	const uint16_t* list = R.touched + (size_t) row_index * (size_t) nbins;
 	double* h = R.hist + (size_t) job * 8 * (size_t) nbins + (cls == 2 ? 4 * (size_t) nbins : 0);
 	unsigned long long* hc = R.hcounts + (size_t) job * 2 * (size_t) nbins + (cls == 2 ? (size_t) nbins : 0);
 	for(uint32_t k = first; k < n; k += stride)
 	{
 		const int b = list[k];
-		const double dt = row[b];
-		const double v2 = row[nbins + b];
-		const uint32_t cn = cnt[b];
+		const double dt = row[b].dt;
+		const double v2 = row[b].v2dt;
+		const uint32_t cn = row[b].cnt;
 		if(cls != 0)
 		{
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -1952,9 +1979,9 @@ DMS_HD void dms_flush_rows(const DmsRows& R, int nbins, uint32_t row_index, uint
 			hc[b] += cn;
 #endif
 		}
-		row[b] = 0.0;
-		row[nbins + b] = 0.0;
-		cnt[b] = 0u;
+		row[b].dt = 0.0;
+		row[b].v2dt = 0.0;
+		row[b].cnt = 0u;
 	}
 }
 

@@ -402,6 +402,7 @@ hipError_t run_trajectories(dms_ctx* ctx, hipStream_t st, hipStream_t pst, const
 	{
 		a.R.rows = rows ? B.rows : nullptr;
 		a.R.counts = rows ? B.counts : nullptr;
+		a.R.bins = nullptr;
 		a.R.touched = nullptr;
 		a.R.per_lane = 0;
 		a.R.hist = nullptr;
@@ -927,12 +928,10 @@ void run_batch_fixed(std::vector<dms_job*>& jobs, hipStream_t st, hipStream_t ps
 	if(ok && want_rows)
 	{
 		LR.per_lane = 1;
-		ok = dalloc(&LR.rows, 2 * (size_t) nb * lanes, st) == hipSuccess && dalloc(&LR.counts, (size_t) nb * lanes, st) == hipSuccess
-		     && dalloc(&LR.touched, (size_t) nb * lanes, st) == hipSuccess && dalloc(&LR.hist, 8 * (size_t) nb * nj, st) == hipSuccess
-		     && dalloc(&LR.hcounts, 2 * (size_t) nb * nj, st) == hipSuccess;
+		ok = dalloc(&LR.bins, (size_t) nb * lanes, st) == hipSuccess && dalloc(&LR.touched, (size_t) nb * lanes, st) == hipSuccess
+		     && dalloc(&LR.hist, 8 * (size_t) nb * nj, st) == hipSuccess && dalloc(&LR.hcounts, 2 * (size_t) nb * nj, st) == hipSuccess;
 		if(ok)
-			ok = hipMemsetAsync(LR.rows, 0, sizeof(double) * 2 * (size_t) nb * lanes, st) == hipSuccess
-			     && hipMemsetAsync(LR.counts, 0, sizeof(uint32_t) * (size_t) nb * lanes, st) == hipSuccess
+			ok = hipMemsetAsync(LR.bins, 0, sizeof(DmsBinAcc) * (size_t) nb * lanes, st) == hipSuccess
 			     && hipMemsetAsync(LR.hist, 0, sizeof(double) * 8 * (size_t) nb * nj, st) == hipSuccess
 			     && hipMemsetAsync(LR.hcounts, 0, sizeof(unsigned long long) * 2 * (size_t) nb * nj, st) == hipSuccess;
 	}
@@ -1024,8 +1023,7 @@ void run_batch_fixed(std::vector<dms_job*>& jobs, hipStream_t st, hipStream_t ps
 			job->other_ms = total * 1e3 - kms;
 		}
 	}
-	dfree(LR.rows, st);
-	dfree(LR.counts, st);
+	dfree(LR.bins, st);
 	dfree(LR.touched, st);
 	dfree(LR.hist, st);
 	dfree(LR.hcounts, st);
@@ -1057,7 +1055,7 @@ size_t batch_device_bytes(const dms_ctx* ctx, const std::vector<dms_job*>& jobs)
 	const size_t lanes = (size_t) traj_grid(ctx, n) * DMS_TRAJ_BLOCK;
 	size_t bytes = n * (sizeof(dms_event) + sizeof(int32_t) + sizeof(DmsSummary) + (recs ? sizeof(dms_traj_record) : 0));
 	if(rows)
-		bytes += lanes * nb * (2 * sizeof(double) + sizeof(uint32_t) + sizeof(uint16_t))
+		bytes += lanes * nb * (sizeof(DmsBinAcc) + sizeof(uint16_t))
 		         + jobs.size() * nb * (8 * sizeof(double) + 2 * sizeof(unsigned long long));
 	return bytes;
 }
@@ -1260,8 +1258,7 @@ struct dms_server
 	uint32_t* d_round_done = nullptr;
 	double* d_round_hist = nullptr;
 	unsigned long long* d_round_counts = nullptr;
-	double* d_rows = nullptr;
-	uint32_t* d_counts = nullptr;
+	DmsBinAcc* d_bins = nullptr;
 	uint16_t* d_touched = nullptr;
 	/* driver state (driver thread only) */
 	std::vector<ServeRequest*> pos_req;
@@ -1308,8 +1305,7 @@ void serve_release(dms_server* S)
 	dfree(S->d_round_done, cs);
 	dfree(S->d_round_hist, cs);
 	dfree(S->d_round_counts, cs);
-	dfree(S->d_rows, cs);
-	dfree(S->d_counts, cs);
+	dfree(S->d_bins, cs);
 	dfree(S->d_touched, cs);
 	hipStreamSynchronize(cs);
 	if(S->ended)
@@ -1344,8 +1340,7 @@ bool serve_launch(dms_server* S, std::string& err)
 	a.S = ctx->dev;
 	a.J = S->J;
 	a.R.per_lane = S->want_rows ? 1 : 0;
-	a.R.rows = S->d_rows;
-	a.R.counts = S->d_counts;
+	a.R.bins = S->d_bins;
 	a.R.touched = S->d_touched;
 	a.R.hist = S->d_round_hist;
 	a.R.hcounts = S->d_round_counts;
@@ -2464,6 +2459,7 @@ int dms_simulate_events_noalloc(dms_point* pt, const dms_job_desc* job, const dm
 	a.summ = nullptr;
 	a.R.rows = rows ? d_hist_rows : nullptr;
 	a.R.counts = nullptr;
+	a.R.bins = nullptr;
 	a.R.touched = nullptr;
 	a.R.per_lane = 0;
 	a.R.hist = nullptr;
@@ -2816,12 +2812,10 @@ int dms_server_create(dms_ctx* ctx, const dms_server_desc* desc, dms_server** ou
 		     && hipMemsetAsync(S->d_round_done, 0, sizeof(uint32_t) * (R ? R : 1), cs) == hipSuccess;
 	if(ok && S->want_rows)
 		ok = dalloc(&S->d_round_hist, R * 8 * nb, cs) == hipSuccess && dalloc(&S->d_round_counts, R * 2 * nb, cs) == hipSuccess
-		     && dalloc(&S->d_rows, S->lanes * 2 * nb, cs) == hipSuccess && dalloc(&S->d_counts, S->lanes * nb, cs) == hipSuccess
-		     && dalloc(&S->d_touched, S->lanes * nb, cs) == hipSuccess
+		     && dalloc(&S->d_bins, S->lanes * nb, cs) == hipSuccess && dalloc(&S->d_touched, S->lanes * nb, cs) == hipSuccess
 		     && hipMemsetAsync(S->d_round_hist, 0, sizeof(double) * R * 8 * nb, cs) == hipSuccess
 		     && hipMemsetAsync(S->d_round_counts, 0, sizeof(unsigned long long) * R * 2 * nb, cs) == hipSuccess
-		     && hipMemsetAsync(S->d_rows, 0, sizeof(double) * S->lanes * 2 * nb, cs) == hipSuccess
-		     && hipMemsetAsync(S->d_counts, 0, sizeof(uint32_t) * S->lanes * nb, cs) == hipSuccess;
+		     && hipMemsetAsync(S->d_bins, 0, sizeof(DmsBinAcc) * S->lanes * nb, cs) == hipSuccess;
 	if(ok)
 		ok = hipEventCreateWithFlags(&S->ended, hipEventDisableTiming) == hipSuccess && hipStreamSynchronize(cs) == hipSuccess;
 	ctx_lock.unlock();

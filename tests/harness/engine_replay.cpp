@@ -304,13 +304,14 @@ int rp_lane_hist(rp_point* p, const dms_job_desc* job, const dms_event* ics, con
 	const size_t nb = (size_t) c.J.nbins;
 	if(c.J.terminate_on_capture || nlanes == 0 || nb > 65536)
 		return -1;
-	std::vector<double> rows(2 * nb * nlanes, 0.0);
-	std::vector<uint32_t> counts(nb * nlanes, 0u);
+	std::vector<DmsBinAcc> bins(nb * nlanes);
+	memset(bins.data(), 0, sizeof(DmsBinAcc) * bins.size());
 	std::vector<uint16_t> touched(nb * nlanes, 0);
 	memset(hist, 0, sizeof(double) * 8 * nb);
 	memset(hcounts, 0, sizeof(unsigned long long) * 2 * nb);
-	c.R.rows = rows.data();
-	c.R.counts = counts.data();
+	c.R.rows = nullptr;
+	c.R.counts = nullptr;
+	c.R.bins = bins.data();
 	c.R.touched = touched.data();
 	c.R.per_lane = 1;
 	c.R.hist = hist;
@@ -327,12 +328,9 @@ int rp_lane_hist(rp_point* p, const dms_job_desc* job, const dms_event* ics, con
 		DmsSummary s;
 		replay_lane(L, c, ics[i], ok[i] != 0, nullptr, &s);
 	}
-	for(size_t k = 0; k < 2 * nb * nlanes; k++)
-		if(rows[k] != 0.0)
-			return -2; /* scratch not cleared after a flush */
 	for(size_t k = 0; k < nb * nlanes; k++)
-		if(counts[k] != 0u)
-			return -2;
+		if(bins[k].dt != 0.0 || bins[k].v2dt != 0.0 || bins[k].cnt != 0u)
+			return -2; /* scratch not cleared after a flush */
 	return 0;
 }
 
@@ -346,6 +344,7 @@ int rp_simulate(rp_point* p, const dms_job_desc* job, const dms_event* ics, size
 	fill_job_view(J, job);
 	c.R.rows = J.terminate_on_capture ? nullptr : rows;
 	c.R.counts = nullptr;
+	c.R.bins = nullptr;
 	c.R.touched = nullptr;
 	c.R.per_lane = 0;
 	c.R.hist = nullptr;
@@ -474,6 +473,7 @@ int rp_simulate_pipe(rp_point* p, const dms_job_desc* job, const dms_event* ics,
 	fill_job_view(J, job);
 	c.R.rows = J.terminate_on_capture ? nullptr : rows;
 	c.R.counts = nullptr;
+	c.R.bins = nullptr;
 	c.R.touched = nullptr;
 	c.R.per_lane = 0;
 	c.R.hist = nullptr;

@@ -325,3 +325,64 @@ def test_capture_workload_full_size(gpu_ctx, oracle_solar):
     same = (rec_bytes(res["records"]) == rec_bytes(ores["records"])).all(axis=1)
     assert same.all(), f"{(~same).sum()} of {n} records differ"
     assert res["counters"].captured > 100
+
+
+# Long bound trajectories of the parameter-point workload (m = 0.3 GeV,
+# sigma_p = 1e-36, v_obs = 248 km/s): 1.8e5 and 1.9e5 RKF45 attempts, about a
+# fifth of their steps rejected on optical depth, excursions outside the Sun.
+LONG_IDS = (65143, 16020326)
+
+
+@pytest.mark.parametrize("precision", [dmsun.FP64, dmsun.MIXED], ids=["fp64", "mixed"])
+def test_lone_trajectory_split_matches_serial(gpu_ctx, oracle_solar, precision):
+    """A trajectory left alone in its wave has its next steps prepared by a
+    finished wave of the block (the two-wave split, on by default); with the
+    split turned off (lone_serial) its wave runs every step itself.  Records
+    are bit-identical either way, and (FP64) equal the oracle's; a batch whose
+    waves turn lone at different times and share the finished waves gives the
+    same records as well."""
+    desc = dmsun.make_point(0.3, 1e-36, interpolation_points=1000)
+    pt = gpu_ctx.point(desc)
+    halo = dmsun.default_halo((0.0, 248.0, 0.0))
+    runs = {}
+    try:
+        for serial in (1, 0):
+            gpu_ctx.set_lone_serial(serial)
+            out = []
+            for tid in LONG_IDS:
+                job = dmsun.make_job(mode=dmsun.MODE_PARAMETER_POINT, max_trajectories=1, max_scatterings=10**14,
+                                     n_bins=1000, first_trajectory_id=tid, precision=precision, halo=halo,
+                                     rk_attempt_budget=RK_BUDGET)
+                ev, ok = gpu_ctx.initial_conditions(job.halo, job.seed, job.point_id, tid, 1)
+                recs, _ = pt.simulate_events(job, ev)
+                print(f"id {tid} lone_serial {serial}: {recs[0].n_rk_attempts} attempts", flush=True)
+                out.append(rec_bytes(recs))
+            # 200 trajectories from 65100 (one block: four waves that finish at different times)
+            job = dmsun.make_job(mode=dmsun.MODE_PARAMETER_POINT, max_trajectories=200, max_scatterings=10**14,
+                                 n_bins=1000, first_trajectory_id=65100, precision=precision, halo=halo,
+                                 rk_attempt_budget=RK_BUDGET)
+            ev, ok = gpu_ctx.initial_conditions(job.halo, job.seed, job.point_id, 65100, 200)
+            recs, rows = pt.simulate_events(job, ev, want_rows=True)
+            out.append(rec_bytes(recs))
+            out.append(rows.view(np.uint64))
+            runs[serial] = out
+    finally:
+        gpu_ctx.set_lone_serial(0)
+    for a, b in zip(runs[0], runs[1]):
+        assert np.array_equal(a, b)
+    assert recs[43].n_rk_attempts > 100000  # id 65143 is in the batch
+    if precision == dmsun.FP64:
+        for tid, got in zip(LONG_IDS, runs[0]):
+            job = dmsun.make_job(mode=dmsun.MODE_PARAMETER_POINT, max_trajectories=1, max_scatterings=10**14,
+                                 n_bins=1000, first_trajectory_id=tid, halo=halo, rk_attempt_budget=RK_BUDGET)
+            ev, ok = oracle_lib.initial_conditions(oracle_solar, job.halo, job.seed, job.point_id, tid, 1)
+            orecs, _, _ = oracle_lib.Point(oracle_solar, desc).simulate_events(job, ev)
+            assert np.array_equal(got, rec_bytes(orecs))
+
+
+def test_lone_serial_knob_checked(gpu_ctx):
+    kn = gpu_ctx.knobs()
+    kn.lone_serial = 2
+    with pytest.raises(dmsun.DmsError):
+        gpu_ctx.check(dmsun.lib().dms_set_launch_knobs(gpu_ctx.h, C.byref(kn)), "dms_set_launch_knobs")
+    assert gpu_ctx.knobs().lone_serial == 0
