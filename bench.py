@@ -62,8 +62,8 @@ PEAK_FP64_TFLOPS = 78.6
 # wave instructions weighted by operations (ADD + MUL + TRANS + 2 FMA); see
 # profiles/dms_traj_kernel_parameter_point_m0.3_20x1e6.md.  Other step counts
 # are other launches: their traffic is reported as null (unmeasured).
-PROFILED_LAUNCH = {"steps": 20, "trajectories": 1_000_000, "hbm_bytes": 1413.940441600e9 + 801.543888608e9,
-                   "f64_op_weighted_wave_instructions": 4.1455e11}
+PROFILED_LAUNCH = {"steps": 20, "trajectories": 1_000_000, "hbm_bytes": 2.0 * 306_910_352 * 1024 + 466_512_423 * 1024,
+                   "f64_op_weighted_wave_instructions": 4.2809e11}
 
 
 def algorithmic_flops(c, capture_mode):
@@ -237,6 +237,31 @@ def measure_bulk(dmsun, pt, precision):
             "truncated": c.computational_truncations}
 
 
+def measure_lone(dmsun, ctx, pt, precision, tid=16020326):
+    """One long trajectory of the timed workload (id 16020326: 193,570 RKF45
+    attempts in FP64) launched alone, with the two-wave split off
+    (lone_serial) and on: launch time per RKF45 attempt of a lone lane, and
+    whether the two records are identical."""
+    import numpy as np
+    out, recs = {}, {}
+    try:
+        for serial in (1, 0):
+            ctx.set_lone_serial(serial)
+            _, job = make_workload(dmsun, 1, tid, precision)
+            job.record_trajectories = 1
+            res = pt.run_job(job, want_records=True)
+            n_rk = int(res["records"][0].n_rk_attempts)
+            recs[serial] = np.ctypeslib.as_array(res["records"]).view(np.uint8).copy()
+            out["serial_us_per_attempt" if serial else "split_us_per_attempt"] = 1e3 * res["kernel_ms"] / max(1, n_rk)
+            out["rk_attempts"] = n_rk
+    finally:
+        ctx.set_lone_serial(0)
+    out["trajectory_id"] = tid
+    out["speedup"] = out["serial_us_per_attempt"] / out["split_us_per_attempt"]
+    out["records_identical"] = bool(np.array_equal(recs[0], recs[1]))
+    return out
+
+
 def validate_mixed(dmsun, ctx, n):
     """The mixed-precision workload at full size (SI m = 0.5 GeV, sigma_p =
     1e-37 cm^2, n trajectories, parameter point): FP64 and mixed kernels on the
@@ -402,7 +427,13 @@ def main():
         "us_per_attempt_of_longest": 1e3 * kern_ms / max(1, fp64["max_rk"]),
         "mixed_longest_trajectory_rk_attempts": mixed["max_rk"] if mixed else None,
         "mixed_us_per_attempt_of_longest": 1e3 * mixed["kern_ms"] / max(1, mixed["max_rk"]) if mixed else None,
+        "lone_split_note": "a trajectory left alone in its wave has its next step prepared by a finished wave of "
+                           "its block while its own wave post-processes the current one (on in every launch above); "
+                           "lone_split times one long trajectory launched alone with that split off and on",
+        "lone_split": {"fp64": measure_lone(dmsun, ctx, pt, dmsun.FP64)},
     }
+    if mixed is not None:
+        out["tail"]["lone_split"]["mixed"] = measure_lone(dmsun, ctx, pt, dmsun.MIXED)
     out["bulk"] = {"fp64": measure_bulk(dmsun, pt, dmsun.FP64)}
     if mixed is not None:
         out["bulk"]["mixed"] = measure_bulk(dmsun, pt, dmsun.MIXED)
