@@ -398,6 +398,7 @@ static int pipe_session(DmsLane& L, const DmsLaneCtx& c, int depth, uint64_t* ta
 	std::vector<DmsPipeEntry> ring;
 	size_t head = 0;
 	bool helper_ended = false;
+	DmsPipeEntry held;                 /* the step taken from the helper, as the kernel holds its ring entry */
 	const DmsStepEvent* pre = nullptr; /* the first post computes its own event */
 	for(;;)
 	{
@@ -418,14 +419,14 @@ static int pipe_session(DmsLane& L, const DmsLaneCtx& c, int depth, uint64_t* ta
 			*dropped += ring.size() - head;
 			return pr;
 		}
-		const DmsPipeEntry& e = ring[head++];
-		if(e.kind == DMS_PIPE_PREP_END)
+		held = ring[head++];
+		if(held.kind == DMS_PIPE_PREP_END)
 		{
 			*dropped += ring.size() - head;
 			return DMS_POST_NEXT;
 		}
-		dms_pipe_take(L, e);
-		pre = &e.ev;
+		dms_pipe_take(L, held);
+		pre = &held.ev;
 		(*taken)++;
 	}
 }
