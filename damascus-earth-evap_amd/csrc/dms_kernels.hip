@@ -327,7 +327,9 @@ __device__ __forceinline__ void dms_pipe_serve(const DmsTrajArgs& a, const DmsSo
 			{
 				DmsPipeSeed seed = sl->seed;
 				/* the seed was read after the session word: if the session
-				 * moved meanwhile, the next pass starts over */
+				 * moved meanwhile, the next pass starts over (the fence keeps
+				 * the seed's loads before the second look) */
+				__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 				if(dms_lds_load(&sl->session) != s)
 				{
 					cur = 0u;
@@ -502,6 +504,11 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 				session = ++sessions;
 				if(L.phase == PH_POST)
 				{
+					/* the seed's stores stay after the previous session's end
+					 * (session 0) and before this session's number: a helper
+					 * that reads the seed and then still sees the same session
+					 * read one whole seed */
+					__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
 					dms_pipe_make_seed(L, L.job, my->seed);
 					dms_lds_store(&my->session, session);
 				}
