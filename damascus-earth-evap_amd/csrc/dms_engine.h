@@ -2152,7 +2152,8 @@ DMS_HD void dms_lane_advance_b(DmsLane& L, const DmsLaneCtx& c)
 #define DMS_PIPE_PREP_END 1 /* the step preparation ended the segment */
 
 /* Session start: the propagator after the accepted RKF45 attempt of the step
- * whose post the trajectory's wave is about to run. */
+ * whose post the trajectory's wave is about to run, and its event (which that
+ * post takes too). */
 struct DmsPipeSeed
 {
 	DmsProp prop;
@@ -2161,6 +2162,7 @@ struct DmsPipeSeed
 	double time_origin;
 	uint32_t job;
 	uint32_t pad;
+	DmsStepEvent ev;
 };
 
 /* One step prepared by the helper: what dms_step_prep and the RKF45 attempts
@@ -2180,6 +2182,7 @@ struct DmsPipeEntry
 	DmsStepEvent ev;
 };
 
+template <bool MX = false>
 DMS_HD void dms_pipe_make_seed(const DmsLane& L, uint32_t job, DmsPipeSeed& s)
 {
 	s.prop = L.prop;
@@ -2188,6 +2191,7 @@ DMS_HD void dms_pipe_make_seed(const DmsLane& L, uint32_t job, DmsPipeSeed& s)
 	s.time_origin = L.time_origin;
 	s.job = job;
 	s.pad = 0;
+	dms_step_event<MX>(L.prop, s.ev);
 }
 
 DMS_HD void dms_pipe_from_seed(DmsLane& A, const DmsPipeSeed& s)

@@ -338,7 +338,7 @@ __device__ __forceinline__ void dms_pipe_serve(const DmsTrajArgs& a, const DmsSo
 				dms_pipe_from_seed(A, seed);
 				if constexpr(MP)
 					P = a.job_points[seed.job];
-				dms_step_event<MX>(A.prop, ev);
+				ev = seed.ev;
 			}
 		}
 		if(ended)
@@ -487,7 +487,7 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 	/* two-wave split state (wave-uniform): lone = one live lane, the others
 	 * finished; attached = a helper serves this wave; session = the open
 	 * session (0: none); tail = ring entries read */
-	bool lone = false, attached = false, held = false;
+	bool lone = false, attached = false, held = false, seeded = false;
 	bool split_ok = PIPE && a.lone_serial == 0u;
 	unsigned int session = 0u, sessions = 0u, tail = 0u;
 	for(;;)
@@ -509,15 +509,19 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 					 * that reads the seed and then still sees the same session
 					 * read one whole seed */
 					__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-					dms_pipe_make_seed(L, L.job, my->seed);
+					dms_pipe_make_seed<MX>(L, L.job, my->seed);
 					dms_lds_store(&my->session, session);
 				}
+				seeded = true;
 			}
 		}
 		int post_ret;
 		/* a step taken from the helper: its entry (with the end event the post
-		 * uses) stays in the ring until the post is done */
-		dms_lane_advance_post<MX>(L, c, &post_ret, held ? &my->e[tail % DMS_PIPE_DEPTH].ev : nullptr);
+		 * uses) stays in the ring until the post is done; a session's first
+		 * post takes the event its seed holds */
+		dms_lane_advance_post<MX>(L, c, &post_ret,
+		                          held ? &my->e[tail % DMS_PIPE_DEPTH].ev : (seeded ? &my->seed.ev : nullptr));
+		seeded = false;
 		if constexpr(PIPE)
 		{
 			if(__builtin_expect(held, 0))

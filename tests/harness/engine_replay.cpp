@@ -389,17 +389,16 @@ static int pipe_session(DmsLane& L, const DmsLaneCtx& c, int depth, uint64_t* ta
 		L.rk_ok = (r >= 0) ? r : L.rk_ok;
 	} while(r < 0);
 	DmsPipeSeed seed;
-	dms_pipe_make_seed(L, L.job, seed);
+	dms_pipe_make_seed<MX>(L, L.job, seed);
 	DmsLane A;
 	memset(&A, 0xC3, sizeof(A));
 	dms_pipe_from_seed(A, seed);
-	DmsStepEvent ev;
-	dms_step_event<MX>(A.prop, ev);
+	DmsStepEvent ev = seed.ev;
 	std::vector<DmsPipeEntry> ring;
 	size_t head = 0;
 	bool helper_ended = false;
 	DmsPipeEntry held;                 /* the step taken from the helper, as the kernel holds its ring entry */
-	const DmsStepEvent* pre = nullptr; /* the first post computes its own event */
+	const DmsStepEvent* pre = &seed.ev; /* the first post takes the seed's event */
 	for(;;)
 	{
 		while(!helper_ended && ring.size() - head < (size_t) depth)
