@@ -846,6 +846,32 @@ DMS_HD void dms_rates_at_fractions(const DmsPointView& P, const DmsSolarView& S,
 		*xout = q[NQ - 1].res;
 }
 
+/* Element k of dms_rates_at_fractions (k < DMS_MAX_OPTICAL_DEPTH_PIECES + 1:
+ * fraction fr[k]; k = DMS_MAX_OPTICAL_DEPTH_PIECES + 1: the end-point query)
+ * evaluated on its own: the value that function leaves in out[k] (in *xout for
+ * the end-point query), by the same slot, table read and direct-sum steps. */
+DMS_HD double dms_rate_fraction_slot(const DmsPointView& P, const DmsSolarView& S, int k, const double* br, const double* bv,
+                                     const double* ar, const double* av, const double* fr, int n, bool use_cached,
+                                     double cached_rate, bool extra, double xr, double xv)
+{
+	constexpr int NQ = DMS_MAX_OPTICAL_DEPTH_PIECES + 2;
+	if((k == 0) & use_cached)
+		return cached_rate;
+	const bool use_grid = P.use_grid != 0;
+	const bool is_extra = (k == NQ - 1);
+	DmsRateSlot q = dms_rate_slot(P, use_grid, is_extra ? extra : (k < n), !is_extra, br, bv, ar, av, fr[is_extra ? 0 : k], xr, xv);
+	if(use_grid)
+	{
+		const unsigned nv = (unsigned) P.Nv;
+		const double* t = P.grid + q.lo;
+		const double g[4] = {t[0], t[1], t[nv], t[nv + 1]};
+		q.res = dms_rate_slot_bilinear(q, g);
+	}
+	if(__builtin_expect(q.comp, 0))
+		q.res = dms_rate_computed(P, S, q.rad, q.spd);
+	return q.res;
+}
+
 /* Solve_Linear_Rate_Optical_Depth_Fraction, :416-464 */
 DMS_HD double dms_solve_linear_rate(double rs, double re, double dt, double target_tau)
 {
@@ -1186,7 +1212,7 @@ DMS_HD int dms_step_prep(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 #define DMS_POST_NEXT (-4)
 template <bool MX = false>
 DMS_HD int dms_step_post(DmsLane& L, const DmsPointView& P, const DmsSolarView& S, const DmsJobView& J, const DmsRows& R,
-                         const DmsStepEvent* pre = nullptr)
+                         const DmsStepEvent* pre = nullptr, const double* pre_rates = nullptr)
 {
 	const double actual_dt = L.prop.time - L.pb_time;
 	const double r_after = dms_prop_radius(L.prop);
@@ -1364,10 +1390,20 @@ DMS_HD int dms_step_post(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 	#pragma unroll
 				for(int i = 0; i <= DMS_MAX_OPTICAL_DEPTH_PIECES; i++)
 					fr[i] = (i <= od_count) ? is + (double) i * inv_pieces * span : 0.0;
-				/* Scattering_Rate_At_Fraction for all fractions, grid loads issued together */
+				/* Scattering_Rate_At_Fraction for all fractions, grid loads issued together
+				 * (or their values, evaluated beforehand by dms_rate_fraction_slot) */
 				want_next = r_after < DMS_rSun;
-				dms_rates_at_fractions(P, S, L.eb_r, L.eb_v, ea_r, ea_v, fr, od_count + 1, use_cached, L.rate_before, rt,
-				                       want_next, r_after, v_after, &rate_next);
+				if(pre_rates)
+				{
+	#pragma unroll
+					for(int i = 0; i <= DMS_MAX_OPTICAL_DEPTH_PIECES; i++)
+						rt[i] = pre_rates[i];
+					if(want_next)
+						rate_next = pre_rates[DMS_MAX_OPTICAL_DEPTH_PIECES + 1];
+				}
+				else
+					dms_rates_at_fractions(P, S, L.eb_r, L.eb_v, ea_r, ea_v, fr, od_count + 1, use_cached, L.rate_before, rt,
+					                       want_next, r_after, v_after, &rate_next);
 	#pragma unroll
 				for(int i = 0; i <= DMS_MAX_OPTICAL_DEPTH_PIECES; i++)
 					invalid |= (int) ((i <= od_count) & (!dm_isfinite(rt[i]) | (rt[i] < 0.0)));
@@ -2018,12 +2054,13 @@ DMS_HD void dms_lane_advance_rk(DmsLane& L, const DmsLaneCtx& c)
  * uniform are drawn here, before the wave-level target choice).  *post_ret
  * receives dms_step_post's value (0 when no post ran). */
 template <bool MX = false>
-DMS_HD void dms_lane_advance_post(DmsLane& L, const DmsLaneCtx& c, int* post_ret = nullptr, const DmsStepEvent* pre = nullptr)
+DMS_HD void dms_lane_advance_post(DmsLane& L, const DmsLaneCtx& c, int* post_ret = nullptr, const DmsStepEvent* pre = nullptr,
+                                  const double* pre_rates = nullptr)
 {
 	int pr = 0;
 	if(L.phase == PH_POST)
 	{
-		pr = dms_step_post<MX>(L, c.P, c.S, c.J, c.R, pre);
+		pr = dms_step_post<MX>(L, c.P, c.S, c.J, c.R, pre, pre_rates);
 		if((pr == -2) | (pr == DMS_POST_NEXT))
 			L.phase = PH_PREP;
 		else
@@ -2178,8 +2215,17 @@ struct DmsPipeEntry
 	int32_t rk_ok, inner_iter;
 	uint32_t session;
 	int32_t kind;
-	int32_t pad;
+	uint32_t job;
 	DmsStepEvent ev;
+	/* the step's start: the event and radius the post takes as eb and r_before */
+	DmsStepEvent eb;
+	double r_before;
+	/* FP64 only: the post's rates at the optical-depth fractions and the end
+	 * point (dms_rate_fraction_slot), when have_rates, filled by a second
+	 * helper wave */
+	int32_t have_rates;
+	int32_t pad;
+	double rt[DMS_MAX_OPTICAL_DEPTH_PIECES + 2];
 };
 
 template <bool MX = false>
@@ -2242,6 +2288,7 @@ DMS_HD void dms_pipe_fill_entry(const DmsLane& A, int r, DmsPipeEntry& e)
 	e.rk_ok = r;
 	e.inner_iter = A.inner_iter;
 	e.kind = DMS_PIPE_STEP;
+	e.have_rates = 0;
 	dms_step_event<MX>(A.prop, e.ev);
 }
 
@@ -2265,6 +2312,47 @@ DMS_HD bool dms_pipe_prepare_step(DmsLane& A, const DmsPointView& P, const DmsSo
 	} while(r < 0);
 	dms_pipe_fill_entry<MX>(A, r, e);
 	return true;
+}
+
+/* The inputs of the FP64 post's rate queries for the step of entry e (the
+ * post's own steps up to dms_rates_at_fractions: interior interval, fractions,
+ * cached start rate, end-point query).  Returns false when the post makes no
+ * rate query for the step; query k is then dms_rate_fraction_slot(..., k, ...)
+ * with these inputs. */
+struct DmsPipeRateInputs
+{
+	double fr[DMS_MAX_OPTICAL_DEPTH_PIECES + 1];
+	double r_after;
+	int n;
+	bool use_cached, want_next;
+};
+
+DMS_HD bool dms_pipe_rate_inputs(const DmsPipeEntry& e, const DmsJobView& J, DmsPipeRateInputs& q)
+{
+	q.r_after = (e.radius < 0.0) ? 0.0 : e.radius; /* dms_prop_radius */
+	const double actual_dt = e.time - e.pb_time;
+	double is = 0.0, ie = 0.0;
+	if(!dms_interior_interval(e.eb.r, e.ev.r, e.r_before, q.r_after, is, ie))
+		return false;
+	if(!((ie > is) & (actual_dt > 0.0)))
+		return false;
+	const int fast = J.terminate_on_capture;
+	const int od_count = fast ? DMS_CAPTURE_MODE_OPTICAL_DEPTH_PIECES : DMS_MAX_OPTICAL_DEPTH_PIECES;
+	const double inv_pieces = fast ? 1.0 / DMS_CAPTURE_MODE_OPTICAL_DEPTH_PIECES : 1.0 / DMS_MAX_OPTICAL_DEPTH_PIECES;
+	const double span = ie - is;
+	for(int i = 0; i <= DMS_MAX_OPTICAL_DEPTH_PIECES; i++)
+		q.fr[i] = (i <= od_count) ? is + (double) i * inv_pieces * span : 0.0;
+	q.n = od_count + 1;
+	q.use_cached = (e.r_before < DMS_rSun) & (is <= 1.0e-12);
+	q.want_next = q.r_after < DMS_rSun;
+	return true;
+}
+
+/* Query k of entry e (inputs from dms_pipe_rate_inputs). */
+DMS_HD double dms_pipe_rate(const DmsPipeEntry& e, const DmsPipeRateInputs& q, const DmsPointView& P, const DmsSolarView& S, int k)
+{
+	return dms_rate_fraction_slot(P, S, k, e.eb.r, e.eb.v, e.ev.r, e.ev.v, q.fr, q.n, q.use_cached, e.rate_before,
+	                              q.want_next, q.r_after, e.ev.speed);
 }
 
 /* Trajectory side, after its post returned DMS_POST_NEXT: take the helper's

@@ -298,6 +298,9 @@ __device__ __forceinline__ void dms_lds_store(unsigned int* p, unsigned int v)
 /* Spin limits (polls of at least 64 cycles each): a trajectory's wave that
  * waits this long for a helper's step gives the split up and goes on alone. */
 #define DMS_PIPE_WAIT_POLLS (1u << 22)
+/* polls for the second helper's rates of an entry before the post computes
+ * its own */
+#define DMS_PIPE_RATE_POLLS 256u
 
 /* A helper wave serving slot sl (its lane 0 computes): for each session the
  * trajectory's wave opens, it prepares the following steps, assuming each
@@ -312,6 +315,7 @@ __device__ __forceinline__ void dms_pipe_serve(const DmsTrajArgs& a, const DmsSo
 	DmsLane A;
 	DmsStepEvent ev;
 	DmsPointView P = a.P;
+	uint32_t job = 0;
 	unsigned int cur = 0, head = dms_lds_load(&sl->head);
 	bool ended = true;
 	for(;;)
@@ -336,8 +340,9 @@ __device__ __forceinline__ void dms_pipe_serve(const DmsTrajArgs& a, const DmsSo
 					continue;
 				}
 				dms_pipe_from_seed(A, seed);
+				job = seed.job;
 				if constexpr(MP)
-					P = a.job_points[seed.job];
+					P = a.job_points[job];
 				ev = seed.ev;
 			}
 		}
@@ -347,7 +352,10 @@ __device__ __forceinline__ void dms_pipe_serve(const DmsTrajArgs& a, const DmsSo
 			continue;
 		}
 		DmsPipeEntry e;
+		e.eb = ev;
 		dms_pipe_assume_next(A, ev);
+		e.r_before = A.r_before;
+		e.job = job;
 		if(dms_step_prep<MX>(A, P, S, a.J) >= 0)
 		{
 			e.kind = DMS_PIPE_PREP_END;
@@ -393,6 +401,55 @@ __device__ __forceinline__ void dms_pipe_serve(const DmsTrajArgs& a, const DmsSo
 	}
 }
 
+/* A second helper wave serving slot sl (FP64 kernels): for every entry the
+ * first helper writes, in order, the FP64 post's rate queries at the
+ * optical-depth fractions and the end point, one query per lane
+ * (dms_pipe_rate: the same evaluations dms_rates_at_fractions makes one after
+ * the other); then rhead passes the entry.  Entries of any session are
+ * evaluated (a stale one costs a little time, nothing else).  Returns when the
+ * trajectory's wave has finished all its lanes. */
+template <bool MP>
+__device__ __forceinline__ void dms_pipe_rates(const DmsTrajArgs& a, const DmsSolarView& S, DmsPipeSlot* sl, int lane)
+{
+	constexpr int NQ = DMS_MAX_OPTICAL_DEPTH_PIECES + 2;
+	unsigned int b = __builtin_amdgcn_readfirstlane(dms_lds_load(&sl->head));
+	/* entries before b were written before this wave came: their have_rates
+	 * stays 0 (the first helper writes it so), so their posts compute their own */
+	dms_lds_store(&sl->rhead, b);
+	for(;;)
+	{
+		if(__builtin_amdgcn_readfirstlane(dms_lds_load(&sl->want)) == 0u)
+			return;
+		if(__builtin_amdgcn_readfirstlane(dms_lds_load(&sl->head)) == b)
+		{
+			__builtin_amdgcn_s_sleep(1);
+			continue;
+		}
+		DmsPipeEntry* ep = &sl->e[b % DMS_PIPE_DEPTH];
+		bool have = false;
+		if(__builtin_amdgcn_readfirstlane(ep->kind) == DMS_PIPE_STEP)
+		{
+			DmsPipeRateInputs q;
+			have = dms_pipe_rate_inputs(*ep, a.J, q);
+			if(have & (lane < NQ))
+			{
+				DmsPointView P;
+				if constexpr(MP)
+					P = a.job_points[ep->job];
+				else
+					P = a.P;
+				ep->rt[lane] = dms_pipe_rate(*ep, q, P, S, lane);
+			}
+		}
+		if(lane == 0)
+			ep->have_rates = have ? 1 : 0;
+		b++;
+		/* the rates and the flag before the progress word (every lane's stores) */
+		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+		dms_lds_store(&sl->rhead, b);
+	}
+}
+
 /* A wave whose lanes have all finished: serve lone trajectories of the other
  * waves of the block until every wave of the block has finished. */
 template <bool MX, bool MP>
@@ -403,24 +460,37 @@ __device__ __forceinline__ void dms_pipe_idle(const DmsTrajArgs& a, const DmsSol
 	{
 		if(dms_lds_load(&pb->waves_done) >= (unsigned int) nwaves)
 			return;
-		int claimed = -1;
+		/* first helper of a waiting wave, or (FP64) its second helper */
+		int claimed = -1, role = 0;
 		if(lane == 0)
-			for(int w = 0; w < nwaves; w++)
+			for(int w = 0; w < nwaves && claimed < 0; w++)
 			{
 				DmsPipeSlot* sl = &pb->slot[w];
-				if(w == wave || dms_lds_load(&sl->want) == 0u || dms_lds_load(&sl->helper) != 0u)
+				if(w == wave || dms_lds_load(&sl->want) == 0u)
 					continue;
-				unsigned int expect = 0u;
-				if(__hip_atomic_compare_exchange_strong(&sl->helper, &expect, (unsigned int) (wave + 1), __ATOMIC_ACQ_REL,
-				                                        __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP))
+				for(int r = 0; r < (MX ? 1 : 2); r++)
 				{
-					claimed = w;
-					break;
+					unsigned int* word = r ? &sl->helper2 : &sl->helper;
+					unsigned int expect = 0u;
+					if(dms_lds_load(word) == 0u
+					   && __hip_atomic_compare_exchange_strong(word, &expect, (unsigned int) (wave + 1), __ATOMIC_ACQ_REL,
+					                                           __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP))
+					{
+						claimed = w;
+						role = r;
+						break;
+					}
 				}
 			}
 		claimed = __builtin_amdgcn_readfirstlane(claimed);
+		role = __builtin_amdgcn_readfirstlane(role);
 		if(claimed >= 0)
-			dms_pipe_serve<MX, MP>(a, S, &pb->slot[claimed], lane);
+		{
+			if(role == 0)
+				dms_pipe_serve<MX, MP>(a, S, &pb->slot[claimed], lane);
+			else if constexpr(!MX)
+				dms_pipe_rates<MP>(a, S, &pb->slot[claimed], lane);
+		}
 		else
 			__builtin_amdgcn_s_sleep(8);
 	}
@@ -487,7 +557,7 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 	/* two-wave split state (wave-uniform): lone = one live lane, the others
 	 * finished; attached = a helper serves this wave; session = the open
 	 * session (0: none); tail = ring entries read */
-	bool lone = false, attached = false, held = false, seeded = false;
+	bool lone = false, attached = false, held = false, seeded = false, held_rates = false;
 	bool split_ok = PIPE && a.lone_serial == 0u;
 	unsigned int session = 0u, sessions = 0u, tail = 0u;
 	for(;;)
@@ -520,13 +590,14 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 		 * uses) stays in the ring until the post is done; a session's first
 		 * post takes the event its seed holds */
 		dms_lane_advance_post<MX>(L, c, &post_ret,
-		                          held ? &my->e[tail % DMS_PIPE_DEPTH].ev : (seeded ? &my->seed.ev : nullptr));
+		                          held ? &my->e[tail % DMS_PIPE_DEPTH].ev : (seeded ? &my->seed.ev : nullptr),
+		                          held_rates ? my->e[tail % DMS_PIPE_DEPTH].rt : nullptr);
 		seeded = false;
 		if constexpr(PIPE)
 		{
 			if(__builtin_expect(held, 0))
 			{
-				held = false;
+				held = held_rates = false;
 				tail++;
 				if(lane == 0)
 					dms_lds_store(&my->tail, tail);
@@ -815,6 +886,23 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 						}
 						held = true;
 						timed_out = false;
+						/* FP64: the entry's rate queries from a second helper,
+						 * once its progress word has passed the entry */
+						held_rates = false;
+						if constexpr(!MX)
+						{
+							if(__builtin_amdgcn_readfirstlane(dms_lds_load(&my->helper2)) != 0u)
+								for(unsigned int rp = 0; rp < DMS_PIPE_RATE_POLLS; rp++)
+								{
+									const unsigned int rh = __builtin_amdgcn_readfirstlane(dms_lds_load(&my->rhead));
+									if((int) (rh - tail) > 0)
+									{
+										held_rates = __builtin_amdgcn_readfirstlane(ep->have_rates) != 0;
+										break;
+									}
+									__builtin_amdgcn_s_sleep(1);
+								}
+						}
 						break;
 					}
 					tail++;

@@ -399,20 +399,23 @@ static int pipe_session(DmsLane& L, const DmsLaneCtx& c, int depth, uint64_t* ta
 	bool helper_ended = false;
 	DmsPipeEntry held;                 /* the step taken from the helper, as the kernel holds its ring entry */
 	const DmsStepEvent* pre = &seed.ev; /* the first post takes the seed's event */
+	const double* pre_rates = nullptr;  /* and computes its own rates */
 	for(;;)
 	{
 		while(!helper_ended && ring.size() - head < (size_t) depth)
 		{
 			DmsPipeEntry e;
 			memset(&e, 0xA7, sizeof(e));
+			e.eb = ev;
 			dms_pipe_assume_next(A, ev);
+			e.r_before = A.r_before;
 			if(!dms_pipe_prepare_step<MX>(A, c.P, c.S, c.J, e))
 				helper_ended = true;
 			else
 				ev = e.ev;
 			ring.push_back(e);
 		}
-		const int pr = dms_step_post<MX>(L, c.P, c.S, c.J, c.R, pre);
+		const int pr = dms_step_post<MX>(L, c.P, c.S, c.J, c.R, pre, pre_rates);
 		if(pr != DMS_POST_NEXT)
 		{
 			*dropped += ring.size() - head;
@@ -426,6 +429,16 @@ static int pipe_session(DmsLane& L, const DmsLaneCtx& c, int depth, uint64_t* ta
 		}
 		dms_pipe_take(L, held);
 		pre = &held.ev;
+		/* FP64: the step's rate queries evaluated from the entry alone, as the
+		 * kernel's second helper wave does (one query per lane there) */
+		pre_rates = nullptr;
+		DmsPipeRateInputs q;
+		if(!MX && dms_pipe_rate_inputs(held, c.J, q))
+		{
+			for(int k = 0; k < DMS_MAX_OPTICAL_DEPTH_PIECES + 2; k++)
+				held.rt[k] = dms_pipe_rate(held, q, c.P, c.S, k);
+			pre_rates = held.rt;
+		}
 		(*taken)++;
 	}
 }
