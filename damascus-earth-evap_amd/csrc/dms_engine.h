@@ -362,49 +362,137 @@ DMS_HD void dms_stage(const DmsProp& p, const DmsSolarView& S, double r, bool j_
 		dphi = dms_dphi_dt(p, r);
 }
 
+#if defined(__HIP_DEVICE_COMPILE__)
+/* lane l's value of x (l uniform over the wave, lane l active) */
+__device__ inline double dms_lane_value(double x, int l)
+{
+	const int lo = __builtin_amdgcn_readlane(__double2loint(x), l);
+	const int hi = __builtin_amdgcn_readlane(__double2hiint(x), l);
+	return __hiloint2double(hi, lo);
+}
+#endif
+
+/* Two RKF45 stages whose radii are both known: F(ra) into (dva, dpa) and F(rb)
+ * into (dvb, dpb), F one right-hand side evaluation (dv/dt, dphi/dt at a
+ * radius).  On the device lanes 0 and 1 of the calling wave (both active, with
+ * the same inputs; second = lane 1) evaluate one each in a single instruction
+ * stream, and v_readlane hands both results to both lanes; on the host the two
+ * evaluations run one after the other.  Each evaluation is the one a serial
+ * attempt makes, so the results are the same bits. */
+template <class F>
+DMS_HD void dms_stage_pair(F&& stage, double ra, double rb, bool second, double& dva, double& dpa, double& dvb, double& dpb)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+	double dv, dp;
+	stage(second ? rb : ra, dv, dp);
+	dva = dms_lane_value(dv, 0);
+	dpa = dms_lane_value(dp, 0);
+	dvb = dms_lane_value(dv, 1);
+	dpb = dms_lane_value(dp, 1);
+#else
+	(void) second;
+	stage(ra, dva, dpa);
+	stage(rb, dvb, dpb);
+#endif
+}
+
+/* The six stage increments of an RKF45 attempt with step h (the tableau of
+ * :1503-1540).  Stage i's radius needs the velocity increments of stages
+ * before i - 1 only, so stages 0 and 1, 2 and 3, 4 and 5 are evaluated as
+ * pairs (dms_stage_pair): three dependent right-hand side evaluations instead
+ * of six.  Every increment is the same expression of the same values as in
+ * the serial order. */
+template <class F>
+DMS_HD void dms_rk45_stages_paired(const DmsProp& p, double h, F&& stage, bool second, double* kr, double* kv, double* kp)
+{
+	double dv0, dp0, dv1, dp1;
+	kr[0] = h * p.vr;
+	dms_stage_pair(stage, p.radius, p.radius + kr[0] / 4.0, second, dv0, dp0, dv1, dp1);
+	kv[0] = h * dv0;
+	kp[0] = h * dp0;
+	kr[1] = h * (p.vr + kv[0] / 4.0);
+	kv[1] = h * dv1;
+	kp[1] = 0.0;
+
+	const double r2 = p.radius + 3.0 / 32.0 * kr[0] + 9.0 / 32.0 * kr[1];
+	kr[2] = h * (p.vr + 3.0 / 32.0 * kv[0] + 9.0 / 32.0 * kv[1]);
+	const double r3 = p.radius + 1932.0 / 2197.0 * kr[0] - 7200.0 / 2197.0 * kr[1] + 7296.0 / 2197.0 * kr[2];
+	double dv2, dp2, dv3, dp3;
+	dms_stage_pair(stage, r2, r3, second, dv2, dp2, dv3, dp3);
+	kv[2] = h * dv2;
+	kp[2] = h * dp2;
+	kv[3] = h * dv3;
+	kp[3] = h * dp3;
+
+	kr[3] = h * (p.vr + 1932.0 / 2197.0 * kv[0] - 7200.0 / 2197.0 * kv[1] + 7296.0 / 2197.0 * kv[2]);
+	const double r4 = p.radius + 439.0 / 216.0 * kr[0] - 8.0 * kr[1] + 3680.0 / 513.0 * kr[2] - 845.0 / 4104.0 * kr[3];
+	kr[4] = h * (p.vr + 439.0 / 216.0 * kv[0] - 8.0 * kv[1] + 3680.0 / 513.0 * kv[2] - 845.0 / 4104.0 * kv[3]);
+	const double r5 = p.radius - 8.0 / 27.0 * kr[0] + 2.0 * kr[1] - 3544.0 / 2565.0 * kr[2] + 1859.0 / 4104.0 * kr[3] - 11.0 / 40.0 * kr[4];
+	double dv4, dp4, dv5, dp5;
+	dms_stage_pair(stage, r4, r5, second, dv4, dp4, dv5, dp5);
+	kv[4] = h * dv4;
+	kp[4] = h * dp4;
+	kr[5] = h * (p.vr - 8.0 / 27.0 * kv[0] + 2.0 * kv[1] - 3544.0 / 2565.0 * kv[2] + 1859.0 / 4104.0 * kv[3] - 11.0 / 40.0 * kv[4]);
+	kv[5] = h * dv5;
+	kp[5] = h * dp5;
+}
+
 /* One attempt of Runge_Kutta_45_Step (:1487-1602).  Returns 1 accepted,
- * 0 forced-accepted (retry cap or minimum step: the step "fails"), -1 retry. */
-DMS_HD int dms_rk45_attempt(DmsProp& p, const DmsSolarView& S, int inner_iter)
+ * 0 forced-accepted (retry cap or minimum step: the step "fails"), -1 retry.
+ * PAIR: the stages as pairs (dms_rk45_stages_paired; on the device lanes 0
+ * and 1 of the wave run the attempt together, second = lane 1). */
+template <bool PAIR = false>
+DMS_HD int dms_rk45_attempt(DmsProp& p, const DmsSolarView& S, int inner_iter, bool second = false)
 {
 	const double tol0 = 1.0 * DMS_km, tol1 = 1.0e-3 * DMS_km / DMS_sec, tol2 = 1.0e-7;
 	const double h = p.ts;
 	double kr[6], kv[6], kp[6], ri;
 	const double aJ = dm_fabs(p.J);
 	const bool j_ok = (p.J == 0.0) | dms_in_binades(aJ, -99, 166); /* inside [1e-30, 1e50] */
-	double dv, dph = 0.0;
-	kr[0] = h * p.vr;
-	dms_stage(p, S, p.radius, j_ok, true, dv, dph);
-	kv[0] = h * dv;
-	kp[0] = h * dph;
+	if constexpr(PAIR)
+	{
+		(void) ri;
+		dms_rk45_stages_paired(
+		    p, h, [&](double r, double& dv, double& dph) { dms_stage(p, S, r, j_ok, true, dv, dph); }, second, kr, kv, kp);
+	}
+	else
+	{
+		(void) second;
+		double dv, dph = 0.0;
+		kr[0] = h * p.vr;
+		dms_stage(p, S, p.radius, j_ok, true, dv, dph);
+		kv[0] = h * dv;
+		kp[0] = h * dph;
 
-	ri = p.radius + kr[0] / 4.0;
-	kr[1] = h * (p.vr + kv[0] / 4.0);
-	dms_stage(p, S, ri, j_ok, false, dv, dph);
-	kv[1] = h * dv;
+		ri = p.radius + kr[0] / 4.0;
+		kr[1] = h * (p.vr + kv[0] / 4.0);
+		dms_stage(p, S, ri, j_ok, false, dv, dph);
+		kv[1] = h * dv;
 
-	ri = p.radius + 3.0 / 32.0 * kr[0] + 9.0 / 32.0 * kr[1];
-	kr[2] = h * (p.vr + 3.0 / 32.0 * kv[0] + 9.0 / 32.0 * kv[1]);
-	dms_stage(p, S, ri, j_ok, true, dv, dph);
-	kv[2] = h * dv;
-	kp[2] = h * dph;
+		ri = p.radius + 3.0 / 32.0 * kr[0] + 9.0 / 32.0 * kr[1];
+		kr[2] = h * (p.vr + 3.0 / 32.0 * kv[0] + 9.0 / 32.0 * kv[1]);
+		dms_stage(p, S, ri, j_ok, true, dv, dph);
+		kv[2] = h * dv;
+		kp[2] = h * dph;
 
-	ri = p.radius + 1932.0 / 2197.0 * kr[0] - 7200.0 / 2197.0 * kr[1] + 7296.0 / 2197.0 * kr[2];
-	kr[3] = h * (p.vr + 1932.0 / 2197.0 * kv[0] - 7200.0 / 2197.0 * kv[1] + 7296.0 / 2197.0 * kv[2]);
-	dms_stage(p, S, ri, j_ok, true, dv, dph);
-	kv[3] = h * dv;
-	kp[3] = h * dph;
+		ri = p.radius + 1932.0 / 2197.0 * kr[0] - 7200.0 / 2197.0 * kr[1] + 7296.0 / 2197.0 * kr[2];
+		kr[3] = h * (p.vr + 1932.0 / 2197.0 * kv[0] - 7200.0 / 2197.0 * kv[1] + 7296.0 / 2197.0 * kv[2]);
+		dms_stage(p, S, ri, j_ok, true, dv, dph);
+		kv[3] = h * dv;
+		kp[3] = h * dph;
 
-	ri = p.radius + 439.0 / 216.0 * kr[0] - 8.0 * kr[1] + 3680.0 / 513.0 * kr[2] - 845.0 / 4104.0 * kr[3];
-	kr[4] = h * (p.vr + 439.0 / 216.0 * kv[0] - 8.0 * kv[1] + 3680.0 / 513.0 * kv[2] - 845.0 / 4104.0 * kv[3]);
-	dms_stage(p, S, ri, j_ok, true, dv, dph);
-	kv[4] = h * dv;
-	kp[4] = h * dph;
+		ri = p.radius + 439.0 / 216.0 * kr[0] - 8.0 * kr[1] + 3680.0 / 513.0 * kr[2] - 845.0 / 4104.0 * kr[3];
+		kr[4] = h * (p.vr + 439.0 / 216.0 * kv[0] - 8.0 * kv[1] + 3680.0 / 513.0 * kv[2] - 845.0 / 4104.0 * kv[3]);
+		dms_stage(p, S, ri, j_ok, true, dv, dph);
+		kv[4] = h * dv;
+		kp[4] = h * dph;
 
-	ri = p.radius - 8.0 / 27.0 * kr[0] + 2.0 * kr[1] - 3544.0 / 2565.0 * kr[2] + 1859.0 / 4104.0 * kr[3] - 11.0 / 40.0 * kr[4];
-	kr[5] = h * (p.vr - 8.0 / 27.0 * kv[0] + 2.0 * kv[1] - 3544.0 / 2565.0 * kv[2] + 1859.0 / 4104.0 * kv[3] - 11.0 / 40.0 * kv[4]);
-	dms_stage(p, S, ri, j_ok, true, dv, dph);
-	kv[5] = h * dv;
-	kp[5] = h * dph;
+		ri = p.radius - 8.0 / 27.0 * kr[0] + 2.0 * kr[1] - 3544.0 / 2565.0 * kr[2] + 1859.0 / 4104.0 * kr[3] - 11.0 / 40.0 * kr[4];
+		kr[5] = h * (p.vr - 8.0 / 27.0 * kv[0] + 2.0 * kv[1] - 3544.0 / 2565.0 * kv[2] + 1859.0 / 4104.0 * kv[3] - 11.0 / 40.0 * kv[4]);
+		dms_stage(p, S, ri, j_ok, true, dv, dph);
+		kv[5] = h * dv;
+		kp[5] = h * dph;
+	}
 
 	const double r4 = p.radius + 25.0 / 216.0 * kr[0] + 1408.0 / 2565.0 * kr[2] + 2197.0 / 4104.0 * kr[3] - 1.0 / 5.0 * kr[4];
 	const double v4 = p.vr + 25.0 / 216.0 * kv[0] + 1408.0 / 2565.0 * kv[2] + 2197.0 / 4104.0 * kv[3] - 1.0 / 5.0 * kv[4];
@@ -2190,7 +2278,11 @@ DMS_HD void dms_lane_advance_b(DmsLane& L, const DmsLaneCtx& c)
 
 /* Session start: the propagator after the accepted RKF45 attempt of the step
  * whose post the trajectory's wave is about to run, and its event (which that
- * post takes too). */
+ * post takes too).  A redo session (redo = 1) starts instead from a step the
+ * post rejected on optical depth: the propagator is back at the step's start
+ * with the reduced step size, and the helper's first step is that step's
+ * preparation and attempts from the lane's own start point (eb, r_before,
+ * v_before) and cached start rate. */
 struct DmsPipeSeed
 {
 	DmsProp prop;
@@ -2198,8 +2290,13 @@ struct DmsPipeSeed
 	uint64_t rk_all;
 	double time_origin;
 	uint32_t job;
-	uint32_t pad;
+	uint32_t redo;
 	DmsStepEvent ev;
+	/* redo only: the step's start */
+	DmsStepEvent eb; /* speed = v_before */
+	double r_before, rate_before;
+	float rate_before_f;
+	uint32_t pad;
 };
 
 /* One step prepared by the helper: what dms_step_prep and the RKF45 attempts
@@ -2236,17 +2333,63 @@ DMS_HD void dms_pipe_make_seed(const DmsLane& L, uint32_t job, DmsPipeSeed& s)
 	s.rk_all = L.rk_all;
 	s.time_origin = L.time_origin;
 	s.job = job;
-	s.pad = 0;
+	s.redo = 0;
 	dms_step_event<MX>(L.prop, s.ev);
 }
 
+/* The seed of a redo session: the lane after dms_step_post rejected its step
+ * on optical depth (returned -2 with od_retries > 0), before its own
+ * dms_step_prep. */
+DMS_HD void dms_pipe_make_redo_seed(const DmsLane& L, uint32_t job, DmsPipeSeed& s)
+{
+	s.prop = L.prop;
+	s.time_steps = L.time_steps;
+	s.rk_all = L.rk_all;
+	s.time_origin = L.time_origin;
+	s.job = job;
+	s.redo = 1;
+	s.eb.t = L.eb_t;
+	for(int c = 0; c < 3; c++)
+	{
+		s.eb.r[c] = L.eb_r[c];
+		s.eb.v[c] = L.eb_v[c];
+	}
+	s.eb.speed = L.v_before;
+	s.r_before = L.r_before;
+	s.rate_before = L.rate_before;
+	s.rate_before_f = L.rate_before_f;
+	s.pad = 0;
+}
+
+/* A helper's lane state from a seed: the fields dms_step_prep and the RKF45
+ * attempts read (for a redo seed, the step's start too: eb_valid and
+ * rb_cached as the rejection left them). */
 DMS_HD void dms_pipe_from_seed(DmsLane& A, const DmsPipeSeed& s)
 {
 	A.prop = s.prop;
 	A.time_steps = s.time_steps;
 	A.rk_all = s.rk_all;
 	A.time_origin = s.time_origin;
+	if(s.redo)
+	{
+		A.eb_t = s.eb.t;
+		for(int c = 0; c < 3; c++)
+		{
+			A.eb_r[c] = s.eb.r[c];
+			A.eb_v[c] = s.eb.v[c];
+		}
+		A.r_before = s.r_before;
+		A.v_before = s.eb.speed;
+		A.eb_valid = 1;
+		A.rb_cached = 1;
+		A.rate_before = s.rate_before;
+		A.rate_before_f = s.rate_before_f;
+	}
 }
+
+/* dms_step_post's optical-depth rejection: the lane goes on with
+ * dms_step_prep from the same step start. */
+DMS_HD bool dms_post_rejected(const DmsLane& L, int post_ret) { return (post_ret == -2) & (L.od_retries > 0) & (L.eb_valid != 0); }
 
 /* Helper side: the updates of dms_step_post's DMS_POST_NEXT exit that the next
  * dms_step_prep reads, from the event ev of the propagator state (the rate at
@@ -2293,7 +2436,8 @@ DMS_HD void dms_pipe_fill_entry(const DmsLane& A, int r, DmsPipeEntry& e)
 }
 
 /* Helper side: the next step's preparation and RKF45 attempts (as
- * dms_lane_advance_b and dms_lane_advance_a run them) into e.  Returns false
+ * dms_lane_advance_b and dms_lane_advance_a run them, the attempts with paired
+ * stages as the helper wave runs them) into e.  Returns false
  * when the preparation ends the segment (e.kind DMS_PIPE_PREP_END). */
 template <bool MX = false>
 DMS_HD bool dms_pipe_prepare_step(DmsLane& A, const DmsPointView& P, const DmsSolarView& S, const DmsJobView& J, DmsPipeEntry& e)
@@ -2308,7 +2452,7 @@ DMS_HD bool dms_pipe_prepare_step(DmsLane& A, const DmsPointView& P, const DmsSo
 	{
 		A.inner_iter++;
 		A.rk_all++;
-		r = MX ? dms_rk45_attempt_mx(A.prop, S, A.inner_iter) : dms_rk45_attempt(A.prop, S, A.inner_iter);
+		r = MX ? dms_rk45_attempt_mx<true>(A.prop, S, A.inner_iter) : dms_rk45_attempt<true>(A.prop, S, A.inner_iter);
 	} while(r < 0);
 	dms_pipe_fill_entry<MX>(A, r, e);
 	return true;

@@ -59,8 +59,11 @@ DMS_HD double dms_mass_mx(const DmsSolarView& S, double r)
 
 /* One RKF45 attempt (Runge_Kutta_45_Step, src/Simulation_Trajectory.cpp:1487-1602)
  * with the right-hand side J^2/r^3 - G M(r)/r^2, J/r^2 from one refined
- * reciprocal per stage. */
-DMS_HD int dms_rk45_attempt_mx(DmsProp& p, const DmsSolarView& S, int inner_iter)
+ * reciprocal per stage.  PAIR: the stages as pairs (dms_rk45_stages_paired;
+ * on the device lanes 0 and 1 of the wave run the attempt together, second =
+ * lane 1). */
+template <bool PAIR = false>
+DMS_HD int dms_rk45_attempt_mx(DmsProp& p, const DmsSolarView& S, int inner_iter, bool second = false)
 {
 	const double tol0 = 1.0 * DMS_km, tol1 = 1.0e-3 * DMS_km / DMS_sec, tol2 = 1.0e-7;
 	const double h = p.ts;
@@ -77,35 +80,41 @@ DMS_HD int dms_rk45_attempt_mx(DmsProp& p, const DmsSolarView& S, int inner_iter
 		dv = (J2 * q - DMS_G_Newton * m) * (q * q);
 		dphi = (angular && ri > 0.0) ? p.J * (qu * qu) : 0.0;
 	};
-	double dv, dp;
-	rhs(p.radius, dv, dp);
-	kr[0] = h * p.vr;
-	kv[0] = h * dv;
-	kp[0] = h * dp;
-	double ri = p.radius + kr[0] / 4.0;
-	kr[1] = h * (p.vr + kv[0] / 4.0);
-	rhs(ri, dv, dp);
-	kv[1] = h * dv;
-	ri = p.radius + 3.0 / 32.0 * kr[0] + 9.0 / 32.0 * kr[1];
-	kr[2] = h * (p.vr + 3.0 / 32.0 * kv[0] + 9.0 / 32.0 * kv[1]);
-	rhs(ri, dv, dp);
-	kv[2] = h * dv;
-	kp[2] = h * dp;
-	ri = p.radius + 1932.0 / 2197.0 * kr[0] - 7200.0 / 2197.0 * kr[1] + 7296.0 / 2197.0 * kr[2];
-	kr[3] = h * (p.vr + 1932.0 / 2197.0 * kv[0] - 7200.0 / 2197.0 * kv[1] + 7296.0 / 2197.0 * kv[2]);
-	rhs(ri, dv, dp);
-	kv[3] = h * dv;
-	kp[3] = h * dp;
-	ri = p.radius + 439.0 / 216.0 * kr[0] - 8.0 * kr[1] + 3680.0 / 513.0 * kr[2] - 845.0 / 4104.0 * kr[3];
-	kr[4] = h * (p.vr + 439.0 / 216.0 * kv[0] - 8.0 * kv[1] + 3680.0 / 513.0 * kv[2] - 845.0 / 4104.0 * kv[3]);
-	rhs(ri, dv, dp);
-	kv[4] = h * dv;
-	kp[4] = h * dp;
-	ri = p.radius - 8.0 / 27.0 * kr[0] + 2.0 * kr[1] - 3544.0 / 2565.0 * kr[2] + 1859.0 / 4104.0 * kr[3] - 11.0 / 40.0 * kr[4];
-	kr[5] = h * (p.vr - 8.0 / 27.0 * kv[0] + 2.0 * kv[1] - 3544.0 / 2565.0 * kv[2] + 1859.0 / 4104.0 * kv[3] - 11.0 / 40.0 * kv[4]);
-	rhs(ri, dv, dp);
-	kv[5] = h * dv;
-	kp[5] = h * dp;
+	if constexpr(PAIR)
+		dms_rk45_stages_paired(p, h, rhs, second, kr, kv, kp);
+	else
+	{
+		(void) second;
+		double dv, dp;
+		rhs(p.radius, dv, dp);
+		kr[0] = h * p.vr;
+		kv[0] = h * dv;
+		kp[0] = h * dp;
+		double ri = p.radius + kr[0] / 4.0;
+		kr[1] = h * (p.vr + kv[0] / 4.0);
+		rhs(ri, dv, dp);
+		kv[1] = h * dv;
+		ri = p.radius + 3.0 / 32.0 * kr[0] + 9.0 / 32.0 * kr[1];
+		kr[2] = h * (p.vr + 3.0 / 32.0 * kv[0] + 9.0 / 32.0 * kv[1]);
+		rhs(ri, dv, dp);
+		kv[2] = h * dv;
+		kp[2] = h * dp;
+		ri = p.radius + 1932.0 / 2197.0 * kr[0] - 7200.0 / 2197.0 * kr[1] + 7296.0 / 2197.0 * kr[2];
+		kr[3] = h * (p.vr + 1932.0 / 2197.0 * kv[0] - 7200.0 / 2197.0 * kv[1] + 7296.0 / 2197.0 * kv[2]);
+		rhs(ri, dv, dp);
+		kv[3] = h * dv;
+		kp[3] = h * dp;
+		ri = p.radius + 439.0 / 216.0 * kr[0] - 8.0 * kr[1] + 3680.0 / 513.0 * kr[2] - 845.0 / 4104.0 * kr[3];
+		kr[4] = h * (p.vr + 439.0 / 216.0 * kv[0] - 8.0 * kv[1] + 3680.0 / 513.0 * kv[2] - 845.0 / 4104.0 * kv[3]);
+		rhs(ri, dv, dp);
+		kv[4] = h * dv;
+		kp[4] = h * dp;
+		ri = p.radius - 8.0 / 27.0 * kr[0] + 2.0 * kr[1] - 3544.0 / 2565.0 * kr[2] + 1859.0 / 4104.0 * kr[3] - 11.0 / 40.0 * kr[4];
+		kr[5] = h * (p.vr - 8.0 / 27.0 * kv[0] + 2.0 * kv[1] - 3544.0 / 2565.0 * kv[2] + 1859.0 / 4104.0 * kv[3] - 11.0 / 40.0 * kv[4]);
+		rhs(ri, dv, dp);
+		kv[5] = h * dv;
+		kp[5] = h * dp;
+	}
 	const double r4 = p.radius + 25.0 / 216.0 * kr[0] + 1408.0 / 2565.0 * kr[2] + 2197.0 / 4104.0 * kr[3] - 1.0 / 5.0 * kr[4];
 	const double v4 = p.vr + 25.0 / 216.0 * kv[0] + 1408.0 / 2565.0 * kv[2] + 2197.0 / 4104.0 * kv[3] - 1.0 / 5.0 * kv[4];
 	const double p4 = p.phi + 25.0 / 216.0 * kp[0] + 1408.0 / 2565.0 * kp[2] + 2197.0 / 4104.0 * kp[3] - 1.0 / 5.0 * kp[4];

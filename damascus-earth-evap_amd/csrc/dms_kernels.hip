@@ -302,22 +302,25 @@ __device__ __forceinline__ void dms_lds_store(unsigned int* p, unsigned int v)
  * its own */
 #define DMS_PIPE_RATE_POLLS 256u
 
-/* A helper wave serving slot sl (its lane 0 computes): for each session the
- * trajectory's wave opens, it prepares the following steps, assuming each
- * post continues, and writes them to the ring.  A session that changes is
- * noticed between the preparation and every RKF45 attempt.  Returns when the
- * trajectory's wave has finished all its lanes (want = 0). */
+/* A helper wave serving slot sl (its lanes 0 and 1 compute, with the same
+ * values except inside the RKF45 attempts, whose stages they evaluate in
+ * pairs; lane 0 writes the ring): for each session the trajectory's wave
+ * opens, it prepares the following steps, assuming each post continues, and
+ * writes them to the ring.  A session that changes is noticed between the
+ * preparation and every RKF45 attempt.  Returns when the trajectory's wave has
+ * finished all its lanes (want = 0). */
 template <bool MX, bool MP>
 __device__ __forceinline__ void dms_pipe_serve(const DmsTrajArgs& a, const DmsSolarView& S, DmsPipeSlot* sl, int lane)
 {
-	if(lane != 0)
+	if(lane > 1)
 		return;
+	const bool second = lane == 1;
 	DmsLane A;
 	DmsStepEvent ev;
 	DmsPointView P = a.P;
 	uint32_t job = 0;
 	unsigned int cur = 0, head = dms_lds_load(&sl->head);
-	bool ended = true;
+	bool ended = true, redo_first = false;
 	for(;;)
 	{
 		if(dms_lds_load(&sl->want) == 0u)
@@ -343,7 +346,9 @@ __device__ __forceinline__ void dms_pipe_serve(const DmsTrajArgs& a, const DmsSo
 				job = seed.job;
 				if constexpr(MP)
 					P = a.job_points[job];
-				ev = seed.ev;
+				/* a redo session's first step starts where the rejected one did */
+				redo_first = seed.redo != 0u;
+				ev = redo_first ? seed.eb : seed.ev;
 			}
 		}
 		if(ended)
@@ -353,7 +358,9 @@ __device__ __forceinline__ void dms_pipe_serve(const DmsTrajArgs& a, const DmsSo
 		}
 		DmsPipeEntry e;
 		e.eb = ev;
-		dms_pipe_assume_next(A, ev);
+		if(!redo_first)
+			dms_pipe_assume_next(A, ev);
+		redo_first = false;
 		e.r_before = A.r_before;
 		e.job = job;
 		if(dms_step_prep<MX>(A, P, S, a.J) >= 0)
@@ -374,7 +381,8 @@ __device__ __forceinline__ void dms_pipe_serve(const DmsTrajArgs& a, const DmsSo
 				}
 				A.inner_iter++;
 				A.rk_all++;
-				r = MX ? dms_rk45_attempt_mx(A.prop, S, A.inner_iter) : dms_rk45_attempt(A.prop, S, A.inner_iter);
+				r = MX ? dms_rk45_attempt_mx<true>(A.prop, S, A.inner_iter, second)
+				       : dms_rk45_attempt<true>(A.prop, S, A.inner_iter, second);
 			} while(r < 0);
 			if(moved)
 				continue;
@@ -395,9 +403,12 @@ __device__ __forceinline__ void dms_pipe_serve(const DmsTrajArgs& a, const DmsSo
 		}
 		if(stale || dms_lds_load(&sl->session) != cur)
 			continue;
-		sl->e[head % DMS_PIPE_DEPTH] = e;
+		if(!second)
+		{
+			sl->e[head % DMS_PIPE_DEPTH] = e;
+			dms_lds_store(&sl->head, head + 1);
+		}
 		head++;
-		dms_lds_store(&sl->head, head);
 	}
 }
 
@@ -608,6 +619,23 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 			{
 				session = 0u;
 				dms_lds_store(&my->session, 0u);
+			}
+			/* a step rejected on optical depth: the helper prepares it again
+			 * (and the steps after it) from a redo seed, while this wave waits
+			 * for the entry instead of running the preparation and the RKF45
+			 * attempts itself */
+			if(__builtin_expect(attached & (session == 0u), 0)
+			   && __ballot((L.phase == PH_PREP) & dms_post_rejected(L, post_ret)) != 0)
+			{
+				session = ++sessions;
+				if(L.phase == PH_PREP)
+				{
+					/* the seed's stores stay after the previous session's end,
+					 * as for any seed */
+					__builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+					dms_pipe_make_redo_seed(L, L.job, my->seed);
+					dms_lds_store(&my->session, session);
+				}
 			}
 		}
 		/* The wave-level services below (target sampling, scatter completion,

@@ -372,13 +372,17 @@ extern "C++" {
 /* The two-wave split of a lone trajectory, serially: from the step whose
  * RKF45 attempts the trajectory's own lane has just run, a helper lane (all
  * fields it is not given hold garbage) prepares up to `depth` further steps
- * assuming each post continues, and the post takes them (dms_pipe_take);
- * every other post exit discards them.  Returns the last post's value, or
+ * assuming each post continues, and the post takes them (dms_pipe_take).  A
+ * post that rejects its step on optical depth opens a redo session: a fresh
+ * helper lane prepares that step again from the redo seed (and the steps after
+ * it), and the next post takes its first step.  Every other post exit
+ * discards the helper's steps.  Returns the last post's value, or
  * DMS_POST_NEXT when the helper's preparation ended the segment (the lane
  * then runs its own preparation, which ends it the same way).  *taken counts
- * the steps taken from the helper, *dropped the ones discarded. */
+ * the steps taken from the helper, *dropped the ones discarded, *redos the
+ * redo sessions. */
 template <bool MX>
-static int pipe_session(DmsLane& L, const DmsLaneCtx& c, int depth, uint64_t* taken, uint64_t* dropped)
+static int pipe_session(DmsLane& L, const DmsLaneCtx& c, int depth, uint64_t* taken, uint64_t* dropped, uint64_t* redos)
 {
 	int r;
 	do
@@ -394,20 +398,23 @@ static int pipe_session(DmsLane& L, const DmsLaneCtx& c, int depth, uint64_t* ta
 	memset(&A, 0xC3, sizeof(A));
 	dms_pipe_from_seed(A, seed);
 	DmsStepEvent ev = seed.ev;
+	bool redo_first = false;
 	std::vector<DmsPipeEntry> ring;
 	size_t head = 0;
 	bool helper_ended = false;
-	DmsPipeEntry held;                 /* the step taken from the helper, as the kernel holds its ring entry */
-	const DmsStepEvent* pre = &seed.ev; /* the first post takes the seed's event */
-	const double* pre_rates = nullptr;  /* and computes its own rates */
-	for(;;)
-	{
+	auto fill = [&]() {
 		while(!helper_ended && ring.size() - head < (size_t) depth)
 		{
 			DmsPipeEntry e;
 			memset(&e, 0xA7, sizeof(e));
-			e.eb = ev;
-			dms_pipe_assume_next(A, ev);
+			if(redo_first)
+				e.eb = seed.eb;
+			else
+			{
+				e.eb = ev;
+				dms_pipe_assume_next(A, ev);
+			}
+			redo_first = false;
 			e.r_before = A.r_before;
 			if(!dms_pipe_prepare_step<MX>(A, c.P, c.S, c.J, e))
 				helper_ended = true;
@@ -415,11 +422,29 @@ static int pipe_session(DmsLane& L, const DmsLaneCtx& c, int depth, uint64_t* ta
 				ev = e.ev;
 			ring.push_back(e);
 		}
+	};
+	DmsPipeEntry held;                 /* the step taken from the helper, as the kernel holds its ring entry */
+	const DmsStepEvent* pre = &seed.ev; /* the first post takes the seed's event */
+	const double* pre_rates = nullptr;  /* and computes its own rates */
+	for(;;)
+	{
+		fill();
 		const int pr = dms_step_post<MX>(L, c.P, c.S, c.J, c.R, pre, pre_rates);
 		if(pr != DMS_POST_NEXT)
 		{
 			*dropped += ring.size() - head;
-			return pr;
+			if(!dms_post_rejected(L, pr))
+				return pr;
+			/* a redo session from a fresh helper */
+			ring.clear();
+			head = 0;
+			helper_ended = false;
+			dms_pipe_make_redo_seed(L, L.job, seed);
+			memset(&A, 0xC3, sizeof(A));
+			dms_pipe_from_seed(A, seed);
+			redo_first = true;
+			(*redos)++;
+			fill();
 		}
 		held = ring[head++];
 		if(held.kind == DMS_PIPE_PREP_END)
@@ -445,7 +470,7 @@ static int pipe_session(DmsLane& L, const DmsLaneCtx& c, int depth, uint64_t* ta
 
 template <bool MX>
 static void replay_lane_pipe(DmsLane& L, const DmsLaneCtx& c, const dms_event& ic, bool shift_ok, dms_traj_record* rec,
-                             DmsSummary* sum, int depth, uint64_t* taken, uint64_t* dropped)
+                             DmsSummary* sum, int depth, uint64_t* taken, uint64_t* dropped, uint64_t* redos)
 {
 	dms_lane_start(L, c, ic, shift_ok, rec, sum);
 	if(L.phase == PH_NEED)
@@ -455,7 +480,7 @@ static void replay_lane_pipe(DmsLane& L, const DmsLaneCtx& c, const dms_event& i
 	{
 		if(L.phase == PH_RK)
 		{
-			const int pr = pipe_session<MX>(L, c, depth, taken, dropped);
+			const int pr = pipe_session<MX>(L, c, depth, taken, dropped, redos);
 			L.phase = ((pr == -2) | (pr == DMS_POST_NEXT)) ? (int) PH_PREP
 			                                                : dms_after_propagation(L, (pr == -3) ? (int) DMS_SCATTER : pr);
 		}
@@ -475,7 +500,8 @@ static void replay_lane_pipe(DmsLane& L, const DmsLaneCtx& c, const dms_event& i
 } /* extern "C++" */
 
 /* rp_simulate with every step loop run through the two-wave split;
- * stats[0] steps taken from the helper, stats[1] steps discarded. */
+ * stats[0] steps taken from the helper, stats[1] steps discarded, stats[2]
+ * redo sessions. */
 int rp_simulate_pipe(rp_point* p, const dms_job_desc* job, const dms_event* ics, size_t n, dms_traj_record* recs,
                      double* rows, int depth, uint64_t* stats)
 {
@@ -495,7 +521,7 @@ int rp_simulate_pipe(rp_point* p, const dms_job_desc* job, const dms_event* ics,
 		return -1;
 	if(rows)
 		memset(rows, 0, sizeof(double) * 2 * (size_t) J.nbins * n);
-	stats[0] = stats[1] = 0;
+	stats[0] = stats[1] = stats[2] = 0;
 	for(size_t i = 0; i < n; i++)
 	{
 		DmsLane L;
@@ -506,9 +532,9 @@ int rp_simulate_pipe(rp_point* p, const dms_job_desc* job, const dms_event* ics,
 		L.job = 0;
 		L.id = J.first_id + i;
 		if(c.J.mixed)
-			replay_lane_pipe<true>(L, c, ics[i], true, &recs[i], nullptr, depth, &stats[0], &stats[1]);
+			replay_lane_pipe<true>(L, c, ics[i], true, &recs[i], nullptr, depth, &stats[0], &stats[1], &stats[2]);
 		else
-			replay_lane_pipe<false>(L, c, ics[i], true, &recs[i], nullptr, depth, &stats[0], &stats[1]);
+			replay_lane_pipe<false>(L, c, ics[i], true, &recs[i], nullptr, depth, &stats[0], &stats[1], &stats[2]);
 	}
 	return 0;
 }

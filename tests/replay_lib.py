@@ -112,10 +112,10 @@ class ReplayPoint:
         n = len(events)
         recs = dmsun.record_array(n)
         rows = np.zeros((n, 2, job.n_bins)) if want_rows else None
-        stats = np.zeros(2, dtype=np.uint64)
+        stats = np.zeros(3, dtype=np.uint64)
         assert lib().rp_simulate_pipe(self.h, C.byref(job), events, n, recs, dmsun.dptr(rows) if want_rows else None,
                                       depth, stats.ctypes.data_as(_P(C.c_uint64))) == 0
-        return recs, rows, int(stats[0]), int(stats[1])
+        return recs, rows, int(stats[0]), int(stats[1]), int(stats[2])
 
     def lane_hist(self, job, events, ok, nlanes=3):
         """Class sums through per-lane scratch rows (serial, id order)."""

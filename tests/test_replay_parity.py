@@ -85,18 +85,19 @@ def test_trajectories_and_rows(replay, oracle_solar, mode, desc_args, n, max_sca
 def test_two_wave_split_is_the_serial_trajectory(replay, oracle_solar, mode, desc_args, n, max_scat, depth):
     """A lone trajectory's steps prepared ahead by a helper (the kernel's
     two-wave split) give the oracle's records and rows bit for bit; the helper
-    starts from garbage in every field it is not given, and both helper steps
-    taken and steps discarded (at optical-depth rejections, scatterings,
-    Kepler returns) occur."""
+    starts from garbage in every field it is not given, and helper steps taken,
+    steps discarded (at optical-depth rejections, scatterings, Kepler returns)
+    and redo sessions (a rejected step prepared again by a fresh helper) all
+    occur."""
     desc = dmsun.make_point(**desc_args)
     job = dmsun.make_job(mode=mode, max_trajectories=n, max_scatterings=max_scat, n_bins=400)
     ev, ok = O.initial_conditions(oracle_solar, job.halo, job.seed, 0, 0, n)
     r1, rows1, _ = O.Point(oracle_solar, desc).simulate_events(job, ev, want_rows=True)
-    r2, rows2, taken, dropped = replay.point(desc).simulate_events_pipe(job, ev, depth, want_rows=True)
+    r2, rows2, taken, dropped, redos = replay.point(desc).simulate_events_pipe(job, ev, depth, want_rows=True)
     same = (rec_bytes(r1) == rec_bytes(r2)).all(axis=1)
     assert same.all(), f"{(~same).sum()} of {n} records differ"
     assert np.array_equal(rows1.view(np.uint64), rows2.view(np.uint64))
-    assert taken > 0 and dropped > 0
+    assert taken > 0 and dropped > 0 and redos > 0
 
 
 def test_two_wave_split_long_trajectory_both_precisions(replay, oracle_solar):
@@ -104,7 +105,7 @@ def test_two_wave_split_long_trajectory_both_precisions(replay, oracle_solar):
     m = 0.3 GeV, sigma_p = 1e-36: 1.8e5 RKF45 attempts, a quarter of its steps
     rejected on optical depth, excursions outside the Sun) through the split
     equals its serial replay bit for bit, FP64 (also against the oracle) and
-    mixed precision."""
+    mixed precision, with each rejected step redone by a fresh helper."""
     desc = dmsun.make_point(0.3, 1e-36, interpolation_points=1000)
     for precision in (dmsun.FP64, dmsun.MIXED):
         job = dmsun.make_job(mode=dmsun.MODE_PARAMETER_POINT, max_trajectories=1, max_scatterings=10**14, n_bins=1000,
@@ -112,11 +113,11 @@ def test_two_wave_split_long_trajectory_both_precisions(replay, oracle_solar):
         ev, ok = O.initial_conditions(oracle_solar, job.halo, job.seed, 0, 65143, 1)
         rp = replay.point(desc)
         serial, rows_s = rp.simulate_events(job, ev, want_rows=True)
-        split, rows_p, taken, dropped = rp.simulate_events_pipe(job, ev, 2, want_rows=True)
+        split, rows_p, taken, dropped, redos = rp.simulate_events_pipe(job, ev, 2, want_rows=True)
         assert serial[0].n_rk_attempts > 100000 and serial[0].n_tau_rejections > 10000
         assert np.array_equal(rec_bytes(serial), rec_bytes(split))
         assert np.array_equal(rows_s.view(np.uint64), rows_p.view(np.uint64))
-        assert taken > 50000 and dropped > 10000
+        assert taken > 50000 and dropped > 10000 and redos > 10000
         if precision == dmsun.FP64:
             r1, _, _ = O.Point(oracle_solar, desc).simulate_events(job, ev)
             assert np.array_equal(rec_bytes(r1), rec_bytes(split))
