@@ -92,6 +92,11 @@ struct DmsLane
 	int cur_bin;
 	double acc_dt, acc_v2;
 	uint32_t acc_cnt;
+	/* the bin of prev_r_km (-1: none) and its stored values, read when the
+	 * anchor moved (dms_bin_anchor) */
+	int pf_bin;
+	uint32_t pf_cnt;
+	double pf_dt, pf_v2;
 	/* work counters */
 	uint64_t rk_all;
 	uint32_t n_int, n_ext, n_rej, n_tv;
@@ -1099,10 +1104,10 @@ DMS_HD void dms_bin_flush(DmsLane& L, const DmsRows& R, int nbins)
 	L.cur_bin = -1;
 }
 
-DMS_HD void dms_accumulate_bincount(DmsLane& L, const DmsRows& R, const DmsJobView& J, double r_km, double v2, double dt_sec)
+/* Accumulate_Bincount_Step's bin of a radius in km: -1 outside [0, 2 R_sun)
+ * or past the last bin. */
+DMS_HD int dms_bin_index(const DmsJobView& J, double r_km)
 {
-	if(!dm_isfinite(dt_sec) | (dt_sec <= 0.0))
-		return;
 	const double bin_max = 2.0 * DMS_R_SUN_KM;
 	int bin = -1;
 	if(dm_isfinite(r_km) & (r_km >= 0.0) & (r_km < bin_max))
@@ -1113,34 +1118,60 @@ DMS_HD void dms_accumulate_bincount(DmsLane& L, const DmsRows& R, const DmsJobVi
 		if(bin >= J.nbins)
 			bin = -1;
 	}
+	return bin;
+}
+
+/* The bincount anchor (prev_r_km) moved: the next accumulation falls into its
+ * bin, whose stored values are read now, a step before they are used, so that
+ * the load is not waited for in the step that switches bins.  They are read
+ * whatever the bin (bin 0 for none): when it is the bin in registers they are
+ * not used.  No store to that bin's record can come between this read and the
+ * switch: a record is written only when the lane leaves its bin. */
+DMS_HD void dms_bin_anchor(DmsLane& L, const DmsRows& R, const DmsJobView& J)
+{
+	const int bin = dms_bin_index(J, L.prev_r_km);
+	L.pf_bin = bin;
+	const size_t b = (size_t) (bin < 0 ? 0 : bin);
+	if(R.per_lane)
+	{
+		const DmsBinAcc* rec = R.bins + (size_t) L.row * (size_t) J.nbins + b;
+		L.pf_dt = rec->dt;
+		L.pf_v2 = rec->v2dt;
+		L.pf_cnt = rec->cnt;
+		return;
+	}
+	if(R.rows)
+	{
+		const double* row = R.rows + (size_t) L.row * 2 * (size_t) J.nbins;
+		L.pf_dt = row[b];
+		L.pf_v2 = row[J.nbins + b];
+	}
+	else
+	{
+		L.pf_dt = 0.0;
+		L.pf_v2 = 0.0;
+	}
+	L.pf_cnt = R.counts ? R.counts[(size_t) L.row * (size_t) J.nbins + b] : 0u;
+}
+
+/* One step of the bincount into the bin of prev_r_km (dms_bin_anchor read it
+ * and its stored values when the anchor was set). */
+DMS_HD void dms_accumulate_bincount(DmsLane& L, const DmsRows& R, const DmsJobView& J, double v2, double dt_sec)
+{
+	if(!dm_isfinite(dt_sec) | (dt_sec <= 0.0))
+		return;
+	const int bin = L.pf_bin;
 	if(bin < 0)
 		return;
 	if(bin != L.cur_bin)
 	{
 		dms_bin_flush(L, R, J.nbins);
 		L.cur_bin = bin;
-		if(R.per_lane)
-		{
-			const DmsBinAcc* b = R.bins + (size_t) L.row * (size_t) J.nbins + bin;
-			L.acc_dt = b->dt;
-			L.acc_v2 = b->v2dt;
-			L.acc_cnt = b->cnt;
-			if(L.acc_cnt == 0u)
-				R.touched[(size_t) L.row * (size_t) J.nbins + L.n_touched++] = (uint16_t) bin;
-		}
-		else if(R.rows)
-		{
-			const double* row = R.rows + (size_t) L.row * 2 * (size_t) J.nbins;
-			L.acc_dt = row[bin];
-			L.acc_v2 = row[J.nbins + bin];
-		}
-		else
-		{
-			L.acc_dt = 0.0;
-			L.acc_v2 = 0.0;
-		}
-		if(!R.per_lane)
-			L.acc_cnt = R.counts ? R.counts[(size_t) L.row * (size_t) J.nbins + bin] : 0u;
+		L.acc_dt = L.pf_dt;
+		L.acc_v2 = L.pf_v2;
+		L.acc_cnt = L.pf_cnt;
+		if(R.per_lane && L.acc_cnt == 0u)
+			R.touched[(size_t) L.row * (size_t) J.nbins + L.n_touched++] = (uint16_t) bin;
 	}
 	const double v2dt = v2 * dt_sec;
 	L.acc_dt += dt_sec;
@@ -1162,11 +1193,12 @@ DMS_HD int dms_commit(DmsLane& L, const DmsRows& R, const DmsJobView& J, const D
 		if(L.prev_time_sec >= 0.0)
 		{
 			const double dts = t_now - L.prev_time_sec;
-			dms_accumulate_bincount(L, R, J, L.prev_r_km, L.prev_v2, dts);
+			dms_accumulate_bincount(L, R, J, L.prev_v2, dts);
 		}
 		L.prev_time_sec = t_now;
 		L.prev_r_km = r_now;
 		L.prev_v2 = v2;
+		dms_bin_anchor(L, R, J);
 	}
 	return dms_update_capture_state(L, S, dm_mass, J.terminate_on_capture, dms_norm(lr), dms_norm(lv), abs_t, 0);
 }
@@ -1865,6 +1897,7 @@ DMS_HD void dms_trajectory_begin(DmsLane& L, const DmsSolarView& S, double dm_ma
 	L.acc_dt = 0.0;
 	L.acc_v2 = 0.0;
 	L.acc_cnt = 0;
+	L.pf_bin = -1;
 	L.n_touched = 0;
 	L.rk_all = 0;
 	L.n_int = 0;
@@ -1883,7 +1916,7 @@ DMS_HD int dms_after_propagation(DmsLane& L, int pr)
 }
 
 /* Scatter bookkeeping after a successful Scatter (:1327-1341). Returns next phase. */
-DMS_HD int dms_after_scatter(DmsLane& L, const DmsSolarView& S, double dm_mass, const DmsJobView& J)
+DMS_HD int dms_after_scatter(DmsLane& L, const DmsSolarView& S, double dm_mass, const DmsJobView& J, const DmsRows& R)
 {
 	L.n_scat++;
 	const int cap = dms_update_capture_state(L, S, dm_mass, J.terminate_on_capture, dms_norm(L.ev_r), dms_norm(L.ev_v), L.ev_t, 1);
@@ -1894,6 +1927,7 @@ DMS_HD int dms_after_scatter(DmsLane& L, const DmsSolarView& S, double dm_mass, 
 		L.prev_r_km = dms_norm(L.ev_r) / DMS_km;
 		const double v = dms_norm(L.ev_v) / (DMS_km / DMS_sec);
 		L.prev_v2 = v * v;
+		dms_bin_anchor(L, R, J);
 	}
 	if(J.terminate_on_capture && cap)
 	{
@@ -2184,7 +2218,7 @@ DMS_HD void dms_lane_finish_scatter(DmsLane& L, const DmsLaneCtx& c)
 		L.phase = PH_FIN;
 	}
 	else
-		L.phase = dms_after_scatter(L, c.S, c.P.mass, c.J);
+		L.phase = dms_after_scatter(L, c.S, c.P.mass, c.J, c.R);
 }
 
 /* Trajectory end up to the per-lane row flush.  Returns the class of the

@@ -446,10 +446,11 @@ DMS_HD void dms_commit_mx(DmsLane& L, const DmsRows& R, const DmsJobView& J, con
 	if(!J.terminate_on_capture)
 	{
 		if(L.prev_time_sec >= 0.0)
-			dms_accumulate_bincount(L, R, J, L.prev_r_km, L.prev_v2, t_now - L.prev_time_sec);
+			dms_accumulate_bincount(L, R, J, L.prev_v2, t_now - L.prev_time_sec);
 		L.prev_time_sec = t_now;
 		L.prev_r_km = radius * (1.0 / DMS_km);
 		L.prev_v2 = speed2 * ((DMS_sec / DMS_km) * (DMS_sec / DMS_km));
+		dms_bin_anchor(L, R, J);
 	}
 	L.last_E = dm_nan();
 	if(J.terminate_on_capture && !L.captured)
