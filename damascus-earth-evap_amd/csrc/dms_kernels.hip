@@ -680,7 +680,7 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 					/* FP32 rates (1/s), same draw and order as below */
 					float rate = 0.0f;
 					if(k >= 0)
-						rate = dms_rate_target_mx(Ps, a.S, r, v, k);
+						rate = dms_rate_target_mx(Ps, c.S, r, v, k);
 					const bool bad = (k >= 0) && (!mx_isfinite(rate) || rate < 0.0f);
 					if(__ballot(bad) == 0)
 					{
@@ -712,7 +712,7 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 				}
 				double rate = 0.0;
 				if(k >= 0)
-					rate = dms_rate_target(Ps, a.S, r, v, k);
+					rate = dms_rate_target(Ps, c.S, r, v, k);
 				const bool bad = (k >= 0) && (!dm_isfinite(rate) || rate < 0.0);
 				if(__ballot(bad) == 0)
 				{

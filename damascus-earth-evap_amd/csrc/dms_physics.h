@@ -211,9 +211,30 @@ DMS_HD double dms_rate_target(const DmsPointView& P, const DmsSolarView& S, doub
 		return 0.0;
 	const bool electron = (k == DMS_MAX_TARGETS - 1);
 	const double m_target = electron ? DMS_mElectron : S.iso_mass[k];
-	const double v_rel = dms_thermal_averaged_relative_speed(dms_temperature(S, r), m_target, v);
-	const int j = dms_interval(S, r);
-	const double n = dms_density_at(S, j, r - S.x[j], k);
+	/* dms_interval's interval of r: the guess from the uniform spacing, its
+	 * nodes and the temperature and density pieces read together (one memory
+	 * round trip), dms_interval itself when the guess is off */
+	const int nn = S.n;
+	const double t = (r - S.x1) * S.inv_h;
+	int j = !(t >= 0.0) ? 0 : ((t >= (double) nn) ? nn - 2 : (int) t + 1);
+	j = (j > nn - 2) ? nn - 2 : j;
+	double xj = S.x[j];
+	const double xn = S.x[j + 1];
+	DmsCubic ct = S.temp[j];
+	DmsCubic cd = S.dens[(size_t) j * DMS_MAX_TARGETS + k];
+	const bool ok = ((j == nn - 2) | !(xn <= r)) & ((j == 0) | !(xj > r));
+	if(__builtin_expect(!ok, 0))
+	{
+		j = dms_interval(S, r);
+		xj = S.x[j];
+		ct = S.temp[j];
+		cd = S.dens[(size_t) j * DMS_MAX_TARGETS + k];
+	}
+	const double dr = r - xj;
+	/* Solar_Model::Temperature and the number density at r */
+	const double v_rel = dms_thermal_averaged_relative_speed(1.0 * dms_cubic(ct, dr), m_target, v);
+	const double pref = electron ? 1.0 : S.iso_pref[k];
+	const double n = pref * dms_cubic(cd, dr);
 	return n * dms_sigma_target(P, S, k, v) * v_rel;
 }
 
