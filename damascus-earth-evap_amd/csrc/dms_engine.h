@@ -401,14 +401,29 @@ DMS_HD void dms_stage_pair(F&& stage, double ra, double rb, bool second, double&
 #endif
 }
 
+/* Whether the word a helper watches (an LDS word of its workgroup; none on
+ * the host) no longer holds v. */
+DMS_HD bool dms_watch_moved(const unsigned int* w, unsigned int v)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+	return (w != nullptr) && __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != v;
+#else
+	(void) w;
+	(void) v;
+	return false;
+#endif
+}
+
 /* The six stage increments of an RKF45 attempt with step h (the tableau of
  * :1503-1540).  Stage i's radius needs the velocity increments of stages
  * before i - 1 only, so stages 0 and 1, 2 and 3, 4 and 5 are evaluated as
  * pairs (dms_stage_pair): three dependent right-hand side evaluations instead
  * of six.  Every increment is the same expression of the same values as in
- * the serial order. */
+ * the serial order.  Between the pairs the word *watch is checked: when it no
+ * longer holds watch_for, the increments are abandoned (returns false). */
 template <class F>
-DMS_HD void dms_rk45_stages_paired(const DmsProp& p, double h, F&& stage, bool second, double* kr, double* kv, double* kp)
+DMS_HD bool dms_rk45_stages_paired(const DmsProp& p, double h, F&& stage, bool second, double* kr, double* kv, double* kp,
+                                   const unsigned int* watch, unsigned int watch_for)
 {
 	double dv0, dp0, dv1, dp1;
 	kr[0] = h * p.vr;
@@ -418,6 +433,8 @@ DMS_HD void dms_rk45_stages_paired(const DmsProp& p, double h, F&& stage, bool s
 	kr[1] = h * (p.vr + kv[0] / 4.0);
 	kv[1] = h * dv1;
 	kp[1] = 0.0;
+	if(dms_watch_moved(watch, watch_for))
+		return false;
 
 	const double r2 = p.radius + 3.0 / 32.0 * kr[0] + 9.0 / 32.0 * kr[1];
 	kr[2] = h * (p.vr + 3.0 / 32.0 * kv[0] + 9.0 / 32.0 * kv[1]);
@@ -428,6 +445,8 @@ DMS_HD void dms_rk45_stages_paired(const DmsProp& p, double h, F&& stage, bool s
 	kp[2] = h * dp2;
 	kv[3] = h * dv3;
 	kp[3] = h * dp3;
+	if(dms_watch_moved(watch, watch_for))
+		return false;
 
 	kr[3] = h * (p.vr + 1932.0 / 2197.0 * kv[0] - 7200.0 / 2197.0 * kv[1] + 7296.0 / 2197.0 * kv[2]);
 	const double r4 = p.radius + 439.0 / 216.0 * kr[0] - 8.0 * kr[1] + 3680.0 / 513.0 * kr[2] - 845.0 / 4104.0 * kr[3];
@@ -440,14 +459,17 @@ DMS_HD void dms_rk45_stages_paired(const DmsProp& p, double h, F&& stage, bool s
 	kr[5] = h * (p.vr - 8.0 / 27.0 * kv[0] + 2.0 * kv[1] - 3544.0 / 2565.0 * kv[2] + 1859.0 / 4104.0 * kv[3] - 11.0 / 40.0 * kv[4]);
 	kv[5] = h * dv5;
 	kp[5] = h * dp5;
+	return true;
 }
 
 /* One attempt of Runge_Kutta_45_Step (:1487-1602).  Returns 1 accepted,
  * 0 forced-accepted (retry cap or minimum step: the step "fails"), -1 retry.
  * PAIR: the stages as pairs (dms_rk45_stages_paired; on the device lanes 0
- * and 1 of the wave run the attempt together, second = lane 1). */
+ * and 1 of the wave run the attempt together, second = lane 1), abandoned
+ * with p unchanged (returns -2) when *watch moves off watch_for. */
 template <bool PAIR = false>
-DMS_HD int dms_rk45_attempt(DmsProp& p, const DmsSolarView& S, int inner_iter, bool second = false)
+DMS_HD int dms_rk45_attempt(DmsProp& p, const DmsSolarView& S, int inner_iter, bool second = false,
+                            const unsigned int* watch = nullptr, unsigned int watch_for = 0u)
 {
 	const double tol0 = 1.0 * DMS_km, tol1 = 1.0e-3 * DMS_km / DMS_sec, tol2 = 1.0e-7;
 	const double h = p.ts;
@@ -457,12 +479,16 @@ DMS_HD int dms_rk45_attempt(DmsProp& p, const DmsSolarView& S, int inner_iter, b
 	if constexpr(PAIR)
 	{
 		(void) ri;
-		dms_rk45_stages_paired(
-		    p, h, [&](double r, double& dv, double& dph) { dms_stage(p, S, r, j_ok, true, dv, dph); }, second, kr, kv, kp);
+		if(!dms_rk45_stages_paired(
+		       p, h, [&](double r, double& dv, double& dph) { dms_stage(p, S, r, j_ok, true, dv, dph); }, second, kr, kv, kp,
+		       watch, watch_for))
+			return -2;
 	}
 	else
 	{
 		(void) second;
+		(void) watch;
+		(void) watch_for;
 		double dv, dph = 0.0;
 		kr[0] = h * p.vr;
 		dms_stage(p, S, p.radius, j_ok, true, dv, dph);

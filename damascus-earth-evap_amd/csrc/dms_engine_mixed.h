@@ -61,9 +61,11 @@ DMS_HD double dms_mass_mx(const DmsSolarView& S, double r)
  * with the right-hand side J^2/r^3 - G M(r)/r^2, J/r^2 from one refined
  * reciprocal per stage.  PAIR: the stages as pairs (dms_rk45_stages_paired;
  * on the device lanes 0 and 1 of the wave run the attempt together, second =
- * lane 1). */
+ * lane 1), abandoned with p unchanged (returns -2) when *watch moves off
+ * watch_for. */
 template <bool PAIR = false>
-DMS_HD int dms_rk45_attempt_mx(DmsProp& p, const DmsSolarView& S, int inner_iter, bool second = false)
+DMS_HD int dms_rk45_attempt_mx(DmsProp& p, const DmsSolarView& S, int inner_iter, bool second = false,
+                               const unsigned int* watch = nullptr, unsigned int watch_for = 0u)
 {
 	const double tol0 = 1.0 * DMS_km, tol1 = 1.0e-3 * DMS_km / DMS_sec, tol2 = 1.0e-7;
 	const double h = p.ts;
@@ -81,10 +83,15 @@ DMS_HD int dms_rk45_attempt_mx(DmsProp& p, const DmsSolarView& S, int inner_iter
 		dphi = (angular && ri > 0.0) ? p.J * (qu * qu) : 0.0;
 	};
 	if constexpr(PAIR)
-		dms_rk45_stages_paired(p, h, rhs, second, kr, kv, kp);
+	{
+		if(!dms_rk45_stages_paired(p, h, rhs, second, kr, kv, kp, watch, watch_for))
+			return -2;
+	}
 	else
 	{
 		(void) second;
+		(void) watch;
+		(void) watch_for;
 		double dv, dp;
 		rhs(p.radius, dv, dp);
 		kr[0] = h * p.vr;

@@ -307,8 +307,8 @@ __device__ __forceinline__ void dms_lds_store(unsigned int* p, unsigned int v)
  * pairs; lane 0 writes the ring): for each session the trajectory's wave
  * opens, it prepares the following steps, assuming each post continues, and
  * writes them to the ring.  A session that changes is noticed between the
- * preparation and every RKF45 attempt.  Returns when the trajectory's wave has
- * finished all its lanes (want = 0). */
+ * preparation and every RKF45 attempt and between an attempt's stage pairs.
+ * Returns when the trajectory's wave has finished all its lanes (want = 0). */
 template <bool MX, bool MP>
 __device__ __forceinline__ void dms_pipe_serve(const DmsTrajArgs& a, const DmsSolarView& S, DmsPipeSlot* sl, int lane)
 {
@@ -381,8 +381,15 @@ __device__ __forceinline__ void dms_pipe_serve(const DmsTrajArgs& a, const DmsSo
 				}
 				A.inner_iter++;
 				A.rk_all++;
-				r = MX ? dms_rk45_attempt_mx<true>(A.prop, S, A.inner_iter, second)
-				       : dms_rk45_attempt<true>(A.prop, S, A.inner_iter, second);
+				/* an attempt is abandoned between its stage pairs once the
+				 * session has moved */
+				r = MX ? dms_rk45_attempt_mx<true>(A.prop, S, A.inner_iter, second, &sl->session, cur)
+				       : dms_rk45_attempt<true>(A.prop, S, A.inner_iter, second, &sl->session, cur);
+				if(r == -2)
+				{
+					moved = true;
+					break;
+				}
 			} while(r < 0);
 			if(moved)
 				continue;
