@@ -1437,8 +1437,18 @@ DMS_HD int dms_step_post(DmsLane& L, const DmsPointView& P, const DmsSolarView& 
 				const float a[6] = {(float) (ea_r[0] * DMS_MX_INV_RSUN), (float) (ea_r[1] * DMS_MX_INV_RSUN),
 				                    (float) (ea_r[2] * DMS_MX_INV_RSUN), (float) ea_v[0], (float) ea_v[1], (float) ea_v[2]};
 				want_next = r_after < DMS_rSun;
-				dms_rates_at_fractions_mx(P, S, b, a, frf, od_count + 1, use_cached, L.rate_before_f, rtf, want_next,
-				                          (float) (r_after * DMS_MX_INV_RSUN), (float) v_after, &rate_next_f);
+				if(pre_rates)
+				{
+					/* FP32 rates evaluated beforehand, held as doubles (exact) */
+#pragma unroll
+					for(int i = 0; i <= DMS_MAX_OPTICAL_DEPTH_PIECES; i++)
+						rtf[i] = (float) pre_rates[i];
+					if(want_next)
+						rate_next_f = (float) pre_rates[DMS_MAX_OPTICAL_DEPTH_PIECES + 1];
+				}
+				else
+					dms_rates_at_fractions_mx(P, S, b, a, frf, od_count + 1, use_cached, L.rate_before_f, rtf, want_next,
+					                          (float) (r_after * DMS_MX_INV_RSUN), (float) v_after, &rate_next_f);
 #pragma unroll
 				for(int i = 0; i <= DMS_MAX_OPTICAL_DEPTH_PIECES; i++)
 					if(i <= od_count && (!mx_isfinite(rtf[i]) || rtf[i] < 0.0f))
@@ -2377,9 +2387,9 @@ struct DmsPipeEntry
 	/* the step's start: the event and radius the post takes as eb and r_before */
 	DmsStepEvent eb;
 	double r_before;
-	/* FP64 only: the post's rates at the optical-depth fractions and the end
-	 * point (dms_rate_fraction_slot), when have_rates, filled by a second
-	 * helper wave */
+	/* the post's rates at the optical-depth fractions and the end point
+	 * (dms_rate_fraction_slot; mixed precision: dms_rate_fraction_slot_mx,
+	 * widened), when have_rates, filled by a second helper wave */
 	int32_t have_rates;
 	int32_t pad;
 	double rt[DMS_MAX_OPTICAL_DEPTH_PIECES + 2];
@@ -2550,6 +2560,57 @@ DMS_HD bool dms_pipe_rate_inputs(const DmsPipeEntry& e, const DmsJobView& J, Dms
 	q.use_cached = (e.r_before < DMS_rSun) & (is <= 1.0e-12);
 	q.want_next = q.r_after < DMS_rSun;
 	return true;
+}
+
+/* The mixed-precision post's rate-query inputs for the step of entry e (its
+ * own steps up to dms_rates_at_fractions_mx: interior interval, FP32
+ * fractions and chord end points, cached start rate, end-point query).
+ * Returns false when the post makes no rate query for the step. */
+struct DmsPipeRateInputsMx
+{
+	float fr[DMS_MAX_OPTICAL_DEPTH_PIECES + 1];
+	float b[6], a[6];
+	float xr, xv;
+	int n;
+	bool use_cached, want_next;
+};
+
+DMS_HD bool dms_pipe_rate_inputs_mx(const DmsPipeEntry& e, const DmsJobView& J, DmsPipeRateInputsMx& q)
+{
+	const double r_after = (e.radius < 0.0) ? 0.0 : e.radius; /* dms_prop_radius */
+	const double actual_dt = e.time - e.pb_time;
+	double is = 0.0, ie = 0.0;
+	if(!dms_interior_interval(e.eb.r, e.ev.r, e.r_before, r_after, is, ie))
+		return false;
+	if(!(ie > is && actual_dt > 0.0))
+		return false;
+	const int fast = J.terminate_on_capture;
+	const int od_count = fast ? DMS_CAPTURE_MODE_OPTICAL_DEPTH_PIECES : DMS_MAX_OPTICAL_DEPTH_PIECES;
+	const double inv_pieces = fast ? 1.0 / DMS_CAPTURE_MODE_OPTICAL_DEPTH_PIECES : 1.0 / DMS_MAX_OPTICAL_DEPTH_PIECES;
+	const float isf = (float) is, span = (float) (ie - is);
+	for(int i = 0; i <= DMS_MAX_OPTICAL_DEPTH_PIECES; i++)
+		q.fr[i] = (i <= od_count) ? isf + (float) i * (float) inv_pieces * span : 0.0f;
+	for(int c = 0; c < 3; c++)
+	{
+		q.b[c] = (float) (e.eb.r[c] * DMS_MX_INV_RSUN);
+		q.b[3 + c] = (float) e.eb.v[c];
+		q.a[c] = (float) (e.ev.r[c] * DMS_MX_INV_RSUN);
+		q.a[3 + c] = (float) e.ev.v[c];
+	}
+	q.xr = (float) (r_after * DMS_MX_INV_RSUN);
+	q.xv = (float) e.ev.speed;
+	q.n = od_count + 1;
+	q.use_cached = e.r_before < DMS_rSun && is <= 1.0e-12;
+	q.want_next = r_after < DMS_rSun;
+	return true;
+}
+
+/* Query k of entry e in the mixed-precision kernel, widened to double. */
+DMS_HD double dms_pipe_rate_mx(const DmsPipeEntry& e, const DmsPipeRateInputsMx& q, const DmsPointView& P, const DmsSolarView& S,
+                               int k)
+{
+	return (double) dms_rate_fraction_slot_mx(P, S, k, q.b, q.a, q.fr, q.n, q.use_cached, e.rate_before_f, q.want_next, q.xr,
+	                                          q.xv);
 }
 
 /* Query k of entry e (inputs from dms_pipe_rate_inputs). */

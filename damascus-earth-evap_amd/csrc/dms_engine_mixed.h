@@ -300,6 +300,32 @@ DMS_HD float dms_rate_slot_bilinear_mx(const DmsRateSlotMx& q, const float* g)
  * end point (xr, xv), FP32.  Elements 1..n-1 and the end-point query take
  * one batch of table reads; element 0, the step's start point, is evaluated
  * only when its cached rate is not available (use_cached). */
+/* Query k of dms_rates_at_fractions_mx alone (k < NQ - 1: fraction k, k ==
+ * NQ - 1: the extra query), the same evaluation: the lanes of a helper wave
+ * evaluate one query each. */
+DMS_HD float dms_rate_fraction_slot_mx(const DmsPointView& P, const DmsSolarView& S, int k, const float* b, const float* a,
+                                       const float* fr, int n, bool use_cached, float cached_rate, bool extra, float xr,
+                                       float xv)
+{
+	constexpr int NQ = DMS_MAX_OPTICAL_DEPTH_PIECES + 2;
+	if((k == 0) & use_cached)
+		return cached_rate;
+	const bool use_grid = P.use_grid != 0;
+	const bool is_extra = (k == NQ - 1);
+	const float d[6] = {a[0] - b[0], a[1] - b[1], a[2] - b[2], a[3] - b[3], a[4] - b[4], a[5] - b[5]};
+	DmsRateSlotMx q = dms_rate_slot_mx(P, use_grid, is_extra ? extra : (k < n), !is_extra, b, d, fr[is_extra ? 0 : k], xr, xv);
+	if(use_grid)
+	{
+		const unsigned nv = (unsigned) P.Nv;
+		const float* t = P.grid32 + q.lo;
+		const float g[4] = {t[0], t[1], t[nv], t[nv + 1]};
+		q.res = dms_rate_slot_bilinear_mx(q, g);
+	}
+	if(__builtin_expect(q.comp, 0))
+		q.res = dms_rate_computed_mx(P, S, (double) q.rad * DMS_rSun, (double) q.spd);
+	return q.res;
+}
+
 DMS_HD void dms_rates_at_fractions_mx(const DmsPointView& P, const DmsSolarView& S, const float* b, const float* a,
                                       const float* fr, int n, bool use_cached, float cached_rate, float* out, bool extra,
                                       float xr, float xv, float* xout)

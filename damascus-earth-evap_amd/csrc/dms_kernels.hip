@@ -419,14 +419,15 @@ __device__ __forceinline__ void dms_pipe_serve(const DmsTrajArgs& a, const DmsSo
 	}
 }
 
-/* A second helper wave serving slot sl (FP64 kernels): for every entry the
- * first helper writes, in order, the FP64 post's rate queries at the
- * optical-depth fractions and the end point, one query per lane
- * (dms_pipe_rate: the same evaluations dms_rates_at_fractions makes one after
- * the other); then rhead passes the entry.  Entries of any session are
- * evaluated (a stale one costs a little time, nothing else).  Returns when the
- * trajectory's wave has finished all its lanes. */
-template <bool MP>
+/* A second helper wave serving slot sl: for every entry the first helper
+ * writes, in order, the post's rate queries at the optical-depth fractions and
+ * the end point, one query per lane (dms_pipe_rate: the same evaluations
+ * dms_rates_at_fractions makes one after the other; mixed precision
+ * dms_pipe_rate_mx, the FP32 rates widened); then rhead passes the entry.
+ * Entries of any session are evaluated (a stale one costs a little time,
+ * nothing else).  Returns when the trajectory's wave has finished all its
+ * lanes. */
+template <bool MX, bool MP>
 __device__ __forceinline__ void dms_pipe_rates(const DmsTrajArgs& a, const DmsSolarView& S, DmsPipeSlot* sl, int lane)
 {
 	constexpr int NQ = DMS_MAX_OPTICAL_DEPTH_PIECES + 2;
@@ -447,16 +448,24 @@ __device__ __forceinline__ void dms_pipe_rates(const DmsTrajArgs& a, const DmsSo
 		bool have = false;
 		if(__builtin_amdgcn_readfirstlane(ep->kind) == DMS_PIPE_STEP)
 		{
-			DmsPipeRateInputs q;
-			have = dms_pipe_rate_inputs(*ep, a.J, q);
-			if(have & (lane < NQ))
+			DmsPointView P;
+			if constexpr(MP)
+				P = a.job_points[ep->job];
+			else
+				P = a.P;
+			if constexpr(MX)
 			{
-				DmsPointView P;
-				if constexpr(MP)
-					P = a.job_points[ep->job];
-				else
-					P = a.P;
-				ep->rt[lane] = dms_pipe_rate(*ep, q, P, S, lane);
+				DmsPipeRateInputsMx q;
+				have = dms_pipe_rate_inputs_mx(*ep, a.J, q);
+				if(have & (lane < NQ))
+					ep->rt[lane] = dms_pipe_rate_mx(*ep, q, P, S, lane);
+			}
+			else
+			{
+				DmsPipeRateInputs q;
+				have = dms_pipe_rate_inputs(*ep, a.J, q);
+				if(have & (lane < NQ))
+					ep->rt[lane] = dms_pipe_rate(*ep, q, P, S, lane);
 			}
 		}
 		if(lane == 0)
@@ -478,7 +487,7 @@ __device__ __forceinline__ void dms_pipe_idle(const DmsTrajArgs& a, const DmsSol
 	{
 		if(dms_lds_load(&pb->waves_done) >= (unsigned int) nwaves)
 			return;
-		/* first helper of a waiting wave, or (FP64) its second helper */
+		/* first or second helper of a waiting wave */
 		int claimed = -1, role = 0;
 		if(lane == 0)
 			for(int w = 0; w < nwaves && claimed < 0; w++)
@@ -486,7 +495,7 @@ __device__ __forceinline__ void dms_pipe_idle(const DmsTrajArgs& a, const DmsSol
 				DmsPipeSlot* sl = &pb->slot[w];
 				if(w == wave || dms_lds_load(&sl->want) == 0u)
 					continue;
-				for(int r = 0; r < (MX ? 1 : 2); r++)
+				for(int r = 0; r < 2; r++)
 				{
 					unsigned int* word = r ? &sl->helper2 : &sl->helper;
 					unsigned int expect = 0u;
@@ -506,8 +515,8 @@ __device__ __forceinline__ void dms_pipe_idle(const DmsTrajArgs& a, const DmsSol
 		{
 			if(role == 0)
 				dms_pipe_serve<MX, MP>(a, S, &pb->slot[claimed], lane);
-			else if constexpr(!MX)
-				dms_pipe_rates<MP>(a, S, &pb->slot[claimed], lane);
+			else
+				dms_pipe_rates<MX, MP>(a, S, &pb->slot[claimed], lane);
 		}
 		else
 			__builtin_amdgcn_s_sleep(8);
@@ -921,23 +930,20 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 						}
 						held = true;
 						timed_out = false;
-						/* FP64: the entry's rate queries from a second helper,
-						 * once its progress word has passed the entry */
+						/* the entry's rate queries from a second helper, once
+						 * its progress word has passed the entry */
 						held_rates = false;
-						if constexpr(!MX)
-						{
-							if(__builtin_amdgcn_readfirstlane(dms_lds_load(&my->helper2)) != 0u)
-								for(unsigned int rp = 0; rp < DMS_PIPE_RATE_POLLS; rp++)
+						if(__builtin_amdgcn_readfirstlane(dms_lds_load(&my->helper2)) != 0u)
+							for(unsigned int rp = 0; rp < DMS_PIPE_RATE_POLLS; rp++)
+							{
+								const unsigned int rh = __builtin_amdgcn_readfirstlane(dms_lds_load(&my->rhead));
+								if((int) (rh - tail) > 0)
 								{
-									const unsigned int rh = __builtin_amdgcn_readfirstlane(dms_lds_load(&my->rhead));
-									if((int) (rh - tail) > 0)
-									{
-										held_rates = __builtin_amdgcn_readfirstlane(ep->have_rates) != 0;
-										break;
-									}
-									__builtin_amdgcn_s_sleep(1);
+									held_rates = __builtin_amdgcn_readfirstlane(ep->have_rates) != 0;
+									break;
 								}
-						}
+								__builtin_amdgcn_s_sleep(1);
+							}
 						break;
 					}
 					tail++;

@@ -21,10 +21,9 @@
  * trajectory's wave is about to run) starts from seed; the helper writes the
  * steps it prepares to the ring e[] (head, written by the helper) and the
  * trajectory's wave reads them (tail); entries of an older session are
- * skipped.  In the FP64 kernels a second finished wave may claim helper2: it
- * evaluates each entry's rate queries (one per lane) behind the first helper
- * (rhead), and the trajectory's post takes them when rhead has passed the
- * entry. */
+ * skipped.  A second finished wave may claim helper2: it evaluates each
+ * entry's rate queries (one per lane) behind the first helper (rhead), and the
+ * trajectory's post takes them when rhead has passed the entry. */
 #define DMS_PIPE_DEPTH 2
 
 struct DmsPipeSlot
@@ -34,7 +33,7 @@ struct DmsPipeSlot
 	unsigned int session; /* current session, 0 = none */
 	unsigned int head;    /* entries written (helper) */
 	unsigned int tail;    /* entries read (trajectory's wave) */
-	unsigned int helper2; /* FP64: second helper wave + 1 (the entries' rate queries), 0 = none */
+	unsigned int helper2; /* second helper wave + 1 (the entries' rate queries), 0 = none */
 	unsigned int rhead;   /* entries whose rate queries the second helper has evaluated */
 	unsigned int pad;
 	DmsPipeSeed seed;

@@ -454,15 +454,28 @@ static int pipe_session(DmsLane& L, const DmsLaneCtx& c, int depth, uint64_t* ta
 		}
 		dms_pipe_take(L, held);
 		pre = &held.ev;
-		/* FP64: the step's rate queries evaluated from the entry alone, as the
+		/* the step's rate queries evaluated from the entry alone, as the
 		 * kernel's second helper wave does (one query per lane there) */
 		pre_rates = nullptr;
-		DmsPipeRateInputs q;
-		if(!MX && dms_pipe_rate_inputs(held, c.J, q))
+		if constexpr(MX)
 		{
-			for(int k = 0; k < DMS_MAX_OPTICAL_DEPTH_PIECES + 2; k++)
-				held.rt[k] = dms_pipe_rate(held, q, c.P, c.S, k);
-			pre_rates = held.rt;
+			DmsPipeRateInputsMx q;
+			if(dms_pipe_rate_inputs_mx(held, c.J, q))
+			{
+				for(int k = 0; k < DMS_MAX_OPTICAL_DEPTH_PIECES + 2; k++)
+					held.rt[k] = dms_pipe_rate_mx(held, q, c.P, c.S, k);
+				pre_rates = held.rt;
+			}
+		}
+		else
+		{
+			DmsPipeRateInputs q;
+			if(dms_pipe_rate_inputs(held, c.J, q))
+			{
+				for(int k = 0; k < DMS_MAX_OPTICAL_DEPTH_PIECES + 2; k++)
+					held.rt[k] = dms_pipe_rate(held, q, c.P, c.S, k);
+				pre_rates = held.rt;
+			}
 		}
 		(*taken)++;
 	}
