@@ -1010,15 +1010,23 @@ __global__ void __launch_bounds__(DMS_TRAJ_BLOCK) dms_traj_kernel_serve(DmsTrajA
 
 __global__ void __launch_bounds__(DMS_TRAJ_BLOCK) dms_traj_kernel_mixed_serve(DmsTrajArgs a) { dms_traj_body<true, true, true>(a); }
 
+/* The kernels below run beside resident trajectory launches (a point's rate
+ * table and initial conditions are built while another job's or the serving
+ * launch's persistent kernel holds every compute unit).  That kernel leaves
+ * 80 of a SIMD's 512 registers per lane free (256 VGPRs + its accumulation
+ * registers), so a wave of these kernels fits beside it only with at most 80;
+ * eight waves per SIMD caps them at 64. */
+#define DMS_BESIDE_RESIDENT __attribute__((amdgpu_waves_per_eu(8)))
+
 /* FP32 copy of the rate table in 1/s for the mixed-precision kernel */
-__global__ void dms_grid_f32_kernel(const double* grid, size_t n, float* out)
+__global__ void DMS_BESIDE_RESIDENT dms_grid_f32_kernel(const double* grid, size_t n, float* out)
 {
 	const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
 	if(i < n)
 		out[i] = (float) (grid[i] * DMS_sec);
 }
 
-__global__ void dms_ic_kernel(DmsHaloView H, uint32_t seed, uint32_t point_id, uint64_t first_id, uint32_t n, double R_final,
+__global__ void DMS_BESIDE_RESIDENT dms_ic_kernel(DmsHaloView H, uint32_t seed, uint32_t point_id, uint64_t first_id, uint32_t n, double R_final,
                               dms_event* out, int32_t* ok)
 {
 	const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1035,7 +1043,7 @@ __global__ void dms_ic_kernel(DmsHaloView H, uint32_t seed, uint32_t point_id, u
 	ok[i] = s;
 }
 
-__global__ void dms_grid_kernel(DmsPointView P, DmsSolarView S, int Nr, int Nv, double vmax, double* grid)
+__global__ void DMS_BESIDE_RESIDENT dms_grid_kernel(DmsPointView P, DmsSolarView S, int Nr, int Nv, double vmax, double* grid)
 {
 	const size_t idx = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
 	if(idx >= (size_t) Nr * (size_t) Nv)
@@ -1048,7 +1056,7 @@ __global__ void dms_grid_kernel(DmsPointView P, DmsSolarView S, int Nr, int Nv, 
 	grid[idx] = dms_rate_computed(P, S, radius, speed);
 }
 
-__global__ void dms_rates_kernel(DmsPointView P, DmsSolarView S, int which, const double* r, const double* v, double* out, size_t n)
+__global__ void DMS_BESIDE_RESIDENT dms_rates_kernel(DmsPointView P, DmsSolarView S, int which, const double* r, const double* v, double* out, size_t n)
 {
 	const size_t i = (size_t) blockIdx.x * blockDim.x + threadIdx.x;
 	if(i >= n)
@@ -1068,7 +1076,7 @@ __global__ void dms_rates_kernel(DmsPointView P, DmsSolarView S, int which, cons
 /* Partial per-bin sums over trajectories [blk*TB, min(n, (blk+1)*TB)) in id order.
  * cls: 1 captured, 2 completed outward escape, 0 not counted.
  * partial layout: [blk][nbins][8] doubles, pcount: [blk][nbins][2]. */
-__global__ void dms_hist_partial_kernel(const double* rows, const uint32_t* counts, const uint8_t* cls, uint32_t n, int nbins,
+__global__ void DMS_BESIDE_RESIDENT dms_hist_partial_kernel(const double* rows, const uint32_t* counts, const uint8_t* cls, uint32_t n, int nbins,
                                         uint32_t tb, double* partial, unsigned long long* pcount)
 {
 	const int b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1102,7 +1110,7 @@ __global__ void dms_hist_partial_kernel(const double* rows, const uint32_t* coun
 }
 
 /* hist[q][b] += sum over blocks (in block order); counts[k][b] likewise. */
-__global__ void dms_hist_final_kernel(const double* partial, const unsigned long long* pcount, uint32_t nblk, int nbins,
+__global__ void DMS_BESIDE_RESIDENT dms_hist_final_kernel(const double* partial, const unsigned long long* pcount, uint32_t nblk, int nbins,
                                       double* hist, unsigned long long* counts)
 {
 	const int b = blockIdx.x * blockDim.x + threadIdx.x;

@@ -376,7 +376,7 @@ int dms_get_launch_knobs(const dms_ctx* ctx, dms_launch_knobs* knobs);
  * (same jobs, same caps) run once per candidate refill_batch, `repeats` times
  * each in turn; seconds[i] receives candidate i's fastest kernel time.  The
  * first candidate is the reference: another is chosen (*best) only when it is
- * more than 3% faster.  With apply != 0 the context takes *best.  Submit no
+ * more than 10% faster.  With apply != 0 the context takes *best.  Submit no
  * other jobs on the context meanwhile (the knobs are context-wide).  No
  * reference counterpart. */
 int dms_autotune_knobs(dms_point* const* points, const dms_job_desc* descs, size_t n, uint64_t calibration_trajectories,

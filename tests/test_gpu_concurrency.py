@@ -96,9 +96,9 @@ def test_refill_batch_knob_keeps_results_and_autotunes(gpu_ctx, multi_points):
         best, secs = dmsun.autotune_knobs(multi_points, jobs, refill_candidates=(1, 8, 32), repeats=2)
         print(f"refill_batch candidates (1, 8, 32): {np.round(secs, 4).tolist()} s, best {best}", flush=True)
         assert best in (1, 8, 32) and np.all(secs > 0) and gpu_ctx.refill_batch() == best
-        # the first candidate stays unless another is more than 3% faster
+        # the first candidate stays unless another is more than 10% faster
         k = [1, 8, 32].index(best)
-        assert (k == 0 and np.all(secs[1:] >= 0.97 * secs[0])) or (secs[k] < 0.97 * secs[0] and secs[k] == secs.min())
+        assert (k == 0 and np.all(secs[1:] >= 0.90 * secs[0])) or (secs[k] < 0.90 * secs[0] and secs[k] == secs.min())
     finally:
         gpu_ctx.set_refill_batch(1)
     for a, b in zip(base, batched):

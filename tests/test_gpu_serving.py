@@ -138,6 +138,36 @@ def test_requests_join_the_running_launch(gpu_ctx, precision):
         assert_job_equal(r, pt.run_job(job, want_records=False))
 
 
+def test_point_created_beside_the_running_launch(gpu_ctx):
+    """A point created while the serving launch runs has its rate table built
+    beside the launch (the table kernel fits in the registers the resident
+    launch leaves free on each SIMD), so it is ready before the running request
+    ends and a request for it joins the same launch."""
+    pt = gpu_ctx.point(dmsun.make_point(0.3, 1e-36, interpolation_points=300))
+    long_job = dmsun.make_job(mode=dmsun.MODE_PARAMETER_POINT, max_trajectories=300000, capture_target=0,
+                              max_scatterings=MAX_SCAT, rk_attempt_budget=RK, n_bins=BINS, record=0)
+    srv = dmsun.Server(gpu_ctx, launch_desc())
+    try:
+        first = srv.submit(pt, long_job)
+        t0 = time.time()
+        while srv.poll(first).trajectories_done == 0 and time.time() - t0 < 30:
+            time.sleep(0.002)
+        t1 = time.time()
+        pdesc2, job2 = draw_requests(1, 606, 100, 101)[0]
+        pt2 = gpu_ctx.point(pdesc2)
+        built = time.time() - t1
+        during = srv.stats()
+        second = srv.submit(pt2, job2)
+        res = [srv.collect(first), srv.collect(second)]
+        st = srv.stats()
+    finally:
+        srv.close()
+    print(f"point built in {built * 1e3:.1f} ms while the launch ran: {during}; after {st}", flush=True)
+    assert during["running"] == 1 and during["requests_done"] == 0
+    assert st["launches"] == 1 and st["requests_done"] == 2
+    assert_job_equal(res[1], pt2.run_job(job2, want_records=False))
+
+
 def test_idle_launch_ends_and_restarts(gpu_ctx):
     pdesc, job = draw_requests(1, 404, 100, 101)[0]
     pt = gpu_ctx.point(pdesc)
