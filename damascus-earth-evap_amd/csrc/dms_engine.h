@@ -2507,8 +2507,10 @@ DMS_HD void dms_pipe_fill_entry(const DmsLane& A, int r, DmsPipeEntry& e)
 
 /* Helper side: the next step's preparation and RKF45 attempts (as
  * dms_lane_advance_b and dms_lane_advance_a run them, the attempts with paired
- * stages as the helper wave runs them) into e.  Returns false
- * when the preparation ends the segment (e.kind DMS_PIPE_PREP_END). */
+ * stages as the helper wave runs them) into e.  Returns false when the
+ * preparation ends the segment (e.kind DMS_PIPE_PREP_END).  This is the host
+ * replay's helper: on the device the paired attempts need lanes 0 and 1 of a
+ * wave running them together, as the helper wave in dms_kernels.hip does. */
 template <bool MX = false>
 DMS_HD bool dms_pipe_prepare_step(DmsLane& A, const DmsPointView& P, const DmsSolarView& S, const DmsJobView& J, DmsPipeEntry& e)
 {
