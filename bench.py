@@ -103,7 +103,11 @@ def cpu_baseline(sample, threads):
     return dict(value=c.trajectories / dt, unit="trajectories/s", cores=threads, kind="port",
                 host=f"{model}; {os.cpu_count()} logical CPUs visible",
                 sample=f"{sample} trajectories of the same workload (ids 0..{sample - 1}), "
-                       f"{threads} threads, {dt:.2f} s; scatterings/s {c.total_scatterings / dt:.4g}")
+                       f"{threads} threads, {dt:.2f} s; scatterings/s {c.total_scatterings / dt:.4g}",
+                longest_trajectory_rk_attempts=int(c.max_rk_attempts),
+                note="the sample's longest trajectory needs fewer RKF45 attempts than the longest of a 1e6 job "
+                     "(single_job) or of the timed launch (tail), so trajectories/s here and on the GPU are not "
+                     "like-for-like; the roofline fraction measures the kernel")
 
 
 def measure(dmsun, pt, precision, args):
@@ -119,7 +123,9 @@ def measure(dmsun, pt, precision, args):
         single = dict(trajectories_per_s=res["counters"].trajectories / dt, seconds=dt, kernel_ms=res["kernel_ms"],
                       trajectories=res["counters"].trajectories,
                       longest_trajectory_rk_attempts=res["counters"].max_rk_attempts,
-                      us_per_attempt_of_longest=1e3 * res["kernel_ms"] / max(1, res["counters"].max_rk_attempts))
+                      us_per_attempt_of_longest=1e3 * res["kernel_ms"] / max(1, res["counters"].max_rk_attempts),
+                      note="bounded by its longest trajectory; cpu_baseline's sample has a shorter tail, so the "
+                           "ratio of the two is not like-for-like")
         print(f"{name} warmup {w} (one job alone): {dt:.2f} s, kernel {res['kernel_ms']:.1f} ms", file=sys.stderr,
               flush=True)
     jobs = [make_workload(dmsun, args.trajectories, k * args.trajectories, precision)[1] for k in range(args.steps)]
