@@ -700,8 +700,13 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 					const bool bad = (k >= 0) && (!mx_isfinite(rate) || rate < 0.0f);
 					if(__ballot(bad) == 0)
 					{
+						/* lanes n_iso..62 hold +0 (no target): the fixed trip
+						 * count adds them without changing the sum, and the
+						 * constant lane indices let the readlanes run ahead of
+						 * the dependent adds */
 						float total = readlane_f(rate, DMS_MAX_TARGETS - 1);
-						for(int i = 0; i < n_iso; i++)
+#pragma unroll
+						for(int i = 0; i < DMS_MAX_TARGETS - 1; i++)
 							total += readlane_f(rate, i);
 						if(mx_isfinite(total) && total > 0.0f)
 						{
@@ -733,9 +738,15 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 				if(__ballot(bad) == 0)
 				{
 					/* same summation and comparison order as dms_choose_target */
+					/* lanes n_iso..62 hold +0 (no target): adding them changes
+					 * no bit of a sum of non-negative rates (a -0 start only
+					 * turns into +0, which fails the test below as -0 does),
+					 * and the fixed, unrolled trip count lets the readlanes
+					 * run ahead of the dependent adds */
 					const double re = readlane_d(rate, DMS_MAX_TARGETS - 1);
 					double total = re;
-					for(int i = 0; i < n_iso; i++)
+#pragma unroll
+					for(int i = 0; i < DMS_MAX_TARGETS - 1; i++)
 						total = total + readlane_d(rate, i);
 					if(dm_isfinite(total) && total > 0.0)
 					{
