@@ -295,12 +295,13 @@ __device__ __forceinline__ void dms_lds_store(unsigned int* p, unsigned int v)
 	__hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-/* Spin limits (polls of at least 64 cycles each): a trajectory's wave that
- * waits this long for a helper's step gives the split up and goes on alone. */
-#define DMS_PIPE_WAIT_POLLS (1u << 22)
+/* Spin limits (each poll one LDS round trip, no sleep: the trajectory's waits
+ * below sit on its critical path): a trajectory's wave that waits this long
+ * for a helper's step gives the split up and goes on alone. */
+#define DMS_PIPE_WAIT_POLLS (1u << 23)
 /* polls for the second helper's rates of an entry before the post computes
  * its own */
-#define DMS_PIPE_RATE_POLLS 256u
+#define DMS_PIPE_RATE_POLLS 512u
 
 /* A helper wave serving slot sl (its lanes 0 and 1 compute, with the same
  * values except inside the RKF45 attempts, whose stages they evaluate in
@@ -406,7 +407,12 @@ __device__ __forceinline__ void dms_pipe_serve(const DmsTrajArgs& a, const DmsSo
 				stale = true;
 				break;
 			}
-			__builtin_amdgcn_s_sleep(1);
+			/* the helpers' idle polls sleep in the FP64 kernel and spin in the
+			 * mixed one: measured on lone trajectories, FP64 -5% per attempt
+			 * with the trajectory's wave alone spinning, mixed -3% more with
+			 * all three waves spinning */
+			if constexpr(!MX)
+				__builtin_amdgcn_s_sleep(1);
 		}
 		if(stale || dms_lds_load(&sl->session) != cur)
 			continue;
@@ -441,7 +447,9 @@ __device__ __forceinline__ void dms_pipe_rates(const DmsTrajArgs& a, const DmsSo
 			return;
 		if(__builtin_amdgcn_readfirstlane(dms_lds_load(&sl->head)) == b)
 		{
-			__builtin_amdgcn_s_sleep(1);
+			/* as the first helper's idle polls */
+			if constexpr(!MX)
+				__builtin_amdgcn_s_sleep(1);
 			continue;
 		}
 		DmsPipeEntry* ep = &sl->e[b % DMS_PIPE_DEPTH];
@@ -922,10 +930,7 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 				{
 					const unsigned int h = __builtin_amdgcn_readfirstlane(dms_lds_load(&my->head));
 					if(h == tail)
-					{
-						__builtin_amdgcn_s_sleep(1);
 						continue;
-					}
 					const DmsPipeEntry* ep = &my->e[tail % DMS_PIPE_DEPTH];
 					const unsigned int es = __builtin_amdgcn_readfirstlane(ep->session);
 					const int kind = __builtin_amdgcn_readfirstlane(ep->kind);
@@ -953,7 +958,6 @@ __device__ __forceinline__ void dms_traj_body(const DmsTrajArgs& a)
 									held_rates = __builtin_amdgcn_readfirstlane(ep->have_rates) != 0;
 									break;
 								}
-								__builtin_amdgcn_s_sleep(1);
 							}
 						break;
 					}
