@@ -62,8 +62,8 @@ PEAK_FP64_TFLOPS = 78.6
 # wave instructions weighted by operations (ADD + MUL + TRANS + 2 FMA); see
 # profiles/dms_traj_kernel_parameter_point_m0.3_20x1e6.md.  Other step counts
 # are other launches: their traffic is reported as null (unmeasured).
-PROFILED_LAUNCH = {"steps": 20, "trajectories": 1_000_000, "hbm_bytes": 2.0 * 321_871_926 * 1024 + 477_613_431 * 1024,
-                   "f64_op_weighted_wave_instructions": 4.1135e11}
+PROFILED_LAUNCH = {"steps": 20, "trajectories": 1_000_000, "hbm_bytes": 2.0 * 321_819_444 * 1024 + 477_599_739 * 1024,
+                   "f64_op_weighted_wave_instructions": 4.1058e11}
 
 
 def algorithmic_flops(c, capture_mode):
